@@ -1,0 +1,156 @@
+"""Benchmark: DOF-stage updates per second of the SSP-RK3 limited-ESDG step on one MI355X.
+
+Workload (default): the 2D isentropic vortex on periodic [-5,5]^2, 64x64 quads, N = 3
+(16 nodes per element, 65,536 DOFs), FP64, the reference's limiter (SubcellLimiter
+with PositivityBound).  A step is one SSP-RK3 step (three limited RHS evaluations and
+stage updates) of the whole mesh; the state, time and dt stay in HBM for the timed region.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config vortex_2d]
+
+Prints one JSON line.  ``value`` = DOFs x 3 stages x K steps / wall seconds.
+``roofline`` is for the dominant kernel (the element RHS, phase A): algorithmic bytes =
+96 B per DOF-stage (read U^(s) and U^n, write U^(s+1), FP64, 2D; 72 B in 1D) x the DOFs one
+launch processes, over its average duration from HIP events on the library's stream.
+``cpu_baseline`` times the CPU oracle (a restatement of the reference loops, OpenMP over
+elements) on the host cores on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "DOF-stage updates/s, 2D Euler N=3: mixed/batched vs FP64 at matched L1 error"
+
+WORKLOADS = {
+    "vortex_2d": dict(kw=dict(), desc="2D isentropic vortex, periodic [-5,5]^2, 64x64 quads, N=3, FP64"),
+    "riemann4_2d": dict(kw=dict(), desc="2D four-state Riemann, [0,1]^2, 256x256 quads, N=3, FP64"),
+    "kelvin_helmholtz": dict(kw=dict(), desc="2D Kelvin-Helmholtz, periodic [-1,1]^2, 512x512 quads, N=4, FP64"),
+    "sod_1d": dict(kw=dict(), desc="1D Sod tube, K=100, N=3, FP64"),
+}
+
+
+def bytes_per_dof_stage(Nd: int) -> int:
+    return 3 * (Nd + 2) * 8
+
+
+def cpu_baseline(pkg, name, kw, min_seconds=6.0):
+    """Time the oracle on the host: warm-up step, then steps until min_seconds elapse."""
+    from oracle import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    O.set_threads(threads)
+    solver, state, sp = pkg.configs.build(name, **kw)
+    fp = pkg.flat_problem(solver, sp)
+    orc = O.Oracle(fp)
+    U = state.preallocation.Uq.copy()
+    t = 0.0
+    t, _ = orc.step(U, t)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        t, _ = orc.step(U, t)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= min_seconds or steps >= 200:
+            break
+    dofs = fp.K * fp.Nq
+    return {"value": dofs * 3 * steps / el, "unit": "DOF-stage updates/s", "cores": O.get_threads(),
+            "kind": "port",
+            "sample": f"{name} {fp.K} elements N={fp.N1D - 1}: {steps} SSP-RK3 steps after 1 warm-up step, "
+                      f"{el:.2f} s wall on {O.get_threads()} OpenMP threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", default="vortex_2d", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU decomposition is described in DESIGN.md, not run.", file=sys.stderr)
+        sys.exit(2)
+
+    import __graft_entry__ as G
+    pkg = G.load_package()
+    from p2de_jl_amd.solver import DeviceSolver
+
+    wl = WORKLOADS[args.config]
+    solver, state, sp = pkg.configs.build(args.config, **wl["kw"])
+    fp = pkg.flat_problem(solver, sp)
+    dev = DeviceSolver(fp)
+    U0 = state.preallocation.Uq
+    dofs = fp.K * fp.Nq
+
+    # warm-up (untimed)
+    dev.set_state(U0)
+    dev.set_time(0.0)
+    dev.enqueue_steps(args.warmup)
+    dev.synchronize()
+
+    # timed region: K steps, state resident in HBM
+    dev.set_state(U0)
+    dev.set_time(0.0)
+    dev.synchronize()
+    t0 = time.perf_counter()
+    dev.enqueue_steps(args.steps)
+    dev.synchronize()
+    wall = time.perf_counter() - t0
+    t_end, dt_last = dev.get_time()
+    rep = dev.check_positivity()
+
+    # per-kernel durations (HIP events on the library's stream), separate pass
+    dev.set_state(U0)
+    dev.set_time(0.0)
+    dev.set_kernel_timing(True)
+    dev.enqueue_steps(min(args.steps, 50))
+    ms, cnt = dev.kernel_timing()
+    dev.set_kernel_timing(False)
+    avgA = ms[0] / max(cnt[0], 1) * 1e-3
+    avgB = ms[1] / max(cnt[1], 1) * 1e-3
+    bpd = bytes_per_dof_stage(fp.Nd)
+    achieved = bpd * dofs / avgA / 1e9
+
+    value = dofs * 3 * args.steps / wall
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "DOF-stage updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (analytic initial condition of the named problem)",
+        "config": {"workload": wl["desc"], "name": args.config, "elements": fp.K, "nodes_per_element": fp.Nq,
+                   "dofs": dofs, "precision": "fp64", "limiter": "SubcellLimiter(PositivityBound), reference bounds",
+                   "t_end": t_end, "last_dt": dt_last, "positivity_violations": int(rep.violations),
+                   "min_rho": rep.min_rho, "min_p": rep.min_p},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "phaseA_kernel (element RHS + limiter)",
+                     "avg_launch_us": avgA * 1e6, "algorithmic_bytes_per_launch": bpd * dofs,
+                     "phaseB_avg_launch_us": avgB * 1e6},
+    }
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(pkg, args.config, wl["kw"])
+        except Exception as e:  # the baseline is reported, never the thing measured
+            out["cpu_baseline"] = {"value": None, "error": str(e)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,192 @@
+/*
+ * p2de_hip.h -- C-ABI of the MI355X drop-in for P2DE.jl's limited ESDG right-hand side
+ * and SSP-RK3 stage update (1D/2D compressible Euler, Lobatto collocation).
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, never throws, and
+ * returns a status (P2DE_OK = 0).  Host arrays are owned by the caller and only read
+ * or written during the call.  Device memory, the HIP stream and all kernels belong to
+ * the context.  A context is used by one host thread at a time.
+ *
+ * Memory layout of every array is the reference's Julia layout (column-major, isbits
+ * SVector elements stored inline, component fastest):
+ *   Uq, rhsU :: Array{SVector{Nc,Float64},2}(Nq, K)   -> double[K][Nq][Nc]
+ *   operators are Matrix{Float64} column-major; index arrays hold 1-based Int64.
+ */
+#ifndef P2DE_HIP_H
+#define P2DE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status codes */
+#define P2DE_OK 0
+#define P2DE_E_INVALID 1     /* null pointer, size mismatch, index out of range */
+#define P2DE_E_UNSUPPORTED 2 /* option or operator structure not implemented */
+#define P2DE_E_HIP 3         /* HIP runtime error (message in p2de_last_error) */
+#define P2DE_E_ALLOC 4       /* device or host allocation failed */
+#define P2DE_E_STATE 5       /* call out of order (e.g. no state uploaded) */
+#define P2DE_E_POSITIVITY 6  /* a stage produced rho <= 0 or p <= 0 (or NaN) */
+
+/* ------------------------------------------------ option enums (Solver.jl:1-121) */
+enum { P2DE_RHS_LIMITED_DG = 0 };                 /* LimitedDG / ESLimitedLowOrderPos */
+enum { P2DE_LF_ON_NODAL_VAL = 0,                  /* LaxFriedrichsOnNodalVal */
+       P2DE_LF_ON_PROJECTED_VAL = 1 };            /* LaxFriedrichsOnProjectedVal */
+enum { P2DE_CHANDRASHEKAR_FLUX = 0 };             /* ChandrashekarFlux */
+enum { P2DE_BASIS_LOBATTO = 0, P2DE_BASIS_GAUSS = 1 };
+enum { P2DE_NO_ENTROPY_PROJ_LIMITER = 0, P2DE_NODEWISE_SCALED_EXTRAPOLATION = 1 };
+enum { P2DE_SUBCELL_LIMITER = 0, P2DE_ZHANG_SHU_LIMITER = 1 };
+enum { P2DE_POSITIVITY_BOUND = 0 };
+enum { P2DE_NO_SHOCK_CAPTURE = 0 };
+enum { P2DE_PRECISION_FP64 = 0 };
+/* Upper density bound of the subcell limiter.  REFERENCE: as limiter_utils.jl:67
+ * destructures the 5-value bound tuple of subcell.jl:299 into 4 names, so
+ * Urho = Lphi (0 for PositivityBound).  INTENDED: Urho = Urhoe = Inf. */
+enum { P2DE_UPPER_BOUND_REFERENCE = 0, P2DE_UPPER_BOUND_INTENDED = 1 };
+
+/* Param (Solver.jl:151-170) as scalars and enums. */
+typedef struct {
+    int Nd;            /* 1 or 2 */
+    int N;             /* polynomial degree, 1..4 */
+    int Kx, Ky;        /* elements per direction (Ky = 1 in 1D); K = Kx*Ky */
+    double gamma, zeta, eta, POSTOL, ZEROTOL;
+    double CFL, dt0, t0, T;
+    int rhs;                      /* P2DE_RHS_LIMITED_DG */
+    int low_order_surface_flux;   /* P2DE_LF_ON_NODAL_VAL */
+    int high_order_surface_flux;  /* P2DE_LF_ON_PROJECTED_VAL */
+    int high_order_volume_flux;   /* P2DE_CHANDRASHEKAR_FLUX */
+    int basis;                    /* P2DE_BASIS_LOBATTO */
+    int entropyproj_limiter;      /* P2DE_NO_ENTROPY_PROJ_LIMITER */
+    int rhs_limiter;              /* P2DE_SUBCELL_LIMITER */
+    int bound;                    /* P2DE_POSITIVITY_BOUND */
+    int shockcapture;             /* P2DE_NO_SHOCK_CAPTURE */
+    int upper_bound_mode;         /* P2DE_UPPER_BOUND_* */
+    int precision;                /* P2DE_PRECISION_FP64 */
+} p2de_config;
+
+/* Operators{DIM} (Solver.jl:192-212), sizes from SizeData (Solver.jl:173-183). */
+typedef struct {
+    int Nq, Nfp, Nh;
+    const double *Srsh_db;   /* Nd blocks of Nh x Nh */
+    const double *Srs0;      /* Nd blocks of Nq x Nq (dense copy of the sparse matrices) */
+    int nnz_h;
+    const int64_t *Srsh_nnz; /* nnz_h pairs (i, j), i > j, 1-based, in the reference's order */
+    int nnz_0;
+    const int64_t *Srs0_nnz;
+    const double *Brs;       /* Nd blocks of Nfp x Nfp (diagonal) */
+    const double *MinvVhT;   /* Nq x Nh */
+    const double *MinvVfT;   /* Nq x Nfp */
+    const double *Vf;        /* Nfp x Nq */
+    const double *Vf_low;    /* Nfp x Nq */
+    const double *wq;        /* Nq */
+    const int64_t *fq2q;     /* Nfp, 1-based */
+    const int64_t *q2fq;     /* Nq x 2 column-major, 1-based, 0 = absent */
+} p2de_operators;
+
+/* GeomData (Solver.jl:185-190). */
+typedef struct {
+    const double *Jq;        /* Nq x K */
+    const double *GJh;       /* NGEO blocks of Nh x K: rxJh (, sxJh, ryJh, syJh) */
+} p2de_geometry;
+
+/* BCData (StateParam.jl:1-7). */
+typedef struct {
+    const int64_t *mapP;     /* Nfp x K, 1-based linear index into (Nfp, K) */
+    int nI;
+    const int64_t *mapI;     /* inflow face nodes, 1-based */
+    const double *Ival;      /* nI x Nc, component fastest */
+    int nO;
+    const int64_t *mapO;     /* outflow face nodes, 1-based */
+} p2de_bcdata;
+
+typedef struct p2de_ctx p2de_ctx;
+
+/* Positivity report: nodes with !(rho > 0) or !(p > 0), counted over every stage
+ * since the last reset (p2de_check_positivity resets it). */
+typedef struct {
+    int64_t violations;
+    int64_t first_node;      /* i + Nq*k of the first offending node, -1 if none */
+    int first_stage;         /* 1..3, 0 if none */
+    double min_rho, min_p;   /* over the last state checked by p2de_check_positivity */
+} p2de_pos_report;
+
+typedef struct {
+    int steps;               /* steps taken */
+    double t;                /* final time */
+    double last_dt;
+    int64_t violations;      /* positivity violations seen (every stage) */
+} p2de_run_stats;
+
+/* Fields readable with p2de_get_field after p2de_rhs with debug output on. */
+enum {
+    P2DE_FIELD_RHSU = 0,     /* (Nq, K) x Nc */
+    P2DE_FIELD_RHSH = 1,     /* (Nq, K) x Nc */
+    P2DE_FIELD_RHSL = 2,     /* (Nq, K) x Nc */
+    P2DE_FIELD_L_LOCAL = 3   /* (Nq+N1D, Nd, K) for the stage of the last call */
+};
+
+/* Replaces initialize_DG's products as rhs!/SSP33! consume them (init.jl:44-60):
+ * validates the options and operator structure, builds the per-element tables,
+ * allocates device memory and creates the stream.  Returns P2DE_E_UNSUPPORTED for
+ * option sets or operators outside the implemented path. */
+int p2de_create(const p2de_config *cfg, const p2de_operators *ops, const p2de_geometry *geom,
+                const p2de_bcdata *bc, p2de_ctx **out);
+void p2de_destroy(p2de_ctx *ctx);
+
+/* state.preallocation.Uq (State.jl:2): host -> device and device -> host copies,
+ * complete on return. */
+int p2de_set_state(p2de_ctx *ctx, const double *Uq);
+int p2de_get_state(p2de_ctx *ctx, double *Uq);
+
+/* rhs!(state, solver, state_param, TimeParam(t, dt, nstage)) (rhs.jl:5-13): evaluates the
+ * limited RHS of the device state with the limiter using dt.  *dt_out receives the CFL
+ * time step at nstage 1 (low_order_graph_viscosity.jl:222-243) and dt otherwise.  If
+ * rhsU is not NULL the RHS is copied to it. */
+int p2de_rhs(p2de_ctx *ctx, double t, double dt, int nstage, double *dt_out, double *rhsU);
+
+/* Keep rhsH, rhsL and L_local of the next p2de_rhs calls for p2de_get_field (debug). */
+int p2de_set_debug(p2de_ctx *ctx, int on);
+int p2de_get_field(p2de_ctx *ctx, int field, double *out);
+
+/* One iteration of SSP33!'s time loop (SSPRK33.jl:29-43) on the device state:
+ * dt = min(CFL*dt0, T - t), three limited stages, t += dt.  *t_inout is updated,
+ * *dt_out receives the step.  Does nothing (dt = 0) once t >= T. */
+int p2de_ssprk3_step(p2de_ctx *ctx, double *t_inout, double *dt_out);
+
+/* SSP33!'s loop (SSPRK33.jl:28-56) without history output: steps from the current
+ * device time until t >= T or max_steps steps (max_steps <= 0: no limit).  Launches
+ * run back to back; time and dt stay on the device. */
+int p2de_run(p2de_ctx *ctx, int max_steps, p2de_run_stats *stats);
+
+/* Device time control for p2de_run/p2de_ssprk3_step. */
+int p2de_set_time(p2de_ctx *ctx, double t);
+int p2de_get_time(p2de_ctx *ctx, double *t, double *last_dt);
+
+/* Positivity of the device state plus the count of violations seen in every stage
+ * since the last call; then resets the count. */
+int p2de_check_positivity(p2de_ctx *ctx, p2de_pos_report *rep);
+
+/* Launch only the kernels of `steps` SSP-RK3 steps, no host synchronisation. */
+int p2de_enqueue_steps(p2de_ctx *ctx, int steps);
+/* Wait for all queued device work of the context. */
+int p2de_synchronize(p2de_ctx *ctx);
+/* Per-kernel timing with HIP events recorded on the context's stream around every
+ * launch while enabled.  kinds: 0 = element RHS (phase A), 1 = symmetrise+update
+ * (phase B), 2 = step bookkeeping.  ms[k] = total milliseconds, count[k] = launches. */
+int p2de_set_kernel_timing(p2de_ctx *ctx, int on);
+int p2de_get_kernel_timing(p2de_ctx *ctx, double ms[3], int64_t count[3]);
+
+/* Number of degrees of freedom (nodes) K*Nq, and the device address of the state
+ * (double[K][Nq][Nc]) for callers that keep data on the GPU. */
+int64_t p2de_num_dofs(const p2de_ctx *ctx);
+int p2de_device_state(p2de_ctx *ctx, void **dptr);
+
+const char *p2de_last_error(const p2de_ctx *ctx);
+const char *p2de_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1128 @@
+/*
+ * p2de_oracle.c -- FP64 CPU oracle (TEST INFRASTRUCTURE; see p2de_oracle.h).
+ *
+ * Each function names the reference function it restates (file:line under src/).
+ * The arithmetic keeps the reference's operation order and IEEE semantics: build with
+ * -O2 -fno-fast-math -ffp-contract=off so no product/sum is fused or reassociated.
+ * Julia's min/max (NaN-propagating, -0 < +0) and its generic 2-norm are reproduced.
+ *
+ * Parallelism mirrors Polyester @batch: OpenMP parallel-for over elements.  The one
+ * cross-element write of the reference (2D symmetrize, subcell.jl:428-455, which writes
+ * the neighbour's entry from inside @batch) is done as a two-pass gather here so the
+ * C code has no data race; the values are the same because both sides store the min.
+ */
+#include "p2de_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define NC_MAX 4
+#define ND_MAX 2
+
+/* ------------------------------------------------------------------------------- */
+/* Julia Base semantics                                                             */
+/* ------------------------------------------------------------------------------- */
+/* Base.min / Base.max for IEEE floats (Julia 1.8 base/math.jl) */
+static inline double jmin(double x, double y) {
+    double diff = x - y;
+    double am = signbit(diff) ? x : y;
+    return (isnan(x) || isnan(y)) ? diff : am;
+}
+static inline double jmax(double x, double y) {
+    double diff = x - y;
+    double am = signbit(diff) ? y : x;
+    return (isnan(x) || isnan(y)) ? diff : am;
+}
+/* LinearAlgebra.generic_norm2 on a 2-tuple */
+static double jnorm2(double a, double b) {
+    double maxabs = jmax(fabs(a), fabs(b));
+    if (maxabs == 0.0 || isinf(maxabs) || isnan(maxabs)) return maxabs;
+    if (isfinite(2.0 * maxabs * maxabs) && (maxabs * maxabs) != 0.0) {
+        double s = a * a;
+        s += b * b;
+        return sqrt(s);
+    }
+    double s = (fabs(a) / maxabs) * (fabs(a) / maxabs);
+    s += (fabs(b) / maxabs) * (fabs(b) / maxabs);
+    return maxabs * sqrt(s);
+}
+
+/* ------------------------------------------------------------------------------- */
+/* Pointwise Euler physics: src/math/compressible_Navier_Stokes.jl                  */
+/* ------------------------------------------------------------------------------- */
+/* pfun :18-28 */
+static inline double pfun(int Nd, double g, const double *U) {
+    if (Nd == 1) return (g - 1.0) * (U[2] - 0.5 * (U[1] * U[1]) / U[0]);
+    return (g - 1.0) * (U[3] - 0.5 * (U[1] * U[1] + U[2] * U[2]) / U[0]);
+}
+/* betafun :42-45 */
+static inline double betafun(int Nd, double g, const double *U) {
+    double p = pfun(Nd, g, U);
+    return U[0] / (2.0 * p);
+}
+/* rhoe_ufun :70-78 */
+static inline double rhoe_ufun(int Nd, const double *U) {
+    if (Nd == 1) return U[2] - 0.5 * U[1] * U[1] / U[0];
+    return U[3] - 0.5 * (U[1] * U[1] + U[2] * U[2]) / U[0];
+}
+/* wavespeed_estimate :48-62 (2D rotates to a 1D state along n, then uses the 1D formula) */
+static inline double wavespeed1(double g, double rho, double m, double E) {
+    double U1[3] = {rho, m, E};
+    double p = pfun(1, g, U1);
+    return fabs(m / rho) + sqrt(g * p / rho);
+}
+static inline double wavespeed(int Nd, double g, const double *U, const double *n) {
+    if (Nd == 1) return wavespeed1(g, U[0], U[1], U[2]);
+    return wavespeed1(g, U[0], n[0] * U[1] + n[1] * U[2], U[3]);
+}
+/* sfun :64-68 */
+static inline double sfun(int Nd, double g, const double *U) {
+    double p = pfun(Nd, g, U);
+    return log(p / pow(U[0], g));
+}
+/* v_ufun :113-122, :134-144 */
+static void v_ufun(int Nd, double g, const double *U, double *V) {
+    double s = sfun(Nd, g, U);
+    double p = pfun(Nd, g, U);
+    if (Nd == 1) {
+        V[0] = (g + 1.0 - s) - (g - 1.0) * U[2] / p;
+        V[1] = U[1] * (g - 1.0) / p;
+        V[2] = -U[0] * (g - 1.0) / p;
+    } else {
+        V[0] = (g + 1.0 - s) - (g - 1.0) * U[3] / p;
+        V[1] = U[1] * (g - 1.0) / p;
+        V[2] = U[2] * (g - 1.0) / p;
+        V[3] = -U[0] * (g - 1.0) / p;
+    }
+}
+/* s_vfun :87-97, rhoe_vfun :99-104 */
+static inline double rhoe_vfun(int Nd, double g, const double *V) {
+    double s, vE;
+    if (Nd == 1) {
+        s = g - V[0] + V[1] * V[1] / (2.0 * V[2]);
+        vE = V[2];
+    } else {
+        s = g - V[0] + (V[1] * V[1] + V[2] * V[2]) / (2.0 * V[3]);
+        vE = V[3];
+    }
+    return pow((g - 1.0) / pow(-vE, g), 1.0 / (g - 1.0)) * exp(-s / (g - 1.0));
+}
+/* u_vfun :146-163 */
+static void u_vfun(int Nd, double g, const double *V, double *U) {
+    double r = rhoe_vfun(Nd, g, V);
+    if (Nd == 1) {
+        U[0] = -r * V[2];
+        U[1] = r * V[1];
+        U[2] = r * (1.0 - V[1] * V[1] / (2.0 * V[2]));
+    } else {
+        U[0] = -r * V[3];
+        U[1] = r * V[1];
+        U[2] = r * V[2];
+        U[3] = r * (1.0 - (V[1] * V[1] + V[2] * V[2]) / (2.0 * V[3]));
+    }
+}
+/* fluxes :165-194; F[d*Nc + c] */
+static void fluxes(int Nd, double g, const double *U, double *F) {
+    if (Nd == 1) {
+        double p = pfun(1, g, U);
+        double u = U[1] / U[0];
+        F[0] = U[1];
+        F[1] = U[1] * u + p;
+        F[2] = u * (U[2] + p);
+        return;
+    }
+    double p = pfun(2, g, U);
+    double u = U[1] / U[0];
+    double v = U[2] / U[0];
+    double rhouv = U[0] * u * v;
+    double Ep = U[3] + p;
+    F[0] = U[1];
+    F[1] = U[1] * u + p;
+    F[2] = rhouv;
+    F[3] = u * Ep;
+    F[4] = U[2];
+    F[5] = rhouv;
+    F[6] = U[2] * v + p;
+    F[7] = v * Ep;
+}
+/* logmean :307-321 (Winters et al. series with the reference's coefficients) */
+static inline double logmean(double aL, double aR, double logL, double logR) {
+    double da = aR - aL;
+    double aavg = 0.5 * (aR + aL);
+    double f = da / aavg;
+    double v = f * f;
+    if (fabs(f) < 1e-4) {
+        return aavg * (1.0 + v * (-0.2 - v * (0.0512 - v * 0.026038857142857)));
+    }
+    return -da / (logL - logR);
+}
+/* fS :196-249.  UL/UR = (rho, u, [v,] beta, log rho, log beta); F[d*Nc + c] */
+static void fS(int Nd, double g, const double *UL, const double *UR, double *F) {
+    if (Nd == 1) {
+        double rhoL = UL[0], uL = UL[1], betaL = UL[2], rhologL = UL[3], betalogL = UL[4];
+        double rhoR = UR[0], uR = UR[1], betaR = UR[2], rhologR = UR[3], betalogR = UR[4];
+        double rholog = logmean(rhoL, rhoR, rhologL, rhologR);
+        double betalog = logmean(betaL, betaR, betalogL, betalogR);
+        double rhoavg = 0.5 * (rhoL + rhoR);
+        double uavg = 0.5 * (uL + uR);
+        double unorm = uL * uR;
+        double pa = rhoavg / (betaL + betaR);
+        double f4aux = rholog / (2.0 * (g - 1.0) * betalog) + pa + 0.5 * rholog * unorm;
+        F[0] = rholog * uavg;
+        F[1] = F[0] * uavg + pa;
+        F[2] = f4aux * uavg;
+        return;
+    }
+    double rhoL = UL[0], uL = UL[1], vL = UL[2], betaL = UL[3], rhologL = UL[4], betalogL = UL[5];
+    double rhoR = UR[0], uR = UR[1], vR = UR[2], betaR = UR[3], rhologR = UR[4], betalogR = UR[5];
+    double rholog = logmean(rhoL, rhoR, rhologL, rhologR);
+    double betalog = logmean(betaL, betaR, betalogL, betalogR);
+    double rhoavg = 0.5 * (rhoL + rhoR);
+    double uavg = 0.5 * (uL + uR);
+    double vavg = 0.5 * (vL + vR);
+    double unorm = uL * uR + vL * vR;
+    double pa = rhoavg / (betaL + betaR);
+    double f4aux = rholog / (2.0 * (g - 1.0) * betalog) + pa + 0.5 * rholog * unorm;
+    double FxS1 = rholog * uavg;
+    double FxS2 = FxS1 * uavg + pa;
+    double FxS3 = FxS1 * vavg;
+    double FxS4 = f4aux * uavg;
+    double FyS1 = rholog * vavg;
+    double FyS3 = FyS1 * vavg + pa;
+    double FyS4 = f4aux * vavg;
+    F[0] = FxS1; F[1] = FxS2; F[2] = FxS3; F[3] = FxS4;
+    F[4] = FyS1; F[5] = FxS3; F[6] = FyS3; F[7] = FyS4;
+}
+
+/* ------------------------------------------------------------------------------- */
+/* Context                                                                          */
+/* ------------------------------------------------------------------------------- */
+struct oracle_ctx {
+    oracle_problem p;
+    int NGEO, NLs; /* NLs = Nq + N1D */
+    /* Preallocation (State.jl:1-26) */
+    double *vq, *v_tilde, *u_tilde, *psi_tilde;
+    double *rhsH, *rhsL, *rhsU, *rhsxyH, *rhsxyL, *rhsxyU;
+    double *BF_H, *BF_L, *fstar_H, *fstar_L, *L_local;
+    double *resW, *resZ;
+    /* LowOrderPositivityCache (State.jl:29-50) */
+    double *flux, *Q0F1, *wavespeed_f, *Uf, *uPL, *lambda, *lambdaB;
+    /* FluxDiffCache (State.jl:52-85) */
+    double *beta, *rholog, *betalog, *uPH, *betaP, *rhologP, *betalogP, *lam, *LFc;
+    double *QF1, *MinvVhTQF1, *MinvVfTBF1;
+    /* SubcellLimiterCache (State.jl:131-185) */
+    double *f_bar_H, *f_bar_L, *f_bar_lim, *lbound_s_modified, *uL, *L_copy;
+    /* ShockCaptureCache */
+    double *blending_factor; /* (K, 3) */
+    double *smooth_factor;   /* (K, 3) */
+    double *dtarr;
+    int nthreads_alloc;
+    double *stage_buf;       /* rhsU copy for the RK driver */
+};
+
+static int g_threads = 0;
+void oracle_set_threads(int n) { g_threads = n; }
+int oracle_get_threads(void) {
+#ifdef _OPENMP
+    return g_threads > 0 ? g_threads : omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+#define OMP_FOR _Pragma("omp parallel for schedule(static) num_threads(nthr)")
+
+static double *zalloc(int64_t n) { return (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double)); }
+
+int oracle_create(const oracle_problem *p, oracle_ctx **out) {
+    if (!p || !out) return 1;
+    if (p->Nd != 1 && p->Nd != 2) return 2;
+    if (p->Nc != p->Nd + 2) return 2;
+    oracle_ctx *c = (oracle_ctx *)calloc(1, sizeof(oracle_ctx));
+    if (!c) return 3;
+    c->p = *p;
+    int64_t K = p->K, Nq = p->Nq, Nh = p->Nh, Nfp = p->Nfp, Nc = p->Nc, Nd = p->Nd;
+    c->NGEO = Nd == 1 ? 1 : 4;
+    c->NLs = p->Nq + p->N1D;
+    int64_t NLs = c->NLs;
+    c->vq = zalloc(Nq * K * Nc);
+    c->v_tilde = zalloc(Nh * K * Nc);
+    c->u_tilde = zalloc(Nh * K * Nc);
+    c->psi_tilde = zalloc(Nh * K * Nd);
+    c->rhsH = zalloc(Nq * K * Nc);
+    c->rhsL = zalloc(Nq * K * Nc);
+    c->rhsU = zalloc(Nq * K * Nc);
+    c->rhsxyH = zalloc(Nq * K * Nd * Nc);
+    c->rhsxyL = zalloc(Nq * K * Nd * Nc);
+    c->rhsxyU = zalloc(Nq * K * Nd * Nc);
+    c->BF_H = zalloc(Nfp * K * Nd * Nc);
+    c->BF_L = zalloc(Nfp * K * Nd * Nc);
+    c->fstar_H = zalloc(Nfp * K * Nd * Nc);
+    c->fstar_L = zalloc(Nfp * K * Nd * Nc);
+    c->L_local = zalloc(NLs * Nd * K * 3);
+    c->L_copy = zalloc(NLs * Nd * K);
+    c->resW = zalloc(Nq * K * Nc);
+    c->resZ = zalloc(Nq * K * Nc);
+    c->flux = zalloc(Nh * K * Nd * Nc);
+    c->Q0F1 = zalloc(Nq * K * Nd * Nc);
+    c->wavespeed_f = zalloc(Nfp * K);
+    c->Uf = zalloc(Nfp * K * Nc);
+    c->uPL = zalloc(Nfp * K * Nc);
+    c->lambda = zalloc(Nq * Nq * K);
+    c->lambdaB = zalloc(Nfp * K);
+    c->beta = zalloc(Nh * K);
+    c->rholog = zalloc(Nh * K);
+    c->betalog = zalloc(Nh * K);
+    c->uPH = zalloc(Nfp * K * Nc);
+    c->betaP = zalloc(Nfp * K);
+    c->rhologP = zalloc(Nfp * K);
+    c->betalogP = zalloc(Nfp * K);
+    c->lam = zalloc(Nfp * K);
+    c->LFc = zalloc(Nfp * K);
+    c->QF1 = zalloc(Nh * K * Nd * Nc);
+    c->MinvVhTQF1 = zalloc(Nq * K * Nd * Nc);
+    c->MinvVfTBF1 = zalloc(Nq * K * Nd * Nc);
+    c->f_bar_H = zalloc(Nd * NLs * K * Nc);
+    c->f_bar_L = zalloc(Nd * NLs * K * Nc);
+    c->f_bar_lim = zalloc(Nd * NLs * K * Nc);
+    c->lbound_s_modified = zalloc(Nq * K);
+    c->uL = zalloc(Nq * K * Nc);
+    c->blending_factor = zalloc(K * 3);
+    c->smooth_factor = zalloc(K * 3);
+    c->dtarr = zalloc(K);
+    c->stage_buf = zalloc(Nq * K * Nc);
+    *out = c;
+    return 0;
+}
+
+void oracle_destroy(oracle_ctx *c) {
+    if (!c) return;
+    double **f[] = {&c->vq, &c->v_tilde, &c->u_tilde, &c->psi_tilde, &c->rhsH, &c->rhsL, &c->rhsU,
+                    &c->rhsxyH, &c->rhsxyL, &c->rhsxyU, &c->BF_H, &c->BF_L, &c->fstar_H, &c->fstar_L,
+                    &c->L_local, &c->L_copy, &c->resW, &c->resZ, &c->flux, &c->Q0F1, &c->wavespeed_f,
+                    &c->Uf, &c->uPL, &c->lambda, &c->lambdaB, &c->beta, &c->rholog, &c->betalog,
+                    &c->uPH, &c->betaP, &c->rhologP, &c->betalogP, &c->lam, &c->LFc, &c->QF1,
+                    &c->MinvVhTQF1, &c->MinvVfTBF1, &c->f_bar_H, &c->f_bar_L, &c->f_bar_lim,
+                    &c->lbound_s_modified, &c->uL, &c->blending_factor, &c->smooth_factor,
+                    &c->dtarr, &c->stage_buf};
+    for (size_t i = 0; i < sizeof(f) / sizeof(f[0]); i++) free(*f[i]);
+    free(c);
+}
+
+int64_t oracle_field_size(const oracle_ctx *c, int field) {
+    const oracle_problem *p = &c->p;
+    int64_t K = p->K, Nq = p->Nq, Nh = p->Nh, Nfp = p->Nfp, Nc = p->Nc, Nd = p->Nd;
+    switch (field) {
+    case ORACLE_RHSU: case ORACLE_RHSH: case ORACLE_RHSL: case ORACLE_VQ: return Nq * K * Nc;
+    case ORACLE_RHSXYH: case ORACLE_RHSXYL: return Nq * K * Nd * Nc;
+    case ORACLE_L_LOCAL: return (int64_t)c->NLs * Nd * K * 3;
+    case ORACLE_BF_H: case ORACLE_BF_L: case ORACLE_FSTAR_H: case ORACLE_FSTAR_L: return Nfp * K * Nd * Nc;
+    case ORACLE_U_TILDE: return Nh * K * Nc;
+    case ORACLE_QF1: return Nh * K * Nd * Nc;
+    case ORACLE_LAMBDA: return Nq * Nq * K;
+    case ORACLE_LAMBDAB: return Nfp * K;
+    case ORACLE_FBAR_H: case ORACLE_FBAR_L: return Nd * (int64_t)c->NLs * K * Nc;
+    case ORACLE_PSI_TILDE: return Nh * K * Nd;
+    default: return -1;
+    }
+}
+
+const double *oracle_field(const oracle_ctx *c, int field) {
+    switch (field) {
+    case ORACLE_RHSU: return c->rhsU;
+    case ORACLE_RHSH: return c->rhsH;
+    case ORACLE_RHSL: return c->rhsL;
+    case ORACLE_RHSXYH: return c->rhsxyH;
+    case ORACLE_RHSXYL: return c->rhsxyL;
+    case ORACLE_L_LOCAL: return c->L_local;
+    case ORACLE_BF_H: return c->BF_H;
+    case ORACLE_BF_L: return c->BF_L;
+    case ORACLE_U_TILDE: return c->u_tilde;
+    case ORACLE_QF1: return c->QF1;
+    case ORACLE_LAMBDA: return c->lambda;
+    case ORACLE_LAMBDAB: return c->lambdaB;
+    case ORACLE_FBAR_H: return c->f_bar_H;
+    case ORACLE_FBAR_L: return c->f_bar_L;
+    case ORACLE_VQ: return c->vq;
+    case ORACLE_PSI_TILDE: return c->psi_tilde;
+    case ORACLE_FSTAR_H: return c->fstar_H;
+    case ORACLE_FSTAR_L: return c->fstar_L;
+    default: return NULL;
+    }
+}
+
+/* ------------------------------------------------------------------------------- */
+/* Geometry helpers: rhs_utils.jl                                                    */
+/* ------------------------------------------------------------------------------- */
+#define IDX_V(i, k, N, Nc) ((int64_t)(Nc) * ((int64_t)(i) + (int64_t)(N) * (int64_t)(k)))
+#define IDX_XY(d, i, k, N, Nd, Nc) ((int64_t)(Nc) * ((int64_t)(d) + (int64_t)(Nd) * ((int64_t)(i) + (int64_t)(N) * (int64_t)(k))))
+
+static inline double gjh(const oracle_ctx *c, int g, int i, int k) {
+    return c->p.GJh[(int64_t)g * c->p.Nh * c->p.K + i + (int64_t)c->p.Nh * k];
+}
+/* reference_to_physical :1-12 */
+static inline void ref2phys(const oracle_ctx *c, const double *Urs, int i, int k, double *out) {
+    if (c->p.Nd == 1) {
+        out[0] = gjh(c, 0, i, k) * Urs[0];
+        return;
+    }
+    double rxJ = gjh(c, 0, i, k), sxJ = gjh(c, 1, i, k), ryJ = gjh(c, 2, i, k), syJ = gjh(c, 3, i, k);
+    out[0] = rxJ * Urs[0] + sxJ * Urs[1];
+    out[1] = ryJ * Urs[0] + syJ * Urs[1];
+}
+/* Bx :13-27 (face node i, 0-based; geometric factors at hybrid index Nq+i).  The
+ * reference's 1D method passes its arguments in the wrong order (:18) and would throw;
+ * this computes its evident intent rxJh * Br[i,i]. */
+static inline void Bx(const oracle_ctx *c, int i, int k, double *B) {
+    int Nfp = c->p.Nfp;
+    double Urs[2];
+    for (int d = 0; d < c->p.Nd; d++) Urs[d] = c->p.Brs[(int64_t)d * Nfp * Nfp + i + (int64_t)Nfp * i];
+    ref2phys(c, Urs, c->p.Nq + i, k, B);
+}
+/* Bx_with_n :29-37 */
+static inline double Bx_with_n(const oracle_ctx *c, int i, int k, double *B) {
+    Bx(c, i, k, B);
+    if (c->p.Nd == 1) return fabs(B[0]);
+    return jnorm2(B[0], B[1]);
+}
+/* Sx :39-51 (geometric factors at node i, 0-based hybrid indices) */
+static inline void Sx(const oracle_ctx *c, int i, int j, int k, double *S) {
+    int Nh = c->p.Nh;
+    double Urs[2];
+    for (int d = 0; d < c->p.Nd; d++) Urs[d] = c->p.Srsh_db[(int64_t)d * Nh * Nh + i + (int64_t)Nh * j];
+    ref2phys(c, Urs, i, k, S);
+}
+/* Sx0_with_n :53-75 */
+static inline double Sx0_with_n(const oracle_ctx *c, int i, int j, int k, double *S) {
+    int Nq = c->p.Nq;
+    double Urs[2];
+    for (int d = 0; d < c->p.Nd; d++) Urs[d] = c->p.Srs0[(int64_t)d * Nq * Nq + i + (int64_t)Nq * j];
+    ref2phys(c, Urs, i, k, S);
+    if (c->p.Nd == 1) return fabs(S[0]);
+    return jnorm2(S[0], S[1]);
+}
+
+/* ------------------------------------------------------------------------------- */
+/* entropy_projection!(::NoEntropyProjectionLimiter) rhs.jl:59-110                   */
+/* ------------------------------------------------------------------------------- */
+static void entropy_projection(oracle_ctx *c, const double *Uq, int nthr) {
+    const oracle_problem *p = &c->p;
+    int K = p->K, Nq = p->Nq, Nfp = p->Nfp, Nh = p->Nh, Nc = p->Nc, Nd = p->Nd;
+    double g = p->gamma;
+    OMP_FOR
+    for (int k = 0; k < K; k++) {
+        double l = 1.0;
+        /* calculate_entropy_var! :72-77 */
+        for (int i = 0; i < Nq; i++) v_ufun(Nd, g, Uq + IDX_V(i, k, Nq, Nc), c->vq + IDX_V(i, k, Nq, Nc));
+        /* entropy_projection_volume_node! :79-82 */
+        for (int i = 0; i < Nq; i++)
+            for (int cc = 0; cc < Nc; cc++) {
+                c->v_tilde[IDX_V(i, k, Nh, Nc) + cc] = c->vq[IDX_V(i, k, Nq, Nc) + cc];
+                c->u_tilde[IDX_V(i, k, Nh, Nc) + cc] = Uq[IDX_V(i, k, Nq, Nc) + cc];
+            }
+        /* entropy_projection_face_node! :84-94 */
+        for (int i = 0; i < Nfp; i++) {
+            double *vt = c->v_tilde + IDX_V(i + Nq, k, Nh, Nc);
+            for (int cc = 0; cc < Nc; cc++) vt[cc] = 0.0;
+            for (int j = 0; j < Nq; j++) {
+                double coef = l * p->Vf[i + (int64_t)Nfp * j] + (1.0 - l) * p->Vf_low[i + (int64_t)Nfp * j];
+                for (int cc = 0; cc < Nc; cc++) vt[cc] += coef * c->vq[IDX_V(j, k, Nq, Nc) + cc];
+            }
+            u_vfun(Nd, g, vt, c->u_tilde + IDX_V(i + Nq, k, Nh, Nc));
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------------- */
+/* rhs_low_graph_visc! low_order_graph_viscosity.jl:4-41                            */
+/* ------------------------------------------------------------------------------- */
+static double rhs_low_graph_visc(oracle_ctx *c, const double *Uq, double t, double dt, int nstage, int nthr) {
+    const oracle_problem *p = &c->p;
+    int K = p->K, Nq = p->Nq, Nfp = p->Nfp, Nh = p->Nh, Nc = p->Nc, Nd = p->Nd;
+    double g = p->gamma;
+    /* calculate_wavespeed_and_inviscid_flux! :43-49 */
+    OMP_FOR
+    for (int k = 0; k < K; k++) {
+        /* update_face_values!(::LaxFriedrichsOnNodalVal) :51-61 */
+        for (int i = 0; i < Nfp; i++) {
+            int iq = (int)p->fq2q[i] - 1;
+            for (int cc = 0; cc < Nc; cc++) c->Uf[IDX_V(i, k, Nfp, Nc) + cc] = Uq[IDX_V(iq, k, Nq, Nc) + cc];
+        }
+        /* update_wavespeed_and_inviscid_flux! :73-92 */
+        for (int i = 0; i < Nq; i++) fluxes(Nd, g, Uq + IDX_V(i, k, Nq, Nc), c->flux + IDX_XY(0, i, k, Nh, Nd, Nc));
+        for (int i = 0; i < Nfp; i++) {
+            const double *u_i = c->Uf + IDX_V(i, k, Nfp, Nc);
+            double B[2] = {0.0, 0.0}, n[2] = {0.0, 0.0};
+            double nrm = Bx_with_n(c, i, k, B);
+            for (int d = 0; d < Nd; d++) n[d] = B[d] / nrm;
+            c->wavespeed_f[i + (int64_t)Nfp * k] = wavespeed(Nd, g, u_i, n);
+            fluxes(Nd, g, u_i, c->flux + IDX_XY(0, i + Nq, k, Nh, Nd, Nc));
+        }
+    }
+    /* get_uP_and_enforce_BC! :94-124 */
+    OMP_FOR
+    for (int k = 0; k < K; k++) {
+        for (int i = 0; i < Nfp; i++) {
+            int64_t m = p->mapP[i + (int64_t)Nfp * k] - 1;
+            int64_t iP = m % Nfp, kP = m / Nfp;
+            for (int cc = 0; cc < Nc; cc++) c->uPL[IDX_V(i, k, Nfp, Nc) + cc] = c->Uf[IDX_V(iP, kP, Nfp, Nc) + cc];
+        }
+    }
+    for (int i = 0; i < p->nI; i++) {
+        int64_t ii = p->mapI[i] - 1;
+        for (int cc = 0; cc < Nc; cc++) c->uPL[ii * Nc + cc] = p->Ival[(int64_t)i * Nc + cc];
+    }
+    for (int i = 0; i < p->nO; i++) {
+        int64_t io = p->mapO[i] - 1;
+        int64_t iP = io % Nfp, kP = io / Nfp;
+        int64_t iq = p->fq2q[iP] - 1;
+        for (int cc = 0; cc < Nc; cc++) c->uPL[io * Nc + cc] = Uq[IDX_V(iq, kP, Nq, Nc) + cc];
+    }
+    /* clear_low_order_rhs! :126-137 */
+    memset(c->rhsxyL, 0, sizeof(double) * (size_t)Nq * K * Nd * Nc);
+    memset(c->Q0F1, 0, sizeof(double) * (size_t)Nq * K * Nd * Nc);
+    /* accumulate_low_order_rhs_volume! :139-173 */
+    OMP_FOR
+    for (int k = 0; k < K; k++) {
+        for (int pidx = 0; pidx < p->nnz_0; pidx++) {
+            int i = (int)p->Srs0_nnz[2 * pidx] - 1, j = (int)p->Srs0_nnz[2 * pidx + 1] - 1;
+            const double *u_i = Uq + IDX_V(i, k, Nq, Nc), *u_j = Uq + IDX_V(j, k, Nq, Nc);
+            double Fxy[ND_MAX * NC_MAX];
+            for (int d = 0; d < Nd; d++)
+                for (int cc = 0; cc < Nc; cc++)
+                    Fxy[d * Nc + cc] = 0.5 * (c->flux[IDX_XY(d, i, k, Nh, Nd, Nc) + cc] + c->flux[IDX_XY(d, j, k, Nh, Nd, Nc) + cc]);
+            double S0[2] = {0.0, 0.0}, n_ij[2] = {0.0, 0.0}, n_ji[2] = {0.0, 0.0};
+            double nrm = Sx0_with_n(c, i, j, k, S0);
+            for (int d = 0; d < Nd; d++) { n_ij[d] = S0[d] / nrm; n_ji[d] = -n_ij[d]; }
+            double ws = jmax(wavespeed(Nd, g, u_i, n_ij), wavespeed(Nd, g, u_j, n_ji));
+            double lam_ij = nrm * ws;
+            c->lambda[i + (int64_t)Nq * (j + (int64_t)Nq * k)] = lam_ij;
+            c->lambda[j + (int64_t)Nq * (i + (int64_t)Nq * k)] = lam_ij;
+            /* graph_viscosity :237-261 */
+            double visc[NC_MAX];
+            for (int cc = 0; cc < Nc; cc++) visc[cc] = lam_ij * (u_j[cc] - u_i[cc]);
+            int vdir = 0;
+            if (Nd == 2) vdir = fabs(S0[0]) > 1e-10 ? 0 : 1;
+            for (int d = 0; d < Nd; d++)
+                for (int cc = 0; cc < Nc; cc++) {
+                    double lD = (d == vdir) ? visc[cc] : 0.0;
+                    double v = 2.0 * S0[d] * Fxy[d * Nc + cc] - lD;
+                    c->Q0F1[IDX_XY(d, i, k, Nq, Nd, Nc) + cc] += v;
+                    c->Q0F1[IDX_XY(d, j, k, Nq, Nd, Nc) + cc] += -v;
+                }
+        }
+    }
+    OMP_FOR
+    for (int k = 0; k < K; k++)
+        for (int i = 0; i < Nq; i++)
+            for (int x = 0; x < Nd * Nc; x++) c->rhsxyL[IDX_XY(0, i, k, Nq, Nd, Nc) + x] -= c->Q0F1[IDX_XY(0, i, k, Nq, Nd, Nc) + x];
+    /* accumulate_low_order_rhs_surface! :175-204 */
+    int N1D = p->N1D;
+    OMP_FOR
+    for (int k = 0; k < K; k++) {
+        for (int i = 0; i < Nfp; i++) {
+            int64_t m = p->mapP[i + (int64_t)Nfp * k] - 1;
+            int64_t iP = m % Nfp, kP = m / Nfp;
+            double B[2] = {0.0, 0.0};
+            double nrm = Bx_with_n(c, i, k, B);
+            double lB = 0.5 * nrm * jmax(c->wavespeed_f[i + (int64_t)Nfp * k], c->wavespeed_f[iP + (int64_t)Nfp * kP]);
+            c->lambdaB[i + (int64_t)Nfp * k] = lB;
+            const double *uP = c->uPL + IDX_V(i, k, Nfp, Nc), *Uf = c->Uf + IDX_V(i, k, Nfp, Nc);
+            double fP[ND_MAX * NC_MAX];
+            fluxes(Nd, g, uP, fP);
+            double *fs = c->fstar_L + IDX_XY(0, i, k, Nfp, Nd, Nc);
+            double *BF = c->BF_L + IDX_XY(0, i, k, Nfp, Nd, Nc);
+            for (int d = 0; d < Nd; d++)
+                for (int cc = 0; cc < Nc; cc++) {
+                    fs[d * Nc + cc] = 0.5 * (c->flux[IDX_XY(d, i + Nq, k, Nh, Nd, Nc) + cc] + fP[d * Nc + cc]);
+                    BF[d * Nc + cc] = B[d] * fs[d * Nc + cc];
+                }
+            /* apply_LF_dissipation_to_BF / _to_fstar (rhs_utils.jl:78-102) */
+            int ld = (Nd == 1 || i < 2 * N1D) ? 0 : 1;
+            for (int cc = 0; cc < Nc; cc++) {
+                double lf = lB * (uP[cc] - Uf[cc]);
+                BF[ld * Nc + cc] = BF[ld * Nc + cc] - lf;
+                fs[ld * Nc + cc] = fs[ld * Nc + cc] - lf / B[ld];
+            }
+            int iq = (int)p->fq2q[i] - 1;
+            for (int x = 0; x < Nd * Nc; x++) c->rhsxyL[IDX_XY(0, iq, k, Nq, Nd, Nc) + x] -= BF[x];
+        }
+    }
+    /* scale_low_order_rhs_by_mass! :206-220 */
+    OMP_FOR
+    for (int k = 0; k < K; k++)
+        for (int i = 0; i < Nq; i++) {
+            double wJq = p->Jq[i + (int64_t)Nq * k] * p->wq[i];
+            double *r = c->rhsxyL + IDX_XY(0, i, k, Nq, Nd, Nc);
+            for (int x = 0; x < Nd * Nc; x++) r[x] = r[x] / wJq;
+            for (int cc = 0; cc < Nc; cc++) {
+                double s = r[cc];
+                for (int d = 1; d < Nd; d++) s = s + r[d * Nc + cc];
+                c->rhsL[IDX_V(i, k, Nq, Nc) + cc] = s;
+            }
+        }
+    /* calculate_lambda_and_low_order_CFL! :222-243 (nstage 1 only) */
+    if (nstage == 1) {
+        double dt0 = jmin(p->CFL * p->dt0, p->T - t);
+        OMP_FOR
+        for (int k = 0; k < K; k++) {
+            double dtk = dt0;
+            for (int i = 0; i < Nq; i++) {
+                double wq_i = p->wq[i];
+                double wJq_i = p->Jq[i + (int64_t)Nq * k] * wq_i;
+                /* lambda_i :263-281 */
+                double li = 0.0;
+                for (int j = 0; j < Nq; j++) li += c->lambda[i + (int64_t)Nq * (j + (int64_t)Nq * k)];
+                for (int m = 0; m < 2; m++) {
+                    int64_t f = p->q2fq[i + (int64_t)Nq * m];
+                    if (f <= 0) continue;
+                    li += c->lambdaB[(f - 1) + (int64_t)Nfp * k];
+                }
+                dtk = jmin(dtk, p->CFL * 0.5 * wJq_i / li);
+            }
+            c->dtarr[k] = dtk;
+        }
+        double dmin = dt0;
+        for (int k = 0; k < K; k++) dmin = jmin(dmin, c->dtarr[k]);
+        dt = dmin;
+    }
+    return dt;
+}
+
+/* ------------------------------------------------------------------------------- */
+/* rhs_fluxdiff! flux_differencing.jl:4-37                                          */
+/* ------------------------------------------------------------------------------- */
+static inline void Uvol(const oracle_ctx *c, int idx, int k, double *out) {
+    /* U(dim, ::CompressibleIdealGas, ::ChandrashekarFlux, ...) :179-191 */
+    const double *u = c->u_tilde + IDX_V(idx, k, c->p.Nh, c->p.Nc);
+    int64_t h = idx + (int64_t)c->p.Nh * k;
+    out[0] = u[0];
+    if (c->p.Nd == 1) {
+        out[1] = u[1] / u[0];
+        out[2] = c->beta[h]; out[3] = c->rholog[h]; out[4] = c->betalog[h];
+    } else {
+        out[1] = u[1] / u[0];
+        out[2] = u[2] / u[0];
+        out[3] = c->beta[h]; out[4] = c->rholog[h]; out[5] = c->betalog[h];
+    }
+}
+
+static void rhs_fluxdiff(oracle_ctx *c, const double *Uq, int nthr) {
+    const oracle_problem *p = &c->p;
+    int K = p->K, Nq = p->Nq, Nfp = p->Nfp, Nh = p->Nh, Nc = p->Nc, Nd = p->Nd, N1D = p->N1D;
+    double g = p->gamma;
+    /* calculate_primitive_variables! :39-70 */
+    OMP_FOR
+    for (int k = 0; k < K; k++)
+        for (int i = 0; i < Nh; i++) {
+            const double *u = c->u_tilde + IDX_V(i, k, Nh, Nc);
+            int64_t h = i + (int64_t)Nh * k;
+            for (int d = 0; d < Nd; d++) c->psi_tilde[h * Nd + d] = (g - 1.0) * u[1 + d];
+            c->beta[h] = betafun(Nd, g, u);
+            c->rholog[h] = log(u[0]);
+            c->betalog[h] = log(c->beta[h]);
+        }
+    OMP_FOR
+    for (int k = 0; k < K; k++)
+        for (int i = 0; i < Nfp; i++) {
+            int64_t m = p->mapP[i + (int64_t)Nfp * k] - 1;
+            int64_t iP = m % Nfp, kP = m / Nfp;
+            int64_t hP = Nq + iP + (int64_t)Nh * kP;
+            for (int cc = 0; cc < Nc; cc++) c->uPH[IDX_V(i, k, Nfp, Nc) + cc] = c->u_tilde[hP * Nc + cc];
+            c->betaP[i + (int64_t)Nfp * k] = c->beta[hP];
+            c->rhologP[i + (int64_t)Nfp * k] = c->rholog[hP];
+            c->betalogP[i + (int64_t)Nfp * k] = c->betalog[hP];
+        }
+    /* calculate_interface_dissipation_coeff! :90-114 */
+    OMP_FOR
+    for (int k = 0; k < K; k++)
+        for (int i = 0; i < Nfp; i++) {
+            double B[2] = {0.0, 0.0}, n[2] = {0.0, 0.0};
+            double nrm = Bx_with_n(c, i, k, B);
+            for (int d = 0; d < Nd; d++) n[d] = B[d] / nrm;
+            c->lam[i + (int64_t)Nfp * k] = wavespeed(Nd, g, c->u_tilde + IDX_V(Nq + i, k, Nh, Nc), n);
+            c->LFc[i + (int64_t)Nfp * k] = 0.5 * nrm;
+        }
+    OMP_FOR
+    for (int k = 0; k < K; k++)
+        for (int i = 0; i < Nfp; i++) {
+            int64_t f = i + (int64_t)Nfp * k;
+            c->LFc[f] = c->LFc[f] * jmax(c->lam[f], c->lam[p->mapP[f] - 1]);
+        }
+    /* enforce_BC! :116-151 */
+    for (int i = 0; i < p->nI; i++) c->LFc[p->mapI[i] - 1] = 0.0;
+    for (int i = 0; i < p->nO; i++) c->LFc[p->mapO[i] - 1] = 0.0;
+    for (int i = 0; i < p->nI; i++) {
+        int64_t ii = p->mapI[i] - 1;
+        double *uP = c->uPH + ii * Nc;
+        for (int cc = 0; cc < Nc; cc++) uP[cc] = p->Ival[(int64_t)i * Nc + cc];
+        c->betaP[ii] = betafun(Nd, g, uP);
+        c->rhologP[ii] = log(uP[0]);
+        c->betalogP[ii] = log(c->betaP[ii]);
+    }
+    for (int i = 0; i < p->nO; i++) {
+        int64_t io = p->mapO[i] - 1;
+        int64_t iP = io % Nfp, kP = io / Nfp;
+        int64_t iq = p->fq2q[iP] - 1;
+        double *uP = c->uPH + io * Nc;
+        for (int cc = 0; cc < Nc; cc++) uP[cc] = Uq[IDX_V(iq, kP, Nq, Nc) + cc];
+        c->betaP[io] = betafun(Nd, g, uP);
+        c->rhologP[io] = log(uP[0]);
+        c->betalogP[io] = log(c->betaP[io]);
+    }
+    /* clear_flux_differencing_cache! :153-162 */
+    memset(c->QF1, 0, sizeof(double) * (size_t)Nh * K * Nd * Nc);
+    /* flux_differencing_volume! :164-176 with accumulate_QF1! :201-211 */
+    OMP_FOR
+    for (int k = 0; k < K; k++) {
+        for (int pidx = 0; pidx < p->nnz_h; pidx++) {
+            int i = (int)p->Srsh_nnz[2 * pidx] - 1, j = (int)p->Srsh_nnz[2 * pidx + 1] - 1;
+            double Ui[6] = {0}, Uj[6] = {0}, S[2] = {0}, F[ND_MAX * NC_MAX];
+            Uvol(c, i, k, Ui);
+            Uvol(c, j, k, Uj);
+            Sx(c, i, j, k, S);
+            fS(Nd, g, Ui, Uj, F);
+            for (int d = 0; d < Nd; d++)
+                for (int cc = 0; cc < Nc; cc++) {
+                    double v = S[d] * F[d * Nc + cc];
+                    c->QF1[IDX_XY(d, i, k, Nh, Nd, Nc) + cc] += v;
+                    c->QF1[IDX_XY(d, j, k, Nh, Nd, Nc) + cc] += -v;
+                }
+        }
+    }
+    /* flux_differencing_surface! :223-247 with eval_high_order_surface_flux(::LaxFriedrichsOnProjectedVal) :262-272 */
+    OMP_FOR
+    for (int k = 0; k < K; k++)
+        for (int i = 0; i < Nfp; i++) {
+            const double *uf = c->u_tilde + IDX_V(Nq + i, k, Nh, Nc);
+            const double *uP = c->uPH + IDX_V(i, k, Nfp, Nc);
+            double ff[ND_MAX * NC_MAX], fP[ND_MAX * NC_MAX], B[2] = {0.0, 0.0};
+            fluxes(Nd, g, uf, ff);
+            fluxes(Nd, g, uP, fP);
+            double *fs = c->fstar_H + IDX_XY(0, i, k, Nfp, Nd, Nc);
+            double *BF = c->BF_H + IDX_XY(0, i, k, Nfp, Nd, Nc);
+            for (int x = 0; x < Nd * Nc; x++) fs[x] = 0.5 * (ff[x] + fP[x]);
+            Bx(c, i, k, B);
+            for (int d = 0; d < Nd; d++)
+                for (int cc = 0; cc < Nc; cc++) BF[d * Nc + cc] = B[d] * fs[d * Nc + cc];
+            int ld = (Nd == 1 || i < 2 * N1D) ? 0 : 1;
+            double LF = c->LFc[i + (int64_t)Nfp * k];
+            for (int cc = 0; cc < Nc; cc++) {
+                double lf = LF * (uP[cc] - uf[cc]);
+                BF[ld * Nc + cc] = BF[ld * Nc + cc] - lf;
+                fs[ld * Nc + cc] = fs[ld * Nc + cc] - lf / B[ld];
+            }
+        }
+    /* assemble_rhs! :331-361 (project_flux_difference_to_quad_unlimited! :274-281: generic mul!) */
+    OMP_FOR
+    for (int k = 0; k < K; k++) {
+        for (int i = 0; i < Nq; i++) {
+            double A[ND_MAX * NC_MAX], Bv[ND_MAX * NC_MAX];
+            for (int x = 0; x < Nd * Nc; x++) { A[x] = 0.0; Bv[x] = 0.0; }
+            for (int j = 0; j < Nh; j++) {
+                double a = p->MinvVhT[i + (int64_t)Nq * j];
+                const double *q = c->QF1 + IDX_XY(0, j, k, Nh, Nd, Nc);
+                for (int x = 0; x < Nd * Nc; x++) A[x] += a * q[x];
+            }
+            for (int j = 0; j < Nfp; j++) {
+                double a = p->MinvVfT[i + (int64_t)Nq * j];
+                const double *q = c->BF_H + IDX_XY(0, j, k, Nfp, Nd, Nc);
+                for (int x = 0; x < Nd * Nc; x++) Bv[x] += a * q[x];
+            }
+            double J = p->Jq[i + (int64_t)Nq * k];
+            double *r = c->rhsxyH + IDX_XY(0, i, k, Nq, Nd, Nc);
+            for (int x = 0; x < Nd * Nc; x++) {
+                c->MinvVhTQF1[IDX_XY(0, i, k, Nq, Nd, Nc) + x] = A[x];
+                c->MinvVfTBF1[IDX_XY(0, i, k, Nq, Nd, Nc) + x] = Bv[x];
+                r[x] = -(A[x] + Bv[x]) / J;
+            }
+            for (int cc = 0; cc < Nc; cc++) {
+                double s = r[cc];
+                for (int d = 1; d < Nd; d++) s = s + r[d * Nc + cc];
+                c->rhsH[IDX_V(i, k, Nq, Nc) + cc] = s;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------------- */
+/* Subcell limiter: limiter.jl:20-56, subcell.jl, limiter_utils.jl                  */
+/* ------------------------------------------------------------------------------- */
+/* rhoe_quadratic_coefficients limiter_utils.jl:134-146 and rhoe_quadratic_solve :108-132 */
+static double rhoe_quadratic_solve(const oracle_ctx *c, const double *U, const double *P, double Lrhoe) {
+    double ZEROTOL = c->p.ZEROTOL;
+    if (Lrhoe == INFINITY) return 1.0;
+    double a, b, cc;
+    if (c->p.Nd == 1) {
+        a = P[0] * P[2] - 1.0 / 2.0 * (P[1] * P[1]);
+        b = U[2] * P[0] + U[0] * P[2] - U[1] * P[1] - P[0] * Lrhoe;
+        cc = U[2] * U[0] - 1.0 / 2.0 * (U[1] * U[1]) - U[0] * Lrhoe;
+    } else {
+        a = P[0] * P[3] - 1.0 / 2.0 * (P[1] * P[1] + P[2] * P[2]);
+        b = U[3] * P[0] + U[0] * P[3] - U[1] * P[1] - U[2] * P[2] - P[0] * Lrhoe;
+        cc = U[3] * U[0] - 1.0 / 2.0 * (U[1] * U[1] + U[2] * U[2]) - U[0] * Lrhoe;
+    }
+    double l_eps_ij = 1.0;
+    if (b * b - 4.0 * a * cc >= 0.0) {
+        double r1 = (-b + sqrt(b * b - 4.0 * a * cc)) / (2.0 * a);
+        double r2 = (-b - sqrt(b * b - 4.0 * a * cc)) / (2.0 * a);
+        if (r1 > ZEROTOL && r2 > ZEROTOL) {
+            l_eps_ij = jmin(r1, r2);
+        } else if (r1 > ZEROTOL && r2 < -ZEROTOL) {
+            l_eps_ij = r1;
+        } else if (r2 > ZEROTOL && r1 < -ZEROTOL) {
+            l_eps_ij = r2;
+        }
+    }
+    return l_eps_ij;
+}
+/* limiting_param_bound_rho_rhoe limiter_utils.jl:82-96 */
+static double limiting_param_bound_rho_rhoe(const oracle_ctx *c, const double *U, const double *P,
+                                            double Lrho, double Lrhoe, double Urho, double Urhoe) {
+    double l = 1.0;
+    if (U[0] + P[0] < Lrho) l = jmax((Lrho - U[0]) / P[0], 0.0);
+    if (U[0] + P[0] > Urho) l = jmin(l, jmax((Urho - U[0]) / P[0], 0.0));
+    l = jmin(jmin(l, rhoe_quadratic_solve(c, U, P, Lrhoe)), rhoe_quadratic_solve(c, U, P, Urhoe));
+    return l;
+}
+/* limiting_param(::SubcellLimiter, ::PositivityBound) limiter_utils.jl:66-70 with the bound
+ * tuple of subcell.jl:231,299 = (Lrho, Lrhoe, Lphi, Urho, Urhoe) */
+static double limiting_param(const oracle_ctx *c, const double *uL, const double *P, double Lphi) {
+    double Lrho = c->p.zeta * uL[0];            /* rho_bound(::PositivityBound) subcell.jl:370-387 */
+    double Urho_i = INFINITY, Urhoe = INFINITY;
+    double Lrhoe = c->p.zeta * rhoe_ufun(c->p.Nd, uL);
+    if (c->p.upper_bound_mode == 0) /* `Lrho, Lrhoe, Urho, Urhoe = bound_val` takes the first four */
+        return limiting_param_bound_rho_rhoe(c, uL, P, Lrho, Lrhoe, Lphi, Urho_i);
+    return limiting_param_bound_rho_rhoe(c, uL, P, Lrho, Lrhoe, Urho_i, Urhoe);
+}
+
+/* ------------------------------------------------------------------------------- */
+/* Pointwise hooks for the tests                                                    */
+/* ------------------------------------------------------------------------------- */
+void oracle_pt_v_of_u(int Nd, double gamma, const double *U, double *V) { v_ufun(Nd, gamma, U, V); }
+void oracle_pt_u_of_v(int Nd, double gamma, const double *V, double *U) { u_vfun(Nd, gamma, V, U); }
+void oracle_pt_flux(int Nd, double gamma, const double *U, double *F) { fluxes(Nd, gamma, U, F); }
+double oracle_pt_logmean(double aL, double aR) { return logmean(aL, aR, log(aL), log(aR)); }
+double oracle_pt_wavespeed(int Nd, double gamma, const double *U, const double *n) { return wavespeed(Nd, gamma, U, n); }
+void oracle_pt_fS(int Nd, double gamma, const double *UL, const double *UR, double *F) {
+    double L[6], R[6];
+    const double *Us[2] = {UL, UR};
+    double *Ps[2] = {L, R};
+    for (int s = 0; s < 2; s++) {
+        const double *u = Us[s];
+        double *P = Ps[s];
+        double beta = betafun(Nd, gamma, u);
+        P[0] = u[0];
+        P[1] = u[1] / u[0];
+        if (Nd == 2) P[2] = u[2] / u[0];
+        P[Nd + 1] = beta;
+        P[Nd + 2] = log(u[0]);
+        P[Nd + 3] = log(beta);
+    }
+    fS(Nd, gamma, L, R, F);
+}
+double oracle_pt_limiting_param(int Nd, double zeta, double ZEROTOL, int upper_bound_mode, const double *uL,
+                                const double *P) {
+    oracle_ctx c;
+    memset(&c, 0, sizeof(c));
+    c.p.Nd = Nd;
+    c.p.zeta = zeta;
+    c.p.ZEROTOL = ZEROTOL;
+    c.p.upper_bound_mode = upper_bound_mode;
+    return limiting_param(&c, uL, P, 0.0);
+}
+
+#define LLOC(i, d, k, s) (c->L_local[(int64_t)(i) + (int64_t)c->NLs * ((d) + (int64_t)Nd * ((int64_t)(k) + (int64_t)K * (s)))])
+#define FB(arr, d, i, k) ((arr) + (int64_t)(d) * c->NLs * K * Nc + (int64_t)Nc * ((int64_t)(i) + (int64_t)c->NLs * (k)))
+
+static void apply_rhs_limiter(oracle_ctx *c, const double *Uq, double dt, int nstage, int nthr) {
+    const oracle_problem *p = &c->p;
+    int K = p->K, Nq = p->Nq, Nfp = p->Nfp, Nc = p->Nc, Nd = p->Nd, N1D = p->N1D;
+    int s = nstage - 1;
+    int NLs = c->NLs;
+    /* initialize_smoothness_indicator!(NoShockCapture, PositivityBound) shock_capture.jl:4-6: nothing */
+    /* update_blending_factor!(NoShockCapture) shock_capture.jl:111-114 */
+    for (int k = 0; k < K; k++) c->blending_factor[k + (int64_t)K * s] = 1.0;
+    /* update_smoothness_factor!(PositivityBound) subcell.jl:927-930 */
+    for (int k = 0; k < K; k++) c->smooth_factor[k + (int64_t)K * s] = 0.0;
+    /* initialize_entropy_bounds!(PositivityBound) subcell.jl:4-6 */
+    memset(c->lbound_s_modified, 0, sizeof(double) * (size_t)Nq * K);
+    /* initialize_TVD_bounds!(PositivityBound) subcell.jl:82-84: nothing */
+
+    /* accumulate_f_bar! subcell.jl:144-206 */
+    OMP_FOR
+    for (int k = 0; k < K; k++) {
+        if (Nd == 1) {
+            for (int cc = 0; cc < Nc; cc++) {
+                FB(c->f_bar_H, 0, 0, k)[cc] = c->BF_H[IDX_XY(0, 0, k, Nfp, Nd, Nc) + cc];
+                FB(c->f_bar_L, 0, 0, k)[cc] = c->BF_L[IDX_XY(0, 0, k, Nfp, Nd, Nc) + cc];
+            }
+            for (int i = 1; i < Nq + 1; i++) {
+                double wJ = p->Jq[(i - 1) + (int64_t)Nq * k] * p->wq[i - 1];
+                for (int cc = 0; cc < Nc; cc++) {
+                    FB(c->f_bar_H, 0, i, k)[cc] = FB(c->f_bar_H, 0, i - 1, k)[cc] + wJ * c->rhsH[IDX_V(i - 1, k, Nq, Nc) + cc];
+                    FB(c->f_bar_L, 0, i, k)[cc] = FB(c->f_bar_L, 0, i - 1, k)[cc] + wJ * c->rhsL[IDX_V(i - 1, k, Nq, Nc) + cc];
+                }
+            }
+            continue;
+        }
+        int N1Dp1 = N1D + 1;
+        /* x strides: fx_bar[(si, sj)] at linear si + N1Dp1*sj */
+        for (int sj = 0; sj < N1D; sj++) {
+            int iface = sj;
+            for (int cc = 0; cc < Nc; cc++) {
+                FB(c->f_bar_H, 0, 0 + N1Dp1 * sj, k)[cc] = c->BF_H[IDX_XY(0, iface, k, Nfp, Nd, Nc) + cc];
+                FB(c->f_bar_L, 0, 0 + N1Dp1 * sj, k)[cc] = c->BF_L[IDX_XY(0, iface, k, Nfp, Nd, Nc) + cc];
+            }
+            for (int si = 1; si < N1Dp1; si++) {
+                int q = (si - 1) + N1D * sj;
+                double wJ = p->wq[q] * p->Jq[q + (int64_t)Nq * k];
+                for (int cc = 0; cc < Nc; cc++) {
+                    FB(c->f_bar_H, 0, si + N1Dp1 * sj, k)[cc] = FB(c->f_bar_H, 0, (si - 1) + N1Dp1 * sj, k)[cc] + wJ * c->rhsxyH[IDX_XY(0, q, k, Nq, Nd, Nc) + cc];
+                    FB(c->f_bar_L, 0, si + N1Dp1 * sj, k)[cc] = FB(c->f_bar_L, 0, (si - 1) + N1Dp1 * sj, k)[cc] + wJ * c->rhsxyL[IDX_XY(0, q, k, Nq, Nd, Nc) + cc];
+                }
+            }
+        }
+        /* y strides: fy_bar[(si, sj)] at linear si + N1D*sj */
+        for (int si = 0; si < N1D; si++) {
+            int iface = si + 2 * N1D;
+            for (int cc = 0; cc < Nc; cc++) {
+                FB(c->f_bar_H, 1, si, k)[cc] = c->BF_H[IDX_XY(1, iface, k, Nfp, Nd, Nc) + cc];
+                FB(c->f_bar_L, 1, si, k)[cc] = c->BF_L[IDX_XY(1, iface, k, Nfp, Nd, Nc) + cc];
+            }
+            for (int sj = 1; sj < N1Dp1; sj++) {
+                int q = si + N1D * (sj - 1);
+                double wJ = p->wq[q] * p->Jq[q + (int64_t)Nq * k];
+                for (int cc = 0; cc < Nc; cc++) {
+                    FB(c->f_bar_H, 1, si + N1D * sj, k)[cc] = FB(c->f_bar_H, 1, si + N1D * (sj - 1), k)[cc] + wJ * c->rhsxyH[IDX_XY(1, q, k, Nq, Nd, Nc) + cc];
+                    FB(c->f_bar_L, 1, si + N1D * sj, k)[cc] = FB(c->f_bar_L, 1, si + N1D * (sj - 1), k)[cc] + wJ * c->rhsxyL[IDX_XY(1, q, k, Nq, Nd, Nc) + cc];
+                }
+            }
+        }
+    }
+
+    /* subcell_bound_limiter! subcell.jl:208-246 (1D), :248-349 (2D) */
+    for (int64_t x = 0; x < (int64_t)NLs * Nd * K; x++) c->L_local[x + (int64_t)NLs * Nd * K * s] = 1.0;
+    OMP_FOR
+    for (int k = 0; k < K; k++) {
+        double *uLk = c->uL + IDX_V(0, k, Nq, Nc);
+        for (int i = 0; i < Nq; i++)
+            for (int cc = 0; cc < Nc; cc++) uLk[i * Nc + cc] = Uq[IDX_V(i, k, Nq, Nc) + cc] + dt * c->rhsL[IDX_V(i, k, Nq, Nc) + cc];
+        double P[NC_MAX];
+        if (Nd == 1) {
+            for (int i = 0; i < Nq; i++) {
+                double wJq_i = p->wq[i] * p->Jq[i + (int64_t)Nq * k];
+                double Lphi = c->lbound_s_modified[i + (int64_t)Nq * k];
+                for (int cc = 0; cc < Nc; cc++) P[cc] = -2.0 * dt * (FB(c->f_bar_H, 0, i, k)[cc] - FB(c->f_bar_L, 0, i, k)[cc]) / wJq_i;
+                LLOC(i, 0, k, s) = jmin(LLOC(i, 0, k, s), limiting_param(c, uLk + i * Nc, P, Lphi));
+            }
+            for (int i = 1; i < Nq + 1; i++) {
+                double wJq_im1 = p->wq[i - 1] * p->Jq[(i - 1) + (int64_t)Nq * k];
+                double Lphi = c->lbound_s_modified[(i - 1) + (int64_t)Nq * k];
+                for (int cc = 0; cc < Nc; cc++) P[cc] = 2.0 * dt * (FB(c->f_bar_H, 0, i, k)[cc] - FB(c->f_bar_L, 0, i, k)[cc]) / wJq_im1;
+                LLOC(i, 0, k, s) = jmin(LLOC(i, 0, k, s), limiting_param(c, uLk + (i - 1) * Nc, P, Lphi));
+            }
+            double l_shock = c->blending_factor[k + (int64_t)K * s];
+            for (int i = 0; i < NLs; i++) LLOC(i, 0, k, s) = jmin(LLOC(i, 0, k, s), l_shock);
+            continue;
+        }
+        int N1Dp1 = N1D + 1;
+        for (int sj = 0; sj < N1D; sj++) {
+            for (int si = 0; si < N1D; si++) {           /* left subcell faces */
+                int q = si + N1D * sj;
+                double wJq_i = p->wq[q] * p->Jq[q + (int64_t)Nq * k];
+                double Lphi = c->lbound_s_modified[q + (int64_t)Nq * k];
+                for (int cc = 0; cc < Nc; cc++) P[cc] = -4.0 * dt * (FB(c->f_bar_H, 0, si + N1Dp1 * sj, k)[cc] - FB(c->f_bar_L, 0, si + N1Dp1 * sj, k)[cc]) / wJq_i;
+                int li = si + N1Dp1 * sj;
+                LLOC(li, 0, k, s) = jmin(LLOC(li, 0, k, s), limiting_param(c, uLk + q * Nc, P, Lphi));
+            }
+            for (int si = 1; si < N1Dp1; si++) {         /* right subcell faces */
+                int q = (si - 1) + N1D * sj;
+                double wJq_i = p->wq[q] * p->Jq[q + (int64_t)Nq * k];
+                double Lphi = c->lbound_s_modified[q + (int64_t)Nq * k];
+                for (int cc = 0; cc < Nc; cc++) P[cc] = 4.0 * dt * (FB(c->f_bar_H, 0, si + N1Dp1 * sj, k)[cc] - FB(c->f_bar_L, 0, si + N1Dp1 * sj, k)[cc]) / wJq_i;
+                int li = si + N1Dp1 * sj;
+                LLOC(li, 0, k, s) = jmin(LLOC(li, 0, k, s), limiting_param(c, uLk + q * Nc, P, Lphi));
+            }
+        }
+        for (int si = 0; si < N1D; si++) {
+            for (int sj = 0; sj < N1D; sj++) {           /* bottom subcell faces */
+                int q = si + N1D * sj;
+                double wJq_i = p->wq[q] * p->Jq[q + (int64_t)Nq * k];
+                double Lphi = c->lbound_s_modified[q + (int64_t)Nq * k];
+                for (int cc = 0; cc < Nc; cc++) P[cc] = -4.0 * dt * (FB(c->f_bar_H, 1, si + N1D * sj, k)[cc] - FB(c->f_bar_L, 1, si + N1D * sj, k)[cc]) / wJq_i;
+                int li = si + N1D * sj;
+                LLOC(li, 1, k, s) = jmin(LLOC(li, 1, k, s), limiting_param(c, uLk + q * Nc, P, Lphi));
+            }
+            for (int sj = 1; sj < N1Dp1; sj++) {         /* top subcell faces */
+                int q = si + N1D * (sj - 1);
+                double wJq_i = p->wq[q] * p->Jq[q + (int64_t)Nq * k];
+                double Lphi = c->lbound_s_modified[q + (int64_t)Nq * k];
+                for (int cc = 0; cc < Nc; cc++) P[cc] = 4.0 * dt * (FB(c->f_bar_H, 1, si + N1D * sj, k)[cc] - FB(c->f_bar_L, 1, si + N1D * sj, k)[cc]) / wJq_i;
+                int li = si + N1D * sj;
+                LLOC(li, 1, k, s) = jmin(LLOC(li, 1, k, s), limiting_param(c, uLk + q * Nc, P, Lphi));
+            }
+        }
+        double l_shock = c->blending_factor[k + (int64_t)K * s];
+        for (int x = 0; x < N1Dp1 * N1D; x++) {
+            LLOC(x, 0, k, s) = jmin(LLOC(x, 0, k, s), l_shock);
+            LLOC(x, 1, k, s) = jmin(LLOC(x, 1, k, s), l_shock);
+        }
+    }
+
+    /* enforce_ES_subcell!(PositivityBound) subcell.jl:458-460: nothing */
+
+    /* symmetrize_limiting_parameters! */
+    if (Nd == 1) {
+        /* subcell.jl:405-416: `end` of the first dimension is Nq+N1D, not Nq+1, and the
+         * element index wraps periodically whatever the boundary conditions */
+        for (int k = 0; k < K; k++) {
+            int km1 = (k - 1 + K) % K;
+            double l = jmin(LLOC(0, 0, k, s), LLOC(NLs - 1, 0, km1, s));
+            LLOC(0, 0, k, s) = l;
+            LLOC(NLs - 1, 0, km1, s) = l;
+        }
+    } else {
+        /* subcell.jl:418-456 via subcell_index_P_x/_y (limiter_utils.jl:179-237) */
+        int N1Dp1 = N1D + 1;
+        for (int k = 0; k < K; k++)
+            for (int d = 0; d < 2; d++)
+                for (int i = 0; i < NLs; i++) c->L_copy[i + (int64_t)NLs * (d + 2 * (int64_t)k)] = LLOC(i, d, k, s);
+#define LCOPY(i, d, k) (c->L_copy[(int64_t)(i) + (int64_t)NLs * ((d) + 2 * (int64_t)(k))])
+        OMP_FOR
+        for (int k = 0; k < K; k++) {
+            for (int sj = 0; sj < N1D; sj++) {
+                for (int si = 0; si < N1Dp1; si += N1D) {
+                    int iface = (si == 0) ? sj : sj + N1D;
+                    int64_t m = p->mapP[iface + (int64_t)Nfp * k] - 1;
+                    int64_t iP = m % Nfp, kP = m / Nfp;
+                    int sjP = (int)(iP % N1D);
+                    int siP = (iP / N1D == 0) ? 0 : N1D;
+                    int idx = si + sj * N1Dp1, idxP = siP + sjP * N1Dp1;
+                    LLOC(idx, 0, k, s) = jmin(LCOPY(idx, 0, k), LCOPY(idxP, 0, kP));
+                }
+            }
+            for (int si = 0; si < N1D; si++) {
+                for (int sj = 0; sj < N1Dp1; sj += N1D) {
+                    int iface = (sj == 0) ? si + 2 * N1D : si + 3 * N1D;
+                    int64_t m = p->mapP[iface + (int64_t)Nfp * k] - 1;
+                    int64_t iP = m % Nfp, kP = m / Nfp;
+                    int siP = (int)(iP % N1D);
+                    int sjP = (iP / N1D == 2) ? 0 : N1D;
+                    int idx = si + sj * N1D, idxP = siP + sjP * N1D;
+                    LLOC(idx, 1, k, s) = jmin(LCOPY(idx, 1, k), LCOPY(idxP, 1, kP));
+                }
+            }
+        }
+#undef LCOPY
+    }
+
+    /* accumulate_f_bar_limited! subcell.jl:826-876 */
+    OMP_FOR
+    for (int k = 0; k < K; k++) {
+        if (Nd == 1) {
+            for (int i = 0; i < Nq + 1; i++) {
+                double l = LLOC(i, 0, k, s);
+                for (int cc = 0; cc < Nc; cc++)
+                    FB(c->f_bar_lim, 0, i, k)[cc] = l * FB(c->f_bar_H, 0, i, k)[cc] + (1.0 - l) * FB(c->f_bar_L, 0, i, k)[cc];
+            }
+            continue;
+        }
+        int N1Dp1 = N1D + 1;
+        for (int d = 0; d < 2; d++)
+            for (int x = 0; x < N1Dp1 * N1D; x++) {
+                double l = LLOC(x, d, k, s);
+                for (int cc = 0; cc < Nc; cc++)
+                    FB(c->f_bar_lim, d, x, k)[cc] = l * FB(c->f_bar_H, d, x, k)[cc] + (1.0 - l) * FB(c->f_bar_L, d, x, k)[cc];
+            }
+    }
+    /* apply_subcell_limiter! subcell.jl:878-924 */
+    OMP_FOR
+    for (int k = 0; k < K; k++) {
+        if (Nd == 1) {
+            for (int i = 0; i < Nq; i++) {
+                double wJq_i = p->wq[i] * p->Jq[i + (int64_t)Nq * k];
+                for (int cc = 0; cc < Nc; cc++)
+                    c->rhsU[IDX_V(i, k, Nq, Nc) + cc] = (FB(c->f_bar_lim, 0, i + 1, k)[cc] - FB(c->f_bar_lim, 0, i, k)[cc]) / wJq_i;
+            }
+            continue;
+        }
+        int N1Dp1 = N1D + 1;
+        for (int j = 0; j < N1D; j++)
+            for (int i = 0; i < N1D; i++) {
+                int q = i + N1D * j;
+                double wJq_ij = p->wq[q] * p->Jq[q + (int64_t)Nq * k];
+                double *rxy = c->rhsxyU + IDX_XY(0, q, k, Nq, Nd, Nc);
+                for (int cc = 0; cc < Nc; cc++) {
+                    rxy[cc] = (FB(c->f_bar_lim, 0, (i + 1) + N1Dp1 * j, k)[cc] - FB(c->f_bar_lim, 0, i + N1Dp1 * j, k)[cc]) / wJq_ij;
+                    rxy[Nc + cc] = (FB(c->f_bar_lim, 1, i + N1D * (j + 1), k)[cc] - FB(c->f_bar_lim, 1, i + N1D * j, k)[cc]) / wJq_ij;
+                }
+                for (int cc = 0; cc < Nc; cc++) c->rhsU[IDX_V(q, k, Nq, Nc) + cc] = rxy[cc] + rxy[Nc + cc];
+            }
+    }
+}
+
+/* ------------------------------------------------------------------------------- */
+/* rhs! rhs.jl:5-13, 41-55                                                          */
+/* ------------------------------------------------------------------------------- */
+int oracle_rhs(oracle_ctx *c, const double *Uq, double t, double dt, int nstage, double *dt_out) {
+    if (!c || !Uq || nstage < 1 || nstage > 3) return 1;
+    int nthr = oracle_get_threads();
+    entropy_projection(c, Uq, nthr);
+    double dt_ret = rhs_low_graph_visc(c, Uq, t, dt, nstage, nthr);
+    rhs_fluxdiff(c, Uq, nthr);
+    apply_rhs_limiter(c, Uq, dt, nstage, nthr);
+    if (dt_out) *dt_out = dt_ret;
+    return 0;
+}
+
+/* SSP33! loop body SSPRK33.jl:29-43 */
+int oracle_ssprk3_step(oracle_ctx *c, double *Uq, double *t, double *dt_out) {
+    const oracle_problem *p = &c->p;
+    int64_t n = (int64_t)p->Nq * p->K * p->Nc;
+    double dt = jmin(p->CFL * p->dt0, p->T - *t);
+    memcpy(c->resW, Uq, sizeof(double) * (size_t)n);
+    int err = oracle_rhs(c, Uq, *t, dt, 1, &dt);
+    if (err) return err;
+    for (int64_t x = 0; x < n; x++) Uq[x] = c->resW[x] + dt * c->rhsU[x];
+    if ((err = oracle_rhs(c, Uq, *t, dt, 2, NULL))) return err;
+    for (int64_t x = 0; x < n; x++) c->resZ[x] = Uq[x] + dt * c->rhsU[x];
+    for (int64_t x = 0; x < n; x++) Uq[x] = 3.0 / 4.0 * c->resW[x] + 1.0 / 4.0 * c->resZ[x];
+    if ((err = oracle_rhs(c, Uq, *t, dt, 3, NULL))) return err;
+    for (int64_t x = 0; x < n; x++) c->resZ[x] = Uq[x] + dt * c->rhsU[x];
+    for (int64_t x = 0; x < n; x++) Uq[x] = 1.0 / 3.0 * c->resW[x] + 2.0 / 3.0 * c->resZ[x];
+    *t = *t + dt;
+    if (dt_out) *dt_out = dt;
+    return 0;
+}
+
+int64_t oracle_count_nonpositive(const oracle_ctx *c, const double *Uq, int64_t *first) {
+    const oracle_problem *p = &c->p;
+    int64_t n = (int64_t)p->Nq * p->K, bad = 0, f = -1;
+    for (int64_t x = 0; x < n; x++) {
+        const double *U = Uq + x * p->Nc;
+        double pr = pfun(p->Nd, p->gamma, U);
+        if (!(U[0] > 0.0) || !(pr > 0.0)) {
+            if (f < 0) f = x;
+            bad++;
+        }
+    }
+    if (first) *first = f;
+    return bad;
+}
+
+double oracle_check_conservation(const oracle_ctx *c, const double *Uq) {
+    const oracle_problem *p = &c->p;
+    double tot[NC_MAX] = {0, 0, 0, 0};
+    for (int k = 0; k < p->K; k++)
+        for (int i = 0; i < p->Nq; i++) {
+            double a = p->Jq[i + (int64_t)p->Nq * k] * p->wq[i];
+            for (int cc = 0; cc < p->Nc; cc++) tot[cc] += a * Uq[IDX_V(i, k, p->Nq, p->Nc) + cc];
+        }
+    double s = tot[0];
+    for (int cc = 1; cc < p->Nc; cc++) s += tot[cc];
+    return s;
+}

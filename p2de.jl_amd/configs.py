@@ -1,0 +1,181 @@
+"""The benchmark and test problems, written as the reference's driver scripts write them:
+a ``Param``, an ``initial_condition(param, x[, y])`` and an
+``initial_boundary_conditions(param, md)`` (test/test_smoke.jl:22-36,68-104,117-129).
+
+Problems (γ = 1.4, ζ = 0.1, POSTOL = 1e-14, ZEROTOL = 5e-16 unless stated):
+
+* ``sod_1d``          -- Sod tube on [0,1], K=100, N=3, CFL 0.5, T 0.2, transmissive ends.
+* ``vortex_2d``       -- isentropic vortex on periodic [-5,5]^2, 64x64, N=3, strength 5,
+                         free stream (1,1,1,1), T 1.0.
+* ``riemann4_2d``     -- four-state Riemann problem on [0,1]^2, 256x256, N=3, T 0.25,
+                         transmissive boundary.
+* ``kelvin_helmholtz``-- periodic [-1,1]^2, 512x512, N=4 (examples/2D/kelvin-helmholtz.jl:9-18).
+* ``smoke_vortex``    -- the reference's own smoke problem (test/test_smoke.jl:6-20,117-129).
+
+``dt0`` for the first four is ``0.9 * dt_pos(U0) / CFL`` where ``dt_pos`` is the
+low-order positivity time step of the initial state, recorded once (so the CFL cap
+binds at the first step and the limiter's dt equals the update dt there).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import replace
+from typing import Callable, Dict, Tuple
+
+import numpy as np
+
+from .dg import initialize_DG, make_periodic, primitive_to_conservative
+from .operators import BCData
+from .types import (CompressibleEulerIdealGas, Dim1, Dim2, ESLimitedLowOrderPos, GlobalConstant,
+                    LimitingParameter, LobattoCollocation, NoEntropyProjectionLimiter, Param,
+                    SubcellLimiter, TimesteppingParameter)
+
+# dt0 = 0.9*dt_pos(U0)/CFL, recorded from the initial states (tests/golden/record_dt0.py)
+RECORDED_DT0: Dict[str, float] = {
+    "sod_1d": 0.0003169328455231937,
+    "vortex_2d": 0.0009536460551185905,
+    "riemann4_2d": 3.584113472465112e-05,
+    "kelvin_helmholtz": 2.2148392215926948e-05,
+}
+
+
+def _param(N, K, xL, xR, T, CFL, dt0, Nd, zeta=0.1, eta=1.0):
+    return Param(N=N, K=K, xL=xL, xR=xR,
+                 global_constants=GlobalConstant(POSTOL=1e-14, ZEROTOL=5e-16),
+                 timestepping_param=TimesteppingParameter(T=T, CFL=CFL, dt0=dt0, t0=0.0),
+                 limiting_param=LimitingParameter(zeta=zeta, eta=eta),
+                 equation=CompressibleEulerIdealGas(Dim1() if Nd == 1 else Dim2(), 1.4),
+                 rhs=ESLimitedLowOrderPos(),
+                 approximation_basis=LobattoCollocation(),
+                 entropyproj_limiter=NoEntropyProjectionLimiter(),
+                 rhs_limiter=SubcellLimiter())
+
+
+def _periodic_bc(param, md):
+    md = make_periodic(md)
+    return BCData(mapP=md.mapP), md
+
+
+def _transmissive_bc(param, md):
+    """Outflow (uP = own node, no LF dissipation) on every boundary face node
+    (the mapO pattern of examples/2D/astro-jet.jl:53-55 and examples/1D/shu-osher.jl:23)."""
+    Nfp = md.mapP.shape[0]
+    fidx = np.flatnonzero(md.boundary.T.ravel()) + 1         # 1-based linear (Nfp, K) index
+    return BCData(mapP=md.mapP, mapO=fidx.astype(np.int64))
+
+
+# ------------------------------------------------------------------------------- Sod
+def sod_ic(param, x):
+    g = param.equation.gamma
+    left = x < 0.5
+    rho = np.where(left, 1.0, 0.125)
+    u = np.zeros_like(x)
+    p = np.where(left, 1.0, 0.1)
+    return primitive_to_conservative(g, (rho, u, p))
+
+
+def sod_1d(K=100, N=3, T=0.2, CFL=0.5, dt0=None):
+    dt0 = RECORDED_DT0["sod_1d"] if dt0 is None else dt0
+    return _param(N, K, 0.0, 1.0, T, CFL, dt0, 1), sod_ic, _transmissive_bc
+
+
+# ------------------------------------------------------------------------------- vortex
+def vortex_exact_prim(x, y, t, gamma=1.4, beta=5.0, L=10.0, xc=0.0, yc=0.0):
+    """Isentropic vortex advected with (1, 1) on a periodic box of side L (nearest image)."""
+    dx = x - xc - t
+    dy = y - yc - t
+    dx = dx - L * np.round(dx / L)
+    dy = dy - L * np.round(dy / L)
+    r2 = dx * dx + dy * dy
+    e = np.exp(1.0 - r2)
+    u = 1.0 - beta * e * dy / (2.0 * math.pi)
+    v = 1.0 + beta * e * dx / (2.0 * math.pi)
+    rho = (1.0 - (gamma - 1.0) * beta * beta * np.exp(2.0 * (1.0 - r2)) / (16.0 * gamma * math.pi ** 2)) ** (1.0 / (gamma - 1.0))
+    p = rho ** gamma
+    return rho, u, v, p
+
+
+def vortex_ic(param, x, y):
+    t0 = param.timestepping_param.t0
+    return primitive_to_conservative(param.equation.gamma, vortex_exact_prim(x, y, t0, param.equation.gamma))
+
+
+def vortex_2d(K=64, N=3, T=1.0, CFL=0.5, dt0=None):
+    dt0 = RECORDED_DT0["vortex_2d"] if dt0 is None else dt0
+    return _param(N, (K, K), (-5.0, -5.0), (5.0, 5.0), T, CFL, dt0, 2), vortex_ic, _periodic_bc
+
+
+# ------------------------------------------------------------------------------- 4-state Riemann
+RIEMANN4_STATES = {  # (rho, u, v, p) by quadrant
+    "NE": (1.5, 0.0, 0.0, 1.5),
+    "NW": (0.5323, 1.206, 0.0, 0.3),
+    "SW": (0.138, 1.206, 1.206, 0.029),
+    "SE": (0.5323, 0.0, 1.206, 0.3),
+}
+
+
+def riemann4_ic(param, x, y):
+    right = x > 0.5
+    top = y > 0.5
+    out = [np.zeros_like(x) for _ in range(4)]
+    for key, mask in (("NE", right & top), ("NW", ~right & top), ("SW", ~right & ~top), ("SE", right & ~top)):
+        for c in range(4):
+            out[c] = np.where(mask, RIEMANN4_STATES[key][c], out[c])
+    return primitive_to_conservative(param.equation.gamma, tuple(out))
+
+
+def riemann4_2d(K=256, N=3, T=0.25, CFL=0.5, dt0=None):
+    dt0 = RECORDED_DT0["riemann4_2d"] if dt0 is None else dt0
+    return _param(N, (K, K), (0.0, 0.0), (1.0, 1.0), T, CFL, dt0, 2), riemann4_ic, _transmissive_bc
+
+
+# ------------------------------------------------------------------------------- Kelvin-Helmholtz
+def kh_ic(param, x, y):
+    B = np.tanh(15.0 * y + 7.5) - np.tanh(15.0 * y - 7.5)
+    rho = 0.5 + 0.75 * B
+    u = 0.5 * (B - 1.0)
+    v = 0.1 * np.sin(2.0 * math.pi * x)
+    p = np.ones_like(x)
+    return primitive_to_conservative(param.equation.gamma, (rho, u, v, p))
+
+
+def kelvin_helmholtz(K=512, N=4, T=10.0, CFL=0.5, dt0=None):
+    dt0 = RECORDED_DT0["kelvin_helmholtz"] if dt0 is None else dt0
+    return _param(N, (K, K), (-1.0, -1.0), (1.0, 1.0), T, CFL, dt0, 2), kh_ic, _periodic_bc
+
+
+# ------------------------------------------------------------------------------- reference smoke
+def smoke_exact_prim(x, y, t, gamma=1.4):
+    """test/test_smoke.jl:6-20."""
+    x0, y0, beta = 4.5, 5.0, 8.5
+    r2 = (x - x0 - t) ** 2 + (y - y0) ** 2
+    u = 1 - beta * np.exp(1 - r2) * (y - y0) / (2 * math.pi)
+    v = beta * np.exp(1 - r2) * (x - x0 - t) / (2 * math.pi)
+    rho = 1 - (1 / (8 * gamma * math.pi ** 2)) * (gamma - 1) / 2 * (beta * np.exp(1 - r2)) ** 2
+    rho = rho ** (1 / (gamma - 1))
+    p = rho ** gamma
+    return rho, u, v, p
+
+
+def smoke_ic(param, x, y):
+    return primitive_to_conservative(param.equation.gamma, smoke_exact_prim(x, y, param.timestepping_param.t0))
+
+
+def smoke_vortex(N=3, K=5, T=2e-2, CFL=1.0, dt0=1e-2):
+    """test/test_smoke.jl:117-129 with SubcellLimiter(PositivityBound)."""
+    return _param(N, (K, K), (0.0, 0.0), (10.0, 10.0), T, CFL, dt0, 2, zeta=0.1, eta=0.5), smoke_ic, _periodic_bc
+
+
+CONFIGS: Dict[str, Callable] = {
+    "sod_1d": sod_1d,
+    "vortex_2d": vortex_2d,
+    "riemann4_2d": riemann4_2d,
+    "kelvin_helmholtz": kelvin_helmholtz,
+    "smoke_vortex": smoke_vortex,
+}
+
+
+def build(name: str, **kw):
+    """Return ``(solver, state, state_param)`` for a named problem."""
+    param, ic, bc = CONFIGS[name](**kw)
+    return initialize_DG(param, ic, bc)

@@ -1,0 +1,135 @@
+// p2de_internal.h -- data shared between the C-ABI layer (p2de_capi.cpp) and the kernels
+// (rhs_kernels.hip).
+//
+// Device data layout (HBM):
+//   state      double[K][Nq][Nc]           Julia's Array{SVector{Nc},2}(Nq, K)
+//   rhsUA      double[K][Nq][Nc]           element RHS with pre-symmetrisation limiter
+//   lA         double[K][Nfp]              limiter parameter of each element-boundary
+//                                          subcell face (one per face node)
+//   dfA        double[K][Nfp][Nc]          f_bar_H - f_bar_L at those faces
+//   geo        Geo[K]                      per-element metric (constant inside an element)
+//   nb_node    int32[K][Nfp]               kP*Nq + fq2q[iP] of the mapP target
+//   nb_face    int32[K][Nfp]               kP*Nfp + iP of the mapP target
+//   bc         int32[K][Nfp]               0 interior, -1 outflow, m+1 inflow (Ival row m)
+//   tables     ElemTables                  operator tables shared by all elements
+#pragma once
+#include <stdint.h>
+
+namespace p2de {
+
+constexpr int MAX_N1D = 5;
+constexpr int MAX_NQ = 25;
+constexpr int MAX_NFP = 20;
+constexpr int MAX_NH = 45;
+constexpr int MAX_NPH = 120;
+constexpr int MAX_NPL = 40;
+
+// Per-element metric terms.  The reference stores them per node (GeomData, Solver.jl:185-190);
+// on its affine uniform meshes they are constant inside an element, which p2de_create checks.
+struct Geo {
+    double J, rxJ, sxJ, ryJ, syJ;
+};
+
+// Operator tables of the reference element, built once from the operators the caller
+// passes (Operators, Solver.jl:192-212).  Indices are 0-based.
+struct ElemTables {
+    int Nd, Nc, N1D, Nq, Nfp, Nh, nph, npl;
+    // high-order pairs, Srsh_nnz order (init.jl:188-195): hybrid nodes i > j, the metric
+    // direction the pair acts in, and Srh_db[i,j], Ssh_db[i,j]
+    uint8_t hp_i[MAX_NPH], hp_j[MAX_NPH], hp_dir[MAX_NPH];
+    double hp_cr[MAX_NPH], hp_cs[MAX_NPH];
+    // low-order pairs, Srs0_nnz order (init.jl:197-204): Sr0[i,j], Ss0[i,j]
+    uint8_t lp_i[MAX_NPL], lp_j[MAX_NPL], lp_dir[MAX_NPL];
+    double lp_cr[MAX_NPL], lp_cs[MAX_NPL];
+    // QF1 gather lists per hybrid node, in pair order: pair index and sign (+1 if the
+    // node is the pair's i, -1 if it is j)
+    int16_t hl_start[MAX_NH + 1];
+    int16_t hl_pair[2 * MAX_NPH];
+    int8_t hl_sign[2 * MAX_NPH];
+    // Q0F1 gather lists per volume node, in pair order
+    int16_t ll_start[MAX_NQ + 1];
+    int16_t ll_pair[2 * MAX_NPL];
+    int8_t ll_sign[2 * MAX_NPL];
+    // CFL lists: the low-order pairs of node i ordered by partner index j, so
+    // sum_j lambda[i,j] adds in the reference's j order (low_order_graph_viscosity.jl:271-273)
+    int16_t cl_start[MAX_NQ + 1];
+    int16_t cl_pair[2 * MAX_NPL];
+    // node data
+    double wq[MAX_NQ];
+    double a_v[MAX_NQ];           // MinvVhT[i, i]
+    int8_t nf[MAX_NQ];            // faces f with fq2q[f] == i (ascending)
+    int8_t nface[MAX_NQ][2];
+    double a_h[MAX_NQ][2];        // MinvVhT[i, Nq+f]
+    double a_f[MAX_NQ][2];        // MinvVfT[i, f]
+    int8_t nq2fq[MAX_NQ];         // q2fq[i] (CFL face order)
+    int8_t q2fq[MAX_NQ][2];
+    // face data
+    uint8_t fq2q[MAX_NFP];
+    double vf_coef[MAX_NFP];      // (1*Vf + 0*Vf_low)[f, fq2q[f]], the face row's only nonzero
+    double Br[MAX_NFP], Bs[MAX_NFP];
+    int8_t ld[MAX_NFP];           // direction the LF term is applied to (rhs_utils.jl:78-102)
+    int8_t bsign[MAX_NFP];        // -1: left/bottom element-boundary subcell face, +1 right/top
+};
+
+// Device-side time-stepping scalars.
+struct StepScalars {
+    double t;
+    double dt_lim;                // dt the limiter uses (time_param.dt)
+    unsigned long long dt_bits;   // CFL dt accumulator (positive double bit pattern), then the step dt
+    int done;                     // t >= T: kernels of the step do nothing
+    int steps;
+    unsigned long long viol;      // positivity violations
+    unsigned long long first_bad; // min over (stage<<40 | node) of offending nodes
+};
+
+struct PhaseAArgs {
+    const double* U;
+    const ElemTables* T;
+    const Geo* geo;
+    const int* nb_node;
+    const int* nb_face;
+    const int* bc;
+    const double* Ival;
+    StepScalars* sc;
+    int K;
+    double gamma, zeta, ZEROTOL, CFL;
+    int upper_mode;
+    int stage1;                   // accumulate the CFL dt (nstage == 1)
+    double* rhsUA;
+    double* lA;
+    double* dfA;
+    double* dbg_rhsH;             // optional debug outputs (nullptr: off)
+    double* dbg_rhsL;
+    double* dbg_L;                // L_local[:, :, :] of this stage, (Nq+N1D, Nd, K)
+};
+
+enum { MODE_RHS = 0, MODE_STAGE1 = 1, MODE_STAGE2 = 2, MODE_STAGE3 = 3 };
+
+struct PhaseBArgs {
+    const double* rhsUA;
+    const double* lA;
+    const double* dfA;
+    const int* nb_face;
+    const ElemTables* T;
+    const Geo* geo;
+    StepScalars* sc;
+    int K;
+    double gamma;
+    int mode;
+    const double* W;              // state at the start of the step (stage 2/3), or the input (stage 1)
+    const double* Uin;            // stage input
+    double* Uout;                 // stage output (may alias Uin or W elementwise)
+    double* rhsU;                 // MODE_RHS output
+    double* dbg_L;
+    int finalize;                 // stage 3: advance t
+};
+
+// launchers (rhs_kernels.hip); return hipError_t as int
+int launch_phaseA(int Nd, int N1D, const PhaseAArgs& a, void* stream);
+int launch_phaseB(int Nd, int N1D, const PhaseBArgs& a, void* stream);
+int launch_step_init(StepScalars* sc, double CFL, double dt0, double T, void* stream);
+int launch_rhs_init(StepScalars* sc, double dt_lim, double dt_cap, void* stream);
+int launch_positivity(const double* U, int64_t nodes, int Nd, double gamma, unsigned long long* keys,
+                      unsigned long long* cnt, void* stream);
+
+}  // namespace p2de

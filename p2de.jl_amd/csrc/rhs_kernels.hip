@@ -1,0 +1,722 @@
+// rhs_kernels.hip -- FP64 kernels of the limited ESDG right-hand side and the SSP-RK3
+// stage update for gfx950.
+//
+// One stage = two launches:
+//   phase A  (element-local; a workgroup owns E elements, node data staged in LDS)
+//     entropy projection of the face nodes, low-order graph-viscosity RHS, high-order
+//     flux-differencing RHS, subcell flux potentials and the limiter parameters of every
+//     subcell face with the element's own data; writes the element RHS plus, for each
+//     element-boundary subcell face, its parameter and f_bar_H - f_bar_L; at stage 1 it
+//     also reduces the CFL time step (global atomic min).
+//   phase B  (node-parallel) symmetrises the element-boundary parameters with the
+//     neighbour (min), corrects the adjacent nodes' RHS, applies the RK stage and checks
+//     positivity of the new state.
+// The reference runs the same work as ~27 threaded loops per stage over HBM-sized
+// intermediates (src/dg/rhs/*.jl, src/dg/limiter/subcell.jl); the formulas and their
+// operation order are kept (built with -ffp-contract=off) so the FP64 result matches the
+// reference's arithmetic.
+#include <hip/hip_runtime.h>
+
+#include "euler_device.h"
+#include "p2de_internal.h"
+
+namespace p2de {
+
+template <int DIM, int N1D>
+struct Sz {
+    static constexpr int NQ = DIM == 1 ? N1D : N1D * N1D;
+    static constexpr int NFP = DIM == 1 ? 2 : 4 * N1D;
+    static constexpr int NH = NQ + NFP;
+    static constexpr int NC = DIM + 2;
+    static constexpr int NPH = DIM == 1 ? N1D * (N1D - 1) / 2 + 2 : N1D * N1D * (N1D - 1) + NFP;
+    static constexpr int NPL = DIM == 1 ? N1D - 1 : 2 * N1D * (N1D - 1);
+    static constexpr int NLINE = DIM == 1 ? 1 : 2 * N1D;
+    static constexpr int NLS = NQ + N1D;  // first dimension of L_local (State.jl:21)
+};
+
+// elements per workgroup and workgroup size
+template <int DIM, int N1D> struct Cfg;
+template <> struct Cfg<1, 2> { static constexpr int E = 32, NT = 64; };
+template <> struct Cfg<1, 3> { static constexpr int E = 21, NT = 64; };
+template <> struct Cfg<1, 4> { static constexpr int E = 16, NT = 64; };
+template <> struct Cfg<1, 5> { static constexpr int E = 12, NT = 64; };
+template <> struct Cfg<2, 2> { static constexpr int E = 16, NT = 64; };
+template <> struct Cfg<2, 3> { static constexpr int E = 7, NT = 64; };
+template <> struct Cfg<2, 4> { static constexpr int E = 4, NT = 64; };
+template <> struct Cfg<2, 5> { static constexpr int E = 2, NT = 64; };
+
+__device__ __forceinline__ double bits_to_double(unsigned long long b) { return __longlong_as_double((long long)b); }
+__device__ __forceinline__ unsigned long long double_to_bits(double d) { return (unsigned long long)__double_as_longlong(d); }
+
+// Bx / Bx_with_n (rhs_utils.jl:13-37): physical boundary normal at a face node, its norm and
+// the unit normal.  The reference's 1D method passes its arguments in the wrong order
+// (rhs_utils.jl:18); this is its evident intent rxJh * Br[i,i].
+template <int DIM>
+__device__ __forceinline__ void face_normal(const Geo& G, double Br, double Bs, double* B, double& nrm, double* n) {
+    if (DIM == 1) {
+        B[0] = G.rxJ * Br;
+        B[1] = 0.0;
+        nrm = fabs(B[0]);
+        n[0] = B[0] / nrm;
+        n[1] = 0.0;
+    } else {
+        B[0] = G.rxJ * Br + G.sxJ * Bs;
+        B[1] = G.ryJ * Br + G.syJ * Bs;
+        nrm = jnorm2(B[0], B[1]);
+        n[0] = B[0] / nrm;
+        n[1] = B[1] / nrm;
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ void make_prim(double g, const double* u, Prim<DIM>& P) {
+    // U() of flux_differencing.jl:179-191 on top of calculate_primitive_variables! (:39-52)
+    P.rho = u[0];
+    P.u = u[1] / u[0];
+    P.v = DIM == 2 ? u[2] / u[0] : 0.0;
+    double pp = Eq<DIM>::p(g, u);
+    P.beta = u[0] / (2.0 * pp);
+    P.lrho = log(u[0]);
+    P.lbeta = log(P.beta);
+}
+
+// rhoe_quadratic_solve / _coefficients (limiter_utils.jl:108-146)
+template <int DIM>
+__device__ __forceinline__ double rhoe_quadratic_solve(const double* U, const double* P, double Lrhoe, double ZEROTOL) {
+    if (Lrhoe == INFINITY) return 1.0;
+    double a, b, c;
+    if (DIM == 1) {
+        a = P[0] * P[2] - 1.0 / 2.0 * (P[1] * P[1]);
+        b = U[2] * P[0] + U[0] * P[2] - U[1] * P[1] - P[0] * Lrhoe;
+        c = U[2] * U[0] - 1.0 / 2.0 * (U[1] * U[1]) - U[0] * Lrhoe;
+    } else {
+        a = P[0] * P[3] - 1.0 / 2.0 * (P[1] * P[1] + P[2] * P[2]);
+        b = U[3] * P[0] + U[0] * P[3] - U[1] * P[1] - U[2] * P[2] - P[0] * Lrhoe;
+        c = U[3] * U[0] - 1.0 / 2.0 * (U[1] * U[1] + U[2] * U[2]) - U[0] * Lrhoe;
+    }
+    double l = 1.0;
+    if (b * b - 4.0 * a * c >= 0.0) {
+        double r1 = (-b + sqrt(b * b - 4.0 * a * c)) / (2.0 * a);
+        double r2 = (-b - sqrt(b * b - 4.0 * a * c)) / (2.0 * a);
+        if (r1 > ZEROTOL && r2 > ZEROTOL) {
+            l = jmin(r1, r2);
+        } else if (r1 > ZEROTOL && r2 < -ZEROTOL) {
+            l = r1;
+        } else if (r2 > ZEROTOL && r1 < -ZEROTOL) {
+            l = r2;
+        }
+    }
+    return l;
+}
+
+// limiting_param(::SubcellLimiter, ::PositivityBound) (limiter_utils.jl:66-70, 82-96) with the
+// bound tuple of subcell.jl:299 = (zeta*rho_L, zeta*rhoe(u_L), Lphi = 0, Inf, Inf).
+// upper_mode 0 reproduces the reference's 4-of-5 destructuring (Urho = Lphi = 0).
+template <int DIM>
+__device__ __forceinline__ double limiting_param(const double* uL, const double* P, double zeta, double ZEROTOL,
+                                                 int upper_mode) {
+    double Lrho = zeta * uL[0];
+    double Lrhoe = zeta * Eq<DIM>::rhoe(uL);
+    double Urho = upper_mode == 0 ? 0.0 : INFINITY;
+    double Urhoe = INFINITY;
+    double l = 1.0;
+    if (uL[0] + P[0] < Lrho) l = jmax((Lrho - uL[0]) / P[0], 0.0);
+    if (uL[0] + P[0] > Urho) l = jmin(l, jmax((Urho - uL[0]) / P[0], 0.0));
+    l = jmin(jmin(l, rhoe_quadratic_solve<DIM>(uL, P, Lrhoe, ZEROTOL)), rhoe_quadratic_solve<DIM>(uL, P, Urhoe, ZEROTOL));
+    return l;
+}
+
+template <int DIM, int N1D, int E>
+struct SmemA {
+    using S = Sz<DIM, N1D>;
+    double U[E][S::NQ][S::NC];
+    Prim<DIM> np[E][S::NQ];
+    double nF[E][S::NQ][DIM * S::NC];
+    Prim<DIM> fp[E][S::NFP];
+    double BFH[E][S::NFP][S::NC];
+    double BFL[E][S::NFP][S::NC];
+    double lamB[E][S::NFP];
+    union {
+        struct {
+            double PH[E][S::NPH][S::NC];
+            double PL[E][S::NPL][S::NC + 1];
+        } p;
+        double RU[E][S::NQ][DIM][S::NC];
+    } x;
+    double RH[E][S::NQ][DIM][S::NC];
+    double RL[E][S::NQ][DIM][S::NC];
+    double UL[E][S::NQ][S::NC];
+    double red[64];
+};
+
+template <int DIM, int N1D, int E, int NT>
+__global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
+    using S = Sz<DIM, N1D>;
+    constexpr int NQ = S::NQ, NFP = S::NFP, NC = S::NC, NPH = S::NPH, NPL = S::NPL;
+    __shared__ SmemA<DIM, N1D, E> sm;
+    if (a.sc->done) return;
+    const int tid = threadIdx.x;
+    const int k0 = blockIdx.x * E;
+    const int K = a.K;
+    const double g = a.gamma;
+    const ElemTables& T = *a.T;
+
+    // ---- phase 0: element states -> LDS (coalesced)
+    {
+        const int nel = min(E, K - k0);
+        const double* src = a.U + (size_t)k0 * NQ * NC;
+        double* dst = &sm.U[0][0][0];
+        for (int w = tid; w < nel * NQ * NC; w += NT) dst[w] = src[w];
+    }
+    __syncthreads();
+
+    // ---- phase 1: node primitives and physical fluxes
+    for (int w = tid; w < E * NQ; w += NT) {
+        const int e = w / NQ, i = w - e * NQ;
+        if (k0 + e >= K) continue;
+        const double* u = sm.U[e][i];
+        make_prim<DIM>(g, u, sm.np[e][i]);          // u_tilde = Uq at volume nodes (rhs.jl:79-82)
+        Eq<DIM>::flux(g, u, sm.nF[e][i]);           // update_wavespeed_and_inviscid_flux! :79-82
+    }
+
+    // ---- phase 2: face nodes (entropy projection, LF coefficients, both surface fluxes)
+    for (int w = tid; w < E * NFP; w += NT) {
+        const int e = w / NFP, f = w - e * NFP;
+        const int k = k0 + e;
+        if (k >= K) continue;
+        const int q = T.fq2q[f];
+        const double* Uq = sm.U[e][q];
+        const Geo G = a.geo[k];
+        double B[2], nrm, n[2];
+        face_normal<DIM>(G, T.Br[f], T.Bs[f], B, nrm, n);
+
+        // entropy_projection_face_node! (rhs.jl:84-94): v_tilde = sum_j (1*Vf + 0*Vf_low)[f,j] vq_j
+        double V[NC], ut[NC];
+        Eq<DIM>::v_of_u(g, Uq, V);
+        for (int c = 0; c < NC; c++) V[c] = 0.0 + T.vf_coef[f] * V[c];
+        Eq<DIM>::u_of_v(g, V, ut);
+        make_prim<DIM>(g, ut, sm.fp[e][f]);
+
+        // mapP target (calculate_primitive_variables! :54-69; get_uP_and_enforce_BC! :101-108)
+        const int fi = k * NFP + f;
+        const int nbn = a.nb_node[fi], nbf = a.nb_face[fi];
+        const int kP = nbf / NFP, iP = nbf - kP * NFP;
+        double Ut[NC];
+        for (int c = 0; c < NC; c++) Ut[c] = a.U[(size_t)nbn * NC + c];
+        const Geo GP = a.geo[kP];
+        double BP[2], nrmP, nP[2];
+        face_normal<DIM>(GP, T.Br[iP], T.Bs[iP], BP, nrmP, nP);
+        double VP[NC], utP[NC];
+        Eq<DIM>::v_of_u(g, Ut, VP);
+        for (int c = 0; c < NC; c++) VP[c] = 0.0 + T.vf_coef[iP] * VP[c];
+        Eq<DIM>::u_of_v(g, VP, utP);
+
+        // calculate_interface_dissipation_coeff! (flux_differencing.jl:90-114) + enforce_BC! (:116-151)
+        const double lam = Eq<DIM>::wavespeed(g, ut, n);
+        const double lamP = Eq<DIM>::wavespeed(g, utP, nP);
+        double LFc = 0.5 * nrm;
+        LFc = LFc * jmax(lam, lamP);
+        const int bcc = a.bc[fi];
+        double uPH[NC], uPL[NC];
+        for (int c = 0; c < NC; c++) {
+            if (bcc == 0) {
+                uPH[c] = utP[c];
+                uPL[c] = Ut[c];
+            } else if (bcc < 0) {
+                uPH[c] = Uq[c];
+                uPL[c] = Uq[c];
+            } else {
+                uPH[c] = a.Ival[(bcc - 1) * NC + c];
+                uPL[c] = a.Ival[(bcc - 1) * NC + c];
+            }
+        }
+        if (bcc != 0) LFc = 0.0;
+
+        // accumulate_numerical_flux! with LaxFriedrichsOnProjectedVal (:231-247, :262-272)
+        const int ld = T.ld[f];
+        double Ff[DIM * NC], FP[DIM * NC];
+        Eq<DIM>::flux(g, ut, Ff);
+        Eq<DIM>::flux(g, uPH, FP);
+        for (int c = 0; c < NC; c++) {
+            double fs = 0.5 * (Ff[ld * NC + c] + FP[ld * NC + c]);
+            double BF = B[ld] * fs;
+            double lf = LFc * (uPH[c] - ut[c]);
+            sm.BFH[e][f][c] = BF - lf;
+        }
+
+        // low-order surface term (low_order_graph_viscosity.jl:51-61, 85-91, 184-198)
+        const double wsf = Eq<DIM>::wavespeed(g, Uq, n);
+        const double wsP = Eq<DIM>::wavespeed(g, Ut, nP);
+        const double lB = 0.5 * nrm * jmax(wsf, wsP);
+        Eq<DIM>::flux(g, uPL, FP);
+        for (int c = 0; c < NC; c++) {
+            double fs = 0.5 * (sm.nF[e][q][ld * NC + c] + FP[ld * NC + c]);
+            double BF = B[ld] * fs;
+            double lf = lB * (uPL[c] - Uq[c]);
+            sm.BFL[e][f][c] = BF - lf;
+        }
+        sm.lamB[e][f] = lB;
+    }
+    __syncthreads();
+
+    // ---- phase 3a: high-order volume pairs (flux_differencing_volume! :164-176, accumulate_QF1! :201-211)
+    for (int w = tid; w < E * NPH; w += NT) {
+        const int e = w / NPH, p = w - e * NPH;
+        const int k = k0 + e;
+        if (k >= K) continue;
+        const int i = T.hp_i[p], j = T.hp_j[p], dir = T.hp_dir[p];
+        const Prim<DIM>& Pi = i < NQ ? sm.np[e][i] : sm.fp[e][i - NQ];
+        const Prim<DIM>& Pj = j < NQ ? sm.np[e][j] : sm.fp[e][j - NQ];
+        const Geo G = a.geo[k];
+        double Sd;
+        if (DIM == 1) Sd = G.rxJ * T.hp_cr[p];
+        else Sd = dir == 0 ? G.rxJ * T.hp_cr[p] + G.sxJ * T.hp_cs[p] : G.ryJ * T.hp_cr[p] + G.syJ * T.hp_cs[p];
+        double F[NC];
+        fS_dir<DIM>(g, Pi, Pj, dir, F);
+        for (int c = 0; c < NC; c++) sm.x.p.PH[e][p][c] = Sd * F[c];
+    }
+    // ---- phase 3b: low-order volume pairs (accumulate_low_order_rhs_volume! :139-166, graph_viscosity rhs_utils.jl:237-261)
+    for (int w = tid; w < E * NPL; w += NT) {
+        const int e = w / NPL, p = w - e * NPL;
+        const int k = k0 + e;
+        if (k >= K) continue;
+        const int i = T.lp_i[p], j = T.lp_j[p], dir = T.lp_dir[p];
+        const Geo G = a.geo[k];
+        double S0[2], nrm, nij[2], nji[2];
+        if (DIM == 1) {
+            S0[0] = G.rxJ * T.lp_cr[p];
+            S0[1] = 0.0;
+            nrm = fabs(S0[0]);
+        } else {
+            S0[0] = G.rxJ * T.lp_cr[p] + G.sxJ * T.lp_cs[p];
+            S0[1] = G.ryJ * T.lp_cr[p] + G.syJ * T.lp_cs[p];
+            nrm = jnorm2(S0[0], S0[1]);
+        }
+        nij[0] = S0[0] / nrm;
+        nij[1] = S0[1] / nrm;
+        nji[0] = -nij[0];
+        nji[1] = -nij[1];
+        const double* ui = sm.U[e][i];
+        const double* uj = sm.U[e][j];
+        const double ws = jmax(Eq<DIM>::wavespeed(g, ui, nij), Eq<DIM>::wavespeed(g, uj, nji));
+        const double lam = nrm * ws;
+        for (int c = 0; c < NC; c++) {
+            double Favg = 0.5 * (sm.nF[e][i][dir * NC + c] + sm.nF[e][j][dir * NC + c]);
+            double visc = lam * (uj[c] - ui[c]);
+            sm.x.p.PL[e][p][c] = 2.0 * S0[dir] * Favg - visc;
+        }
+        sm.x.p.PL[e][p][NC] = lam;
+    }
+    __syncthreads();
+
+    // ---- phase 4: node assembly of rhsxyH, rhsxyL, CFL and u_L
+    double dtmin = INFINITY;
+    const double dt_lim = a.sc->dt_lim;
+    for (int w = tid; w < E * NQ; w += NT) {
+        const int e = w / NQ, i = w - e * NQ;
+        const int k = k0 + e;
+        if (k >= K) continue;
+        const Geo G = a.geo[k];
+        // QF1 of hybrid node i and of the face hybrid nodes at i, in pair order
+        double Q[DIM][NC], A[DIM][NC], Bv[DIM][NC];
+        for (int d = 0; d < DIM; d++)
+            for (int c = 0; c < NC; c++) { Q[d][c] = 0.0; Bv[d][c] = 0.0; }
+        for (int m = T.hl_start[i]; m < T.hl_start[i + 1]; m++) {
+            const int p = T.hl_pair[m], d = T.hp_dir[p];
+            if (T.hl_sign[m] > 0)
+                for (int c = 0; c < NC; c++) Q[d][c] += sm.x.p.PH[e][p][c];
+            else
+                for (int c = 0; c < NC; c++) Q[d][c] += -sm.x.p.PH[e][p][c];
+        }
+        // assemble_rhs! (:331-361): generic mul! adds column by column from zero
+        const double av = T.a_v[i];
+        for (int d = 0; d < DIM; d++)
+            for (int c = 0; c < NC; c++) A[d][c] = 0.0 + av * Q[d][c];
+        for (int m = 0; m < T.nf[i]; m++) {
+            const int f = T.nface[i][m], h = NQ + f;
+            double Qf[DIM][NC];
+            for (int d = 0; d < DIM; d++)
+                for (int c = 0; c < NC; c++) Qf[d][c] = 0.0;
+            for (int mm = T.hl_start[h]; mm < T.hl_start[h + 1]; mm++) {
+                const int p = T.hl_pair[mm], d = T.hp_dir[p];
+                if (T.hl_sign[mm] > 0)
+                    for (int c = 0; c < NC; c++) Qf[d][c] += sm.x.p.PH[e][p][c];
+                else
+                    for (int c = 0; c < NC; c++) Qf[d][c] += -sm.x.p.PH[e][p][c];
+            }
+            const double ah = T.a_h[i][m];
+            for (int d = 0; d < DIM; d++)
+                for (int c = 0; c < NC; c++) A[d][c] += ah * Qf[d][c];
+            const int ld = T.ld[f];
+            const double af = T.a_f[i][m];
+            for (int c = 0; c < NC; c++) Bv[ld][c] += af * sm.BFH[e][f][c];
+        }
+        const double J = G.J;
+        double rH[DIM][NC];
+        for (int d = 0; d < DIM; d++)
+            for (int c = 0; c < NC; c++) rH[d][c] = -(A[d][c] + Bv[d][c]) / J;
+
+        // low order: rhsxyL = 0 - Q0F1, then -= BF_L at the face nodes, then / (Jq*wq)
+        double rL[DIM][NC];
+        for (int d = 0; d < DIM; d++)
+            for (int c = 0; c < NC; c++) Q[d][c] = 0.0;
+        for (int m = T.ll_start[i]; m < T.ll_start[i + 1]; m++) {
+            const int p = T.ll_pair[m], d = T.lp_dir[p];
+            if (T.ll_sign[m] > 0)
+                for (int c = 0; c < NC; c++) Q[d][c] += sm.x.p.PL[e][p][c];
+            else
+                for (int c = 0; c < NC; c++) Q[d][c] += -sm.x.p.PL[e][p][c];
+        }
+        for (int d = 0; d < DIM; d++)
+            for (int c = 0; c < NC; c++) rL[d][c] = 0.0 - Q[d][c];
+        for (int m = 0; m < T.nf[i]; m++) {
+            const int f = T.nface[i][m], ld = T.ld[f];
+            for (int c = 0; c < NC; c++) rL[ld][c] = rL[ld][c] - sm.BFL[e][f][c];
+        }
+        const double wJ = J * T.wq[i];
+        for (int d = 0; d < DIM; d++)
+            for (int c = 0; c < NC; c++) rL[d][c] = rL[d][c] / wJ;
+
+        // CFL dt (calculate_lambda_and_low_order_CFL! :222-243, lambda_i :263-281)
+        if (a.stage1) {
+            double li = 0.0;
+            for (int m = T.cl_start[i]; m < T.cl_start[i + 1]; m++) li += sm.x.p.PL[e][T.cl_pair[m]][NC];
+            for (int m = 0; m < T.nq2fq[i]; m++) li += sm.lamB[e][T.q2fq[i][m]];
+            const double dti = a.CFL * 0.5 * wJ / li;
+            dtmin = dti < dtmin ? dti : dtmin;
+        }
+        double rhsL[NC];
+        for (int c = 0; c < NC; c++) {
+            double s = rL[0][c];
+            if (DIM == 2) s = s + rL[DIM - 1][c];
+            rhsL[c] = s;
+        }
+        // u_L = Uq + dt*rhsL (subcell.jl:269)
+        for (int c = 0; c < NC; c++) sm.UL[e][i][c] = sm.U[e][i][c] + dt_lim * rhsL[c];
+        for (int d = 0; d < DIM; d++)
+            for (int c = 0; c < NC; c++) {
+                sm.RH[e][i][d][c] = rH[d][c];
+                sm.RL[e][i][d][c] = rL[d][c];
+            }
+        if (a.dbg_rhsH) {
+            for (int c = 0; c < NC; c++) {
+                double s = rH[0][c];
+                if (DIM == 2) s = s + rH[DIM - 1][c];
+                a.dbg_rhsH[((size_t)k * NQ + i) * NC + c] = s;
+                a.dbg_rhsL[((size_t)k * NQ + i) * NC + c] = rhsL[c];
+            }
+        }
+    }
+    if (a.stage1) {
+        // block min, then one global atomic min on the (positive) bit pattern
+        for (int off = 32; off > 0; off >>= 1) {
+            double o = __shfl_xor(dtmin, off, 64);
+            dtmin = o < dtmin ? o : dtmin;
+        }
+        if ((tid & 63) == 0) sm.red[tid >> 6] = dtmin;
+    }
+    __syncthreads();
+    if (a.stage1 && tid == 0) {
+        double m = sm.red[0];
+        for (int x = 1; x < NT / 64; x++) m = sm.red[x] < m ? sm.red[x] : m;
+        atomicMin(&a.sc->dt_bits, double_to_bits(m));
+    }
+
+    // ---- phase 5: subcell lines: f_bar (subcell.jl:163-206), limiter (:248-349),
+    //      f_bar_lim (:841-876) and the line part of rhsU (:894-924)
+    const double zeta = a.zeta, ZEROTOL = a.ZEROTOL;
+    for (int w = tid; w < E * S::NLINE; w += NT) {
+        const int e = w / S::NLINE, line = w - e * S::NLINE;
+        const int k = k0 + e;
+        if (k >= K) continue;
+        const Geo G = a.geo[k];
+        const int d = (DIM == 1 || line < N1D) ? 0 : 1;
+        const int o = d == 0 ? line : line - N1D;    // sj for x lines, si for y lines
+        const double fac = 2.0 * DIM;                 // 2 in 1D, 4 in 2D
+        double fH[NC], fL[NC], fprev[NC];
+        const int f0 = DIM == 1 ? 0 : (d == 0 ? o : 2 * N1D + o);
+        for (int c = 0; c < NC; c++) {
+            fH[c] = sm.BFH[e][f0][c];
+            fL[c] = sm.BFL[e][f0][c];
+        }
+        for (int s = 0; s <= N1D; s++) {
+            if (s > 0) {
+                const int q = DIM == 1 ? s - 1 : (d == 0 ? (s - 1) + N1D * o : o + N1D * (s - 1));
+                const double wJ = T.wq[q] * G.J;
+                for (int c = 0; c < NC; c++) {
+                    fH[c] = fH[c] + wJ * sm.RH[e][q][d][c];
+                    fL[c] = fL[c] + wJ * sm.RL[e][q][d][c];
+                }
+            }
+            double dfb[NC], P[NC];
+            for (int c = 0; c < NC; c++) dfb[c] = fH[c] - fL[c];
+            double l = 1.0;
+            if (s < N1D) {   // node on the right of (above) the subcell face
+                const int q = DIM == 1 ? s : (d == 0 ? s + N1D * o : o + N1D * s);
+                const double wJ = T.wq[q] * G.J;
+                for (int c = 0; c < NC; c++) P[c] = -fac * dt_lim * dfb[c] / wJ;
+                l = jmin(l, limiting_param<DIM>(sm.UL[e][q], P, zeta, ZEROTOL, a.upper_mode));
+            }
+            if (s > 0) {     // node on the left of (below) the subcell face
+                const int q = DIM == 1 ? s - 1 : (d == 0 ? (s - 1) + N1D * o : o + N1D * (s - 1));
+                const double wJ = T.wq[q] * G.J;
+                for (int c = 0; c < NC; c++) P[c] = fac * dt_lim * dfb[c] / wJ;
+                l = jmin(l, limiting_param<DIM>(sm.UL[e][q], P, zeta, ZEROTOL, a.upper_mode));
+            }
+            l = jmin(l, 1.0);   // blending factor of NoShockCapture (shock_capture.jl:111-114)
+            double flim[NC];
+            for (int c = 0; c < NC; c++) flim[c] = l * fH[c] + (1.0 - l) * fL[c];
+            if (s > 0) {
+                const int q = DIM == 1 ? s - 1 : (d == 0 ? (s - 1) + N1D * o : o + N1D * (s - 1));
+                const double wJ = T.wq[q] * G.J;
+                for (int c = 0; c < NC; c++) sm.x.RU[e][q][d][c] = (flim[c] - fprev[c]) / wJ;
+            }
+            for (int c = 0; c < NC; c++) fprev[c] = flim[c];
+            if (s == 0 || s == N1D) {
+                const int f = DIM == 1 ? (s == 0 ? 0 : 1)
+                                       : (d == 0 ? (s == 0 ? o : N1D + o) : (s == 0 ? 2 * N1D + o : 3 * N1D + o));
+                a.lA[(size_t)k * NFP + f] = l;
+                for (int c = 0; c < NC; c++) a.dfA[((size_t)k * NFP + f) * NC + c] = dfb[c];
+            }
+            if (a.dbg_L) {
+                const int idx = DIM == 1 ? s : (d == 0 ? s + (N1D + 1) * o : o + N1D * s);
+                a.dbg_L[idx + (size_t)S::NLS * (d + DIM * (size_t)k)] = l;
+            }
+        }
+        if (DIM == 1 && a.dbg_L) {
+            for (int idx = N1D + 1; idx < S::NLS; idx++) a.dbg_L[idx + (size_t)S::NLS * k] = 1.0;
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 6: rhsU with the element's own limiter parameters
+    {
+        const int nel = min(E, K - k0);
+        double* dst = a.rhsUA + (size_t)k0 * NQ * NC;
+        for (int w = tid; w < nel * NQ * NC; w += NT) {
+            const int e = w / (NQ * NC), r = w - e * NQ * NC, i = r / NC, c = r - i * NC;
+            double s = sm.x.RU[e][i][0][c];
+            if (DIM == 2) s = s + sm.x.RU[e][i][DIM - 1][c];
+            dst[w] = s;
+        }
+    }
+}
+
+// Symmetrise the element-boundary parameters (subcell.jl:405-456), correct rhsU at the
+// adjacent nodes, then the RK stage (SSPRK33.jl:33-39) and the positivity check.
+template <int DIM, int N1D>
+__global__ __launch_bounds__(256) void phaseB_kernel(PhaseBArgs a) {
+    using S = Sz<DIM, N1D>;
+    constexpr int NQ = S::NQ, NFP = S::NFP, NC = S::NC;
+    if (a.sc->done) return;
+    const size_t gidx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const double dt = bits_to_double(a.sc->dt_bits);
+    if (a.finalize && gidx == 0) {
+        a.sc->t = a.sc->t + dt;
+        a.sc->steps = a.sc->steps + 1;
+    }
+    if (a.mode == MODE_STAGE1 && gidx == 0) a.sc->dt_lim = dt;  // stages 2 and 3 limit with the CFL dt
+    if (gidx >= (size_t)a.K * NQ) return;
+    const int k = (int)(gidx / NQ), i = (int)(gidx - (size_t)k * NQ);
+    const ElemTables& T = *a.T;
+    double r[NC];
+    for (int c = 0; c < NC; c++) r[c] = a.rhsUA[gidx * NC + c];
+    const double wJ = T.wq[i] * a.geo[k].J;
+    for (int m = 0; m < T.nf[i]; m++) {
+        const int f = T.nface[i][m];
+        const size_t fi = (size_t)k * NFP + f;
+        const double lown = a.lA[fi];
+        double lsym;
+        if (DIM == 1) {
+            // the reference min's face 1 with slot `end` = Nq+N1D of element k-1, which the
+            // limiter never writes (1.0): the element-boundary faces are not symmetrised
+            lsym = f == 0 ? jmin(lown, 1.0) : lown;
+        } else {
+            lsym = jmin(lown, a.lA[a.nb_face[fi]]);
+        }
+        if (!(lsym == lown)) {
+            for (int c = 0; c < NC; c++) {
+                const double delta = (lsym - lown) * a.dfA[fi * NC + c];
+                r[c] = T.bsign[f] > 0 ? r[c] + delta / wJ : r[c] - delta / wJ;
+            }
+        }
+        if (a.dbg_L) {
+            int idx, d;
+            if (DIM == 1) {
+                idx = f == 0 ? 0 : N1D;
+                d = 0;
+            } else if (f < 2 * N1D) {
+                d = 0;
+                idx = f < N1D ? (N1D + 1) * f : N1D + (N1D + 1) * (f - N1D);
+            } else {
+                d = 1;
+                idx = f < 3 * N1D ? f - 2 * N1D : (f - 3 * N1D) + N1D * N1D;
+            }
+            a.dbg_L[idx + (size_t)S::NLS * (d + DIM * (size_t)k)] = lsym;
+            if (DIM == 1 && f == 0) {
+                const int km1 = (k - 1 + a.K) % a.K;
+                a.dbg_L[(S::NLS - 1) + (size_t)S::NLS * (size_t)km1] = lsym;
+            }
+        }
+    }
+    if (a.mode == MODE_RHS) {
+        for (int c = 0; c < NC; c++) a.rhsU[gidx * NC + c] = r[c];
+        return;
+    }
+    double u[NC], w[NC], un[NC];
+    for (int c = 0; c < NC; c++) {
+        u[c] = a.Uin[gidx * NC + c];
+        w[c] = a.W[gidx * NC + c];
+    }
+    if (a.mode == MODE_STAGE1) {
+        for (int c = 0; c < NC; c++) un[c] = w[c] + dt * r[c];
+    } else if (a.mode == MODE_STAGE2) {
+        for (int c = 0; c < NC; c++) {
+            double z = u[c] + dt * r[c];
+            un[c] = 3.0 / 4.0 * w[c] + 1.0 / 4.0 * z;
+        }
+    } else {
+        for (int c = 0; c < NC; c++) {
+            double z = u[c] + dt * r[c];
+            un[c] = 1.0 / 3.0 * w[c] + 2.0 / 3.0 * z;
+        }
+    }
+    for (int c = 0; c < NC; c++) a.Uout[gidx * NC + c] = un[c];
+    const double pr = Eq<DIM>::p(a.gamma, un);
+    if (!(un[0] > 0.0) || !(pr > 0.0)) {
+        atomicAdd(&a.sc->viol, 1ull);
+        atomicMin(&a.sc->first_bad, ((unsigned long long)a.mode << 40) | (unsigned long long)gidx);
+    }
+}
+
+__global__ void step_init_kernel(StepScalars* sc, double CFL, double dt0, double T) {
+    if (threadIdx.x != 0) return;
+    const double t = sc->t;
+    if (!(t < T)) {
+        sc->done = 1;
+        return;
+    }
+    sc->done = 0;
+    const double dt = jmin(CFL * dt0, T - t);   // SSPRK33.jl:30
+    sc->dt_lim = dt;
+    sc->dt_bits = double_to_bits(dt);
+}
+
+__global__ void rhs_init_kernel(StepScalars* sc, double dt_lim, double dt_cap) {
+    if (threadIdx.x != 0) return;
+    sc->done = 0;
+    sc->dt_lim = dt_lim;
+    sc->dt_bits = double_to_bits(dt_cap);
+}
+
+// order-preserving map of a double to an unsigned key (for atomicMin)
+__device__ __forceinline__ unsigned long long order_key(double v) {
+    unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+template <int DIM>
+__global__ void positivity_kernel(const double* U, int64_t nodes, double gamma, unsigned long long* keys,
+                                  unsigned long long* cnt) {
+    const size_t gidx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double mr = INFINITY, mp = INFINITY;
+    unsigned long long bad = 0;
+    if (gidx < (size_t)nodes) {
+        const double* u = U + gidx * (DIM + 2);
+        const double pr = Eq<DIM>::p(gamma, u);
+        mr = u[0];
+        mp = pr;
+        bad = (!(u[0] > 0.0) || !(pr > 0.0)) ? 1ull : 0ull;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        double o1 = __shfl_xor(mr, off, 64), o2 = __shfl_xor(mp, off, 64);
+        mr = o1 < mr ? o1 : mr;
+        mp = o2 < mp ? o2 : mp;
+        bad += __shfl_xor(bad, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&keys[0], order_key(mr));
+        atomicMin(&keys[1], order_key(mp));
+        if (bad) atomicAdd(cnt, bad);
+    }
+}
+
+// --------------------------------------------------------------------------- launchers
+template <int DIM, int N1D>
+static int launchA(const PhaseAArgs& a, hipStream_t s) {
+    constexpr int E = Cfg<DIM, N1D>::E, NT = Cfg<DIM, N1D>::NT;
+    const int blocks = (a.K + E - 1) / E;
+    hipLaunchKernelGGL((phaseA_kernel<DIM, N1D, E, NT>), dim3(blocks), dim3(NT), 0, s, a);
+    return (int)hipGetLastError();
+}
+template <int DIM, int N1D>
+static int launchB(const PhaseBArgs& a, hipStream_t s) {
+    const size_t nodes = (size_t)a.K * Sz<DIM, N1D>::NQ;
+    const int blocks = (int)((nodes + 255) / 256);
+    hipLaunchKernelGGL((phaseB_kernel<DIM, N1D>), dim3(blocks), dim3(256), 0, s, a);
+    return (int)hipGetLastError();
+}
+
+int launch_phaseA(int Nd, int N1D, const PhaseAArgs& a, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (Nd == 1) {
+        switch (N1D) {
+            case 2: return launchA<1, 2>(a, s);
+            case 3: return launchA<1, 3>(a, s);
+            case 4: return launchA<1, 4>(a, s);
+            case 5: return launchA<1, 5>(a, s);
+        }
+    } else {
+        switch (N1D) {
+            case 2: return launchA<2, 2>(a, s);
+            case 3: return launchA<2, 3>(a, s);
+            case 4: return launchA<2, 4>(a, s);
+            case 5: return launchA<2, 5>(a, s);
+        }
+    }
+    return (int)hipErrorInvalidValue;
+}
+
+int launch_phaseB(int Nd, int N1D, const PhaseBArgs& a, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (Nd == 1) {
+        switch (N1D) {
+            case 2: return launchB<1, 2>(a, s);
+            case 3: return launchB<1, 3>(a, s);
+            case 4: return launchB<1, 4>(a, s);
+            case 5: return launchB<1, 5>(a, s);
+        }
+    } else {
+        switch (N1D) {
+            case 2: return launchB<2, 2>(a, s);
+            case 3: return launchB<2, 3>(a, s);
+            case 4: return launchB<2, 4>(a, s);
+            case 5: return launchB<2, 5>(a, s);
+        }
+    }
+    return (int)hipErrorInvalidValue;
+}
+
+int launch_step_init(StepScalars* sc, double CFL, double dt0, double T, void* stream) {
+    hipLaunchKernelGGL(step_init_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, sc, CFL, dt0, T);
+    return (int)hipGetLastError();
+}
+
+int launch_rhs_init(StepScalars* sc, double dt_lim, double dt_cap, void* stream) {
+    hipLaunchKernelGGL(rhs_init_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, sc, dt_lim, dt_cap);
+    return (int)hipGetLastError();
+}
+
+int launch_positivity(const double* U, int64_t nodes, int Nd, double gamma, unsigned long long* keys,
+                      unsigned long long* cnt, void* stream) {
+    const int blocks = (int)((nodes + 255) / 256);
+    if (Nd == 1)
+        hipLaunchKernelGGL(positivity_kernel<1>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, U, nodes, gamma,
+                           keys, cnt);
+    else
+        hipLaunchKernelGGL(positivity_kernel<2>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, U, nodes, gamma,
+                           keys, cnt);
+    return (int)hipGetLastError();
+}
+
+}  // namespace p2de

@@ -1,0 +1,288 @@
+"""GPU drop-ins for ``rhs!`` and ``SSP33!`` over the C-ABI library ``libp2de_hip.so``.
+
+``DeviceSolver`` owns one native context (``p2de_create``) built from a ``FlatProblem``.
+``rhs(state, solver, state_param, time_param)`` and ``SSP33(state, solver, state_param)``
+mirror src/dg/rhs/rhs.jl:5-13 and src/timestepping/SSPRK33.jl:1-62 with the same
+arguments and meaning; they run on the GPU and raise ``P2DEError`` with the library's
+message on any failure.  There is no CPU fallback: without the built library every call
+fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+from .dg import FlatProblem, State, StateParam, Solver, TimeParam, flat_problem
+
+LIB_PATH = Path(__file__).resolve().parent / "csrc" / "libp2de_hip.so"
+
+P2DE_OK, P2DE_E_INVALID, P2DE_E_UNSUPPORTED, P2DE_E_HIP, P2DE_E_ALLOC, P2DE_E_STATE, P2DE_E_POSITIVITY = range(7)
+FIELD_RHSU, FIELD_RHSH, FIELD_RHSL, FIELD_L_LOCAL = range(4)
+UPPER_BOUND_REFERENCE, UPPER_BOUND_INTENDED = 0, 1
+
+
+class P2DEError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"[{status}] {msg}")
+        self.status = status
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int64)
+
+
+class _Config(C.Structure):
+    _fields_ = [("Nd", C.c_int), ("N", C.c_int), ("Kx", C.c_int), ("Ky", C.c_int),
+                ("gamma", C.c_double), ("zeta", C.c_double), ("eta", C.c_double), ("POSTOL", C.c_double),
+                ("ZEROTOL", C.c_double), ("CFL", C.c_double), ("dt0", C.c_double), ("t0", C.c_double),
+                ("T", C.c_double),
+                ("rhs", C.c_int), ("low_order_surface_flux", C.c_int), ("high_order_surface_flux", C.c_int),
+                ("high_order_volume_flux", C.c_int), ("basis", C.c_int), ("entropyproj_limiter", C.c_int),
+                ("rhs_limiter", C.c_int), ("bound", C.c_int), ("shockcapture", C.c_int),
+                ("upper_bound_mode", C.c_int), ("precision", C.c_int)]
+
+
+class _Operators(C.Structure):
+    _fields_ = [("Nq", C.c_int), ("Nfp", C.c_int), ("Nh", C.c_int), ("Srsh_db", _dp), ("Srs0", _dp),
+                ("nnz_h", C.c_int), ("Srsh_nnz", _ip), ("nnz_0", C.c_int), ("Srs0_nnz", _ip), ("Brs", _dp),
+                ("MinvVhT", _dp), ("MinvVfT", _dp), ("Vf", _dp), ("Vf_low", _dp), ("wq", _dp), ("fq2q", _ip),
+                ("q2fq", _ip)]
+
+
+class _Geometry(C.Structure):
+    _fields_ = [("Jq", _dp), ("GJh", _dp)]
+
+
+class _BCData(C.Structure):
+    _fields_ = [("mapP", _ip), ("nI", C.c_int), ("mapI", _ip), ("Ival", _dp), ("nO", C.c_int), ("mapO", _ip)]
+
+
+class PosReport(C.Structure):
+    _fields_ = [("violations", C.c_int64), ("first_node", C.c_int64), ("first_stage", C.c_int),
+                ("min_rho", C.c_double), ("min_p", C.c_double)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("steps", C.c_int), ("t", C.c_double), ("last_dt", C.c_double), ("violations", C.c_int64)]
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library; raises if it has not been built (no CPU fallback exists)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise P2DEError(P2DE_E_STATE, f"{LIB_PATH} is missing: run __graft_entry__.build()")
+        L = C.CDLL(str(LIB_PATH))
+        vp = C.c_void_p
+        L.p2de_create.argtypes = [C.POINTER(_Config), C.POINTER(_Operators), C.POINTER(_Geometry),
+                                  C.POINTER(_BCData), C.POINTER(vp)]
+        L.p2de_destroy.argtypes = [vp]
+        L.p2de_destroy.restype = None
+        L.p2de_set_state.argtypes = [vp, _dp]
+        L.p2de_get_state.argtypes = [vp, _dp]
+        L.p2de_rhs.argtypes = [vp, C.c_double, C.c_double, C.c_int, _dp, _dp]
+        L.p2de_set_debug.argtypes = [vp, C.c_int]
+        L.p2de_get_field.argtypes = [vp, C.c_int, _dp]
+        L.p2de_ssprk3_step.argtypes = [vp, _dp, _dp]
+        L.p2de_run.argtypes = [vp, C.c_int, C.POINTER(RunStats)]
+        L.p2de_set_time.argtypes = [vp, C.c_double]
+        L.p2de_get_time.argtypes = [vp, _dp, _dp]
+        L.p2de_check_positivity.argtypes = [vp, C.POINTER(PosReport)]
+        L.p2de_enqueue_steps.argtypes = [vp, C.c_int]
+        L.p2de_synchronize.argtypes = [vp]
+        L.p2de_set_kernel_timing.argtypes = [vp, C.c_int]
+        L.p2de_get_kernel_timing.argtypes = [vp, _dp, _ip]
+        L.p2de_num_dofs.argtypes = [vp]
+        L.p2de_num_dofs.restype = C.c_int64
+        L.p2de_device_state.argtypes = [vp, C.POINTER(vp)]
+        L.p2de_last_error.argtypes = [vp]
+        L.p2de_last_error.restype = C.c_char_p
+        L.p2de_status_string.argtypes = [C.c_int]
+        L.p2de_status_string.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+# every symbol include/p2de_hip.h declares
+EXPORTED_SYMBOLS = (
+    "p2de_create", "p2de_destroy", "p2de_set_state", "p2de_get_state", "p2de_rhs", "p2de_set_debug",
+    "p2de_get_field", "p2de_ssprk3_step", "p2de_run", "p2de_set_time", "p2de_get_time",
+    "p2de_check_positivity", "p2de_enqueue_steps", "p2de_synchronize", "p2de_set_kernel_timing",
+    "p2de_get_kernel_timing", "p2de_num_dofs", "p2de_device_state", "p2de_last_error", "p2de_status_string",
+)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp) if a is not None and a.size else None
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip) if a is not None and a.size else None
+
+
+class DeviceSolver:
+    """One native context; the state lives on the GPU between calls."""
+
+    def __init__(self, fp: FlatProblem, upper_bound_mode: int = None):
+        self.fp = fp
+        mode = fp.upper_bound_mode if upper_bound_mode is None else upper_bound_mode
+        self.Nc, self.Nq, self.K = fp.Nc, fp.Nq, fp.K
+        cfg = _Config(Nd=fp.Nd, N=fp.N1D - 1, Kx=fp.Kx, Ky=fp.Ky, gamma=fp.gamma, zeta=fp.zeta, eta=fp.eta,
+                      POSTOL=fp.POSTOL, ZEROTOL=fp.ZEROTOL, CFL=fp.CFL, dt0=fp.dt0, t0=fp.t0, T=fp.T,
+                      high_order_surface_flux=1, upper_bound_mode=int(mode))
+        keep = {}
+        for n in ("Srsh_db", "Srs0", "Brs", "MinvVhT", "MinvVfT", "Vf", "Vf_low", "wq", "Jq", "GJh", "Ival"):
+            keep[n] = np.ascontiguousarray(getattr(fp, n), dtype=np.float64)
+        for n in ("Srsh_nnz", "Srs0_nnz", "fq2q", "q2fq", "mapP", "mapI", "mapO"):
+            keep[n] = np.ascontiguousarray(getattr(fp, n), dtype=np.int64)
+        ops = _Operators(Nq=fp.Nq, Nfp=fp.Nfp, Nh=fp.Nh, Srsh_db=_d(keep["Srsh_db"]), Srs0=_d(keep["Srs0"]),
+                         nnz_h=len(keep["Srsh_nnz"]) // 2, Srsh_nnz=_i(keep["Srsh_nnz"]),
+                         nnz_0=len(keep["Srs0_nnz"]) // 2, Srs0_nnz=_i(keep["Srs0_nnz"]), Brs=_d(keep["Brs"]),
+                         MinvVhT=_d(keep["MinvVhT"]), MinvVfT=_d(keep["MinvVfT"]), Vf=_d(keep["Vf"]),
+                         Vf_low=_d(keep["Vf_low"]), wq=_d(keep["wq"]), fq2q=_i(keep["fq2q"]), q2fq=_i(keep["q2fq"]))
+        geo = _Geometry(Jq=_d(keep["Jq"]), GJh=_d(keep["GJh"]))
+        bc = _BCData(mapP=_i(keep["mapP"]), nI=len(keep["mapI"]), mapI=_i(keep["mapI"]), Ival=_d(keep["Ival"]),
+                     nO=len(keep["mapO"]), mapO=_i(keep["mapO"]))
+        h = C.c_void_p()
+        st = lib().p2de_create(C.byref(cfg), C.byref(ops), C.byref(geo), C.byref(bc), C.byref(h))
+        if st != P2DE_OK:
+            raise P2DEError(st, lib().p2de_last_error(None).decode())
+        self.h = h
+
+    # --------------------------------------------------------------- plumbing
+    def _check(self, st):
+        if st != P2DE_OK:
+            raise P2DEError(st, lib().p2de_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().p2de_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def shape(self):
+        return (self.K, self.Nq, self.Nc)
+
+    # --------------------------------------------------------------- state
+    def set_state(self, Uq: np.ndarray):
+        U = np.ascontiguousarray(Uq, dtype=np.float64)
+        if U.size != self.K * self.Nq * self.Nc:
+            raise P2DEError(P2DE_E_INVALID, f"state has {U.size} values, expected {self.K * self.Nq * self.Nc}")
+        self._check(lib().p2de_set_state(self.h, _d(U)))
+
+    def get_state(self) -> np.ndarray:
+        out = np.empty(self.shape)
+        self._check(lib().p2de_get_state(self.h, _d(out)))
+        return out
+
+    # --------------------------------------------------------------- rhs!
+    def rhs(self, t: float, dt: float, nstage: int, want_rhs: bool = True):
+        """``rhs!`` on the device state; returns (dt_out, rhsU or None)."""
+        out = np.empty(self.shape) if want_rhs else None
+        dto = C.c_double(0.0)
+        self._check(lib().p2de_rhs(self.h, float(t), float(dt), int(nstage), C.byref(dto), _d(out) if want_rhs else None))
+        return dto.value, out
+
+    def set_debug(self, on: bool = True):
+        self._check(lib().p2de_set_debug(self.h, 1 if on else 0))
+
+    def field(self, which: int) -> np.ndarray:
+        n = self.K * self.Nq * self.Nc
+        if which == FIELD_L_LOCAL:
+            n = (self.Nq + self.fp.N1D) * self.fp.Nd * self.K
+        out = np.empty(n)
+        self._check(lib().p2de_get_field(self.h, int(which), _d(out)))
+        return out
+
+    # --------------------------------------------------------------- time stepping
+    def step(self, t: float):
+        """One SSP33! step of the device state; returns (t_new, dt)."""
+        tt = C.c_double(t)
+        dt = C.c_double(0.0)
+        self._check(lib().p2de_ssprk3_step(self.h, C.byref(tt), C.byref(dt)))
+        return tt.value, dt.value
+
+    def set_time(self, t: float):
+        self._check(lib().p2de_set_time(self.h, float(t)))
+
+    def get_time(self):
+        t, dt = C.c_double(0.0), C.c_double(0.0)
+        self._check(lib().p2de_get_time(self.h, C.byref(t), C.byref(dt)))
+        return t.value, dt.value
+
+    def run(self, max_steps: int = 0) -> RunStats:
+        st = RunStats()
+        self._check(lib().p2de_run(self.h, int(max_steps), C.byref(st)))
+        return st
+
+    def enqueue_steps(self, n: int):
+        self._check(lib().p2de_enqueue_steps(self.h, int(n)))
+
+    def synchronize(self):
+        self._check(lib().p2de_synchronize(self.h))
+
+    def check_positivity(self) -> PosReport:
+        r = PosReport()
+        self._check(lib().p2de_check_positivity(self.h, C.byref(r)))
+        return r
+
+    def set_kernel_timing(self, on: bool):
+        self._check(lib().p2de_set_kernel_timing(self.h, 1 if on else 0))
+
+    def kernel_timing(self):
+        ms = (C.c_double * 3)()
+        cnt = (C.c_int64 * 3)()
+        self._check(lib().p2de_get_kernel_timing(self.h, ms, cnt))
+        return list(ms), list(cnt)
+
+    def num_dofs(self) -> int:
+        return int(lib().p2de_num_dofs(self.h))
+
+
+# ------------------------------------------------------------------- reference-shaped API
+def _device(state: State, solver: Solver, state_param: StateParam) -> DeviceSolver:
+    if state.device is None:
+        state.device = DeviceSolver(flat_problem(solver, state_param))
+    return state.device
+
+
+def rhs(state: State, solver: Solver, state_param: StateParam, time_param: TimeParam) -> float:
+    """``rhs!`` (src/dg/rhs/rhs.jl:5-13): evaluates the limited RHS of
+    ``state.preallocation.Uq`` on the GPU, writes ``state.preallocation.rhsU`` and returns
+    the CFL dt at stage 1 (``time_param.dt`` otherwise)."""
+    dev = _device(state, solver, state_param)
+    dev.set_state(state.preallocation.Uq)
+    dt, r = dev.rhs(time_param.t, time_param.dt, time_param.nstage)
+    state.preallocation.rhsU[...] = r
+    return dt
+
+
+@dataclass
+class DataHistory:
+    """The part of ``DataHistory`` (PostprocessData.jl:1-7) the drop-in keeps."""
+    thist: list
+    dthist: list
+    positivity_violations: int
+
+
+def SSP33(state: State, solver: Solver, state_param: StateParam, max_steps: int = 0) -> DataHistory:
+    """``SSP33!`` (src/timestepping/SSPRK33.jl:1-62): advances ``state.preallocation.Uq``
+    from t0 to T on the GPU (state resident on the device, time and dt too)."""
+    dev = _device(state, solver, state_param)
+    dev.set_state(state.preallocation.Uq)
+    dev.set_time(solver.param.timestepping_param.t0)
+    stats = dev.run(max_steps)
+    state.preallocation.Uq[...] = dev.get_state()
+    return DataHistory(thist=[stats.t], dthist=[stats.last_dt], positivity_violations=int(stats.violations))

@@ -1,0 +1,216 @@
+"""Option and parameter types of the solver, mirroring the reference's dispatch types.
+
+The reference selects methods by the type parameters of ``Param``
+(src/common/types/Solver.jl:151-170).  Here each option is a small frozen
+dataclass with the same name; ``Param`` carries them, and the native library
+receives them as integer enums (``include/p2de_hip.h``).  Combinations the
+MI355X path does not implement are rejected with ``UnsupportedOption`` before
+anything reaches the device.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Tuple, Union
+
+
+class UnsupportedOption(ValueError):
+    """Raised for an option combination the HIP path does not implement."""
+
+
+# --- dimensions and equations (Solver.jl:97-121) ---------------------------------
+@dataclass(frozen=True)
+class Dim1:
+    pass
+
+
+@dataclass(frozen=True)
+class Dim2:
+    pass
+
+
+@dataclass(frozen=True)
+class CompressibleEulerIdealGas:
+    """``CompressibleEulerIdealGas{DIM}(gamma)`` (Solver.jl:110-118)."""
+    dim: Union[Dim1, Dim2]
+    gamma: float = 1.4
+
+    @property
+    def Nd(self) -> int:
+        return 1 if isinstance(self.dim, Dim1) else 2
+
+    @property
+    def Nc(self) -> int:  # num_components (traits.jl:408-414)
+        return self.Nd + 2
+
+
+# --- fluxes (Solver.jl:15-36) ----------------------------------------------------
+@dataclass(frozen=True)
+class ChandrashekarFlux:
+    pass
+
+
+@dataclass(frozen=True)
+class CentralFlux:
+    pass
+
+
+@dataclass(frozen=True)
+class LaxFriedrichsOnNodalVal:
+    pass
+
+
+@dataclass(frozen=True)
+class LaxFriedrichsOnProjectedVal:
+    pass
+
+
+@dataclass(frozen=True)
+class ChandrashekarOnProjectedVal:
+    pass
+
+
+@dataclass(frozen=True)
+class LimitedDG:
+    """``LimitedDG`` / ``ESLimitedLowOrderPos`` (Solver.jl:9-13,26,31-33)."""
+    low_order_surface_flux: object = field(default_factory=LaxFriedrichsOnNodalVal)
+    high_order_surface_flux: object = field(default_factory=LaxFriedrichsOnProjectedVal)
+    high_order_volume_flux: object = field(default_factory=ChandrashekarFlux)
+
+
+def ESLimitedLowOrderPos(low_order_surface_flux=None, high_order_surface_flux=None) -> LimitedDG:
+    """Constructor alias of Solver.jl:31-33."""
+    return LimitedDG(low_order_surface_flux or LaxFriedrichsOnNodalVal(),
+                     high_order_surface_flux or LaxFriedrichsOnProjectedVal(),
+                     ChandrashekarFlux())
+
+
+# --- limiters (Solver.jl:38-87) --------------------------------------------------
+@dataclass(frozen=True)
+class NoEntropyProjectionLimiter:
+    pass
+
+
+@dataclass(frozen=True)
+class NodewiseScaledExtrapolation:
+    pass
+
+
+@dataclass(frozen=True)
+class PositivityBound:
+    pass
+
+
+@dataclass(frozen=True)
+class NoShockCapture:
+    pass
+
+
+@dataclass(frozen=True)
+class HennemannShockCapture:
+    a: float = 0.5
+    c: float = 1.8
+
+
+@dataclass(frozen=True)
+class SubcellLimiter:
+    bound: object = field(default_factory=PositivityBound)
+    shockcapture: object = field(default_factory=NoShockCapture)
+
+
+@dataclass(frozen=True)
+class ZhangShuLimiter:
+    shockcapture: object = field(default_factory=NoShockCapture)
+
+
+# --- bases (Solver.jl:89-95) -----------------------------------------------------
+@dataclass(frozen=True)
+class LobattoCollocation:
+    pass
+
+
+@dataclass(frozen=True)
+class GaussCollocation:
+    pass
+
+
+# --- scalar parameter groups (Solver.jl:123-149) ---------------------------------
+@dataclass(frozen=True)
+class GlobalConstant:
+    POSTOL: float = 1e-14
+    ZEROTOL: float = 5e-16
+
+
+@dataclass(frozen=True)
+class TimesteppingParameter:
+    T: float
+    CFL: float
+    dt0: float
+    t0: float = 0.0
+
+
+@dataclass(frozen=True)
+class LimitingParameter:
+    zeta: float = 0.1
+    eta: float = 1.0
+
+
+@dataclass(frozen=True)
+class PostprocessingParameter:
+    output_interval: int = 10**9
+
+
+@dataclass(frozen=True)
+class Param:
+    """``Param`` (Solver.jl:151-170).  ``K`` is an int in 1D and ``(Kx, Ky)`` in 2D;
+    ``xL``/``xR`` are floats in 1D and pairs in 2D, as in the reference."""
+    N: int
+    K: Union[int, Tuple[int, int]]
+    xL: Union[float, Tuple[float, float]]
+    xR: Union[float, Tuple[float, float]]
+    global_constants: GlobalConstant
+    timestepping_param: TimesteppingParameter
+    limiting_param: LimitingParameter
+    equation: CompressibleEulerIdealGas
+    rhs: LimitedDG = field(default_factory=LimitedDG)
+    approximation_basis: object = field(default_factory=LobattoCollocation)
+    entropyproj_limiter: object = field(default_factory=NoEntropyProjectionLimiter)
+    rhs_limiter: object = field(default_factory=SubcellLimiter)
+    postprocessing_param: PostprocessingParameter = field(default_factory=PostprocessingParameter)
+
+    @property
+    def num_elements(self) -> int:  # traits.jl:424-440
+        if self.equation.Nd == 1:
+            return int(self.K)
+        return int(self.K[0]) * int(self.K[1])
+
+
+def check_supported(param: Param) -> None:
+    """Reject option combinations outside the implemented hot path.
+
+    Implemented: ``ESLimitedLowOrderPos(LaxFriedrichsOnNodalVal, LaxFriedrichsOnProjectedVal)``
+    with ``LobattoCollocation`` + ``NoEntropyProjectionLimiter`` +
+    ``SubcellLimiter(PositivityBound, NoShockCapture)`` for 1D/2D Euler -- the combination
+    test/test_smoke.jl:107-108 exercises.
+    """
+    rhs = param.rhs
+    if not isinstance(rhs, LimitedDG):
+        raise UnsupportedOption(f"rhs {rhs!r}: only LimitedDG (ESLimitedLowOrderPos) is implemented")
+    if not isinstance(rhs.high_order_volume_flux, ChandrashekarFlux):
+        raise UnsupportedOption("high_order_volume_flux must be ChandrashekarFlux")
+    if not isinstance(rhs.low_order_surface_flux, LaxFriedrichsOnNodalVal):
+        raise UnsupportedOption("low_order_surface_flux must be LaxFriedrichsOnNodalVal")
+    if not isinstance(rhs.high_order_surface_flux, LaxFriedrichsOnProjectedVal):
+        raise UnsupportedOption("high_order_surface_flux must be LaxFriedrichsOnProjectedVal")
+    if not isinstance(param.approximation_basis, LobattoCollocation):
+        raise UnsupportedOption("approximation_basis must be LobattoCollocation")
+    if not isinstance(param.entropyproj_limiter, NoEntropyProjectionLimiter):
+        raise UnsupportedOption("entropyproj_limiter must be NoEntropyProjectionLimiter")
+    lim = param.rhs_limiter
+    if not isinstance(lim, SubcellLimiter):
+        raise UnsupportedOption(f"rhs_limiter {lim!r}: only SubcellLimiter is implemented")
+    if not isinstance(lim.bound, PositivityBound):
+        raise UnsupportedOption("SubcellLimiter bound must be PositivityBound")
+    if not isinstance(lim.shockcapture, NoShockCapture):
+        raise UnsupportedOption("SubcellLimiter shockcapture must be NoShockCapture")
+    if not 1 <= param.N <= 4:
+        raise UnsupportedOption(f"polynomial degree N={param.N}: 1..4 are implemented")

@@ -1,0 +1,118 @@
+"""GPU parity: the HIP path through the C-ABI against the CPU oracle on the same inputs.
+
+Bar (FP64 drop-in), per RHS evaluation, per conserved component c:
+  ||rhs_gpu,c - rhs_cpu,c||_2 / ||rhs_cpu,c||_2 <= 1e-12          (relative error)
+  ||rhs_gpu,c - rhs_cpu,c||_inf / ||rhs_cpu,c||_inf <= 1e-11      (worst node)
+for rhsU (what rhs! returns), rhsH and rhsL; dt to 1e-14; positivity (rho > 0, p > 0)
+exactly after every stage.  The worst-node bound is looser because the log mean
+-da/(log aL - log aR) (compressible_Navier_Stokes.jl:319) amplifies a one-ulp difference
+between two faithful log implementations (the host's and the GPU's) by up to
+1/|log aL - log aR| ~ 1e4 next to the series switch; rhsL, which has no logarithms,
+agrees bit for bit.
+"""
+import numpy as np
+import pytest
+
+from conftest import PKG, normwise_rel, perturbed_state, problem, rel2
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+TOL_INF = 1e-11
+
+def limiter_mismatch(orc, L_gpu, fp, nstage):
+    """Largest |l_gpu - l_cpu| over subcell faces whose f_bar_H - f_bar_L density jump is
+    above roundoff.  Where the jump is roundoff (the element-boundary faces, where the
+    high- and low-order surface fluxes agree to the last bits) the reference's own
+    parameter is decided by the sign of that noise under its Urho = 0 bound
+    (limiter_utils.jl:67), and either value changes rhsU only at roundoff."""
+    NLs = fp.Nq + fp.N1D
+    L_cpu = orc.field("L_local").reshape(3, fp.K, fp.Nd, NLs)[nstage - 1]
+    L_gpu = L_gpu.reshape(fp.K, fp.Nd, NLs)
+    fH = orc.field("f_bar_H").reshape(fp.Nd, fp.K, NLs, fp.Nc).transpose(1, 0, 2, 3)
+    fL = orc.field("f_bar_L").reshape(fp.Nd, fp.K, NLs, fp.Nc).transpose(1, 0, 2, 3)
+    drho = np.abs(fH[..., 0] - fL[..., 0])
+    scale = np.max(np.abs(fH[..., 0])) + np.max(np.abs(fL[..., 0]))
+    determined = drho > 1e-11 * scale
+    if fp.Nd == 1:
+        # unused slots hold 1.0; the last slot is a copy of element k+1's first face
+        # (subcell.jl:412-414), determined exactly when that face is
+        determined[:, :, fp.N1D + 1:] = True
+        determined[:, :, NLs - 1] = np.roll(determined[:, :, 0], -1, axis=0)
+    return float(np.max(np.abs(L_gpu - L_cpu) * determined))
+
+
+CASES = [
+    ("vortex_2d", dict(K=8)),
+    ("vortex_2d", dict(K=6, N=1)),
+    ("vortex_2d", dict(K=6, N=2)),
+    ("vortex_2d", dict(K=5, N=4)),
+    ("riemann4_2d", dict(K=12)),
+    ("kelvin_helmholtz", dict(K=8)),
+    ("smoke_vortex", dict(N=3)),
+    ("sod_1d", dict()),
+    ("sod_1d", dict(K=17, N=1)),
+    ("sod_1d", dict(K=9, N=4)),
+]
+
+
+def _ids(c):
+    return f"{c[0]}-" + "-".join(f"{k}{v}" for k, v in c[1].items())
+
+
+@pytest.mark.parametrize("case", CASES, ids=[_ids(c) for c in CASES])
+@pytest.mark.parametrize("mode", [0, 1], ids=["ref-bounds", "intended-bounds"])
+def test_rhs_parity(case, mode):
+    from p2de_jl_amd.solver import DeviceSolver, FIELD_L_LOCAL, FIELD_RHSH, FIELD_RHSL
+    name, kw = case
+    solver, state, sp, fp = problem(name, **kw)
+    print(f"case {name} {kw} mode {mode}", flush=True)
+    fp.upper_bound_mode = mode
+    dev = DeviceSolver(fp)
+    dev.set_debug(True)
+    orc = Oracle(fp)
+    U0 = state.preallocation.Uq
+    for trial, U in enumerate([U0, perturbed_state(U0, seed=1)]):
+        dev.set_state(U)
+        for nstage, dt in [(1, 0.9 * fp.CFL * fp.dt0), (2, 0.5 * fp.CFL * fp.dt0), (3, 0.7 * fp.CFL * fp.dt0)]:
+            dt_gpu, r_gpu = dev.rhs(0.0, dt, nstage)
+            dt_cpu = orc.rhs(U, 0.0, dt, nstage)
+            Nc = fp.Nc
+            e_u = normwise_rel(r_gpu, orc.field("rhsU"), Nc)
+            e_h = normwise_rel(dev.field(FIELD_RHSH), orc.field("rhsH"), Nc)
+            e_l = normwise_rel(dev.field(FIELD_RHSL), orc.field("rhsL"), Nc)
+            e2 = max(rel2(r_gpu, orc.field("rhsU"), Nc), rel2(dev.field(FIELD_RHSH), orc.field("rhsH"), Nc),
+                     rel2(dev.field(FIELD_RHSL), orc.field("rhsL"), Nc))
+            e_L = limiter_mismatch(orc, dev.field(FIELD_L_LOCAL), fp, nstage)
+            print(f"  trial {trial} stage {nstage}: 2-norm {e2:.2e} inf: rhsU {e_u:.2e} rhsH {e_h:.2e} rhsL {e_l:.2e} L {e_L:.2e} "
+                  f"dt {dt_gpu:.17g} vs {dt_cpu:.17g}", flush=True)
+            assert e2 <= TOL
+            assert max(e_u, e_h, e_l) <= TOL_INF
+            assert e_L <= 1e-12
+            assert abs(dt_gpu - dt_cpu) <= 1e-14 * abs(dt_cpu)
+
+
+@pytest.mark.parametrize("case", [("vortex_2d", dict(K=8)), ("riemann4_2d", dict(K=12)), ("sod_1d", dict())],
+                         ids=["vortex", "riemann4", "sod"])
+def test_ssprk3_steps_parity(case):
+    from p2de_jl_amd.solver import DeviceSolver
+    name, kw = case
+    solver, state, sp, fp = problem(name, **kw)
+    print(f"case {name} {kw}", flush=True)
+    dev = DeviceSolver(fp)
+    orc = Oracle(fp)
+    U = state.preallocation.Uq.copy()
+    dev.set_state(U)
+    t_cpu = t_gpu = 0.0
+    for step in range(5):
+        t_cpu, dt_cpu = orc.step(U, t_cpu)
+        t_gpu, dt_gpu = dev.step(t_gpu)
+        Ug = dev.get_state()
+        err = normwise_rel(Ug, U, fp.Nc)
+        print(f"  step {step}: dt {dt_gpu:.17g} vs {dt_cpu:.17g} state err {err:.2e}", flush=True)
+        assert abs(dt_gpu - dt_cpu) <= 1e-13 * dt_cpu
+        assert err <= 1e-12
+        rep = dev.check_positivity()
+        assert rep.violations == 0 and rep.min_rho > 0 and rep.min_p > 0
+    assert abs(t_gpu - t_cpu) <= 1e-14
