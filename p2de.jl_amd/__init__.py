@@ -16,3 +16,5 @@ from . import types, operators, dg, configs  # noqa: F401
 from .types import *  # noqa: F401,F403
 from .dg import (initialize_DG, make_periodic, primitive_to_conservative, flat_problem,  # noqa: F401
                  Solver, State, StateParam, TimeParam)
+from . import solver  # noqa: F401,E402
+from .solver import DeviceSolver, P2DEError, SSP33, rhs  # noqa: F401,E402

@@ -38,10 +38,23 @@ def normwise_rel(a, b, Nc):
     return float(np.max(np.max(np.abs(a - b), axis=0) / den))
 
 
-def perturbed_state(U, seed=0, amp=0.05):
-    """Seeded multiplicative perturbation of a state (keeps rho, p positive)."""
+def perturbed_state(U, seed=0, amp=0.05, gamma=1.4):
+    """Seeded perturbation in primitive variables: rho and p scaled by 1 +- amp, velocities
+    shifted by +- amp times the largest speed; rho and p stay positive for amp < 1."""
     rng = np.random.default_rng(seed)
-    return U * (1.0 + amp * rng.uniform(-1, 1, size=U.shape))
+    Nd = U.shape[-1] - 2
+    rho = U[..., 0]
+    vel = U[..., 1:1 + Nd] / rho[..., None]
+    p = (gamma - 1) * (U[..., -1] - 0.5 * rho * np.sum(vel * vel, axis=-1))
+    vmax = max(float(np.max(np.abs(vel))), 1e-3)
+    rho = rho * (1 + amp * rng.uniform(-1, 1, size=rho.shape))
+    p = p * (1 + amp * rng.uniform(-1, 1, size=p.shape))
+    vel = vel + amp * vmax * rng.uniform(-1, 1, size=vel.shape)
+    out = np.empty_like(U)
+    out[..., 0] = rho
+    out[..., 1:1 + Nd] = rho[..., None] * vel
+    out[..., -1] = p / (gamma - 1) + 0.5 * rho * np.sum(vel * vel, axis=-1)
+    return out
 
 
 def rel2(a, b, Nc):

@@ -1,0 +1,119 @@
+"""The C-ABI library on the host (CPU, no kernel launches): it loads, exports every
+symbol include/p2de_hip.h declares, and rejects bad arguments and unsupported options
+with a status and a message instead of crashing (the checks run before any device call)."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from conftest import PKG, problem
+
+S = PKG.solver
+HEADER = Path(__file__).resolve().parents[1] / "include" / "p2de_hip.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(p2de_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = S.lib()
+    names = declared_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(S.EXPORTED_SYMBOLS)
+
+
+def test_status_strings():
+    L = S.lib()
+    assert L.p2de_status_string(0) == b"ok"
+    assert b"unsupported" in L.p2de_status_string(S.P2DE_E_UNSUPPORTED)
+
+
+def _create(fp, **cfg_over):
+    """Call p2de_create like DeviceSolver does, with config overrides; return (status, msg)."""
+    cfg = S._Config(Nd=fp.Nd, N=fp.N1D - 1, Kx=fp.Kx, Ky=fp.Ky, gamma=fp.gamma, zeta=fp.zeta, eta=fp.eta,
+                    POSTOL=fp.POSTOL, ZEROTOL=fp.ZEROTOL, CFL=fp.CFL, dt0=fp.dt0, t0=fp.t0, T=fp.T,
+                    high_order_surface_flux=1)
+    for k, v in cfg_over.items():
+        setattr(cfg, k, v)
+    keep = {n: np.ascontiguousarray(getattr(fp, n), dtype=np.float64)
+            for n in ("Srsh_db", "Srs0", "Brs", "MinvVhT", "MinvVfT", "Vf", "Vf_low", "wq", "Jq", "GJh", "Ival")}
+    keep.update({n: np.ascontiguousarray(getattr(fp, n), dtype=np.int64)
+                 for n in ("Srsh_nnz", "Srs0_nnz", "fq2q", "q2fq", "mapP", "mapI", "mapO")})
+    return _create_raw(fp, cfg, keep)
+
+
+def _create_raw(fp, cfg, keep):
+    d, i = S._d, S._i
+    ops = S._Operators(Nq=fp.Nq, Nfp=fp.Nfp, Nh=fp.Nh, Srsh_db=d(keep["Srsh_db"]), Srs0=d(keep["Srs0"]),
+                       nnz_h=len(keep["Srsh_nnz"]) // 2, Srsh_nnz=i(keep["Srsh_nnz"]),
+                       nnz_0=len(keep["Srs0_nnz"]) // 2, Srs0_nnz=i(keep["Srs0_nnz"]), Brs=d(keep["Brs"]),
+                       MinvVhT=d(keep["MinvVhT"]), MinvVfT=d(keep["MinvVfT"]), Vf=d(keep["Vf"]),
+                       Vf_low=d(keep["Vf_low"]), wq=d(keep["wq"]), fq2q=i(keep["fq2q"]), q2fq=i(keep["q2fq"]))
+    geo = S._Geometry(Jq=d(keep["Jq"]), GJh=d(keep["GJh"]))
+    bc = S._BCData(mapP=i(keep["mapP"]), nI=len(keep["mapI"]), mapI=i(keep["mapI"]), Ival=d(keep["Ival"]),
+                   nO=len(keep["mapO"]), mapO=i(keep["mapO"]))
+    h = C.c_void_p()
+    st = S.lib().p2de_create(C.byref(cfg), C.byref(ops), C.byref(geo), C.byref(bc), C.byref(h))
+    msg = S.lib().p2de_last_error(None).decode()
+    if st == 0:
+        S.lib().p2de_destroy(h)
+    return st, msg
+
+
+@pytest.fixture(scope="module")
+def fp2():
+    return problem("vortex_2d", K=4)[3]
+
+
+@pytest.mark.parametrize("field,value,code", [
+    ("basis", 1, S.P2DE_E_UNSUPPORTED), ("rhs_limiter", 1, S.P2DE_E_UNSUPPORTED),
+    ("entropyproj_limiter", 1, S.P2DE_E_UNSUPPORTED), ("shockcapture", 1, S.P2DE_E_UNSUPPORTED),
+    ("high_order_surface_flux", 0, S.P2DE_E_UNSUPPORTED), ("precision", 7, S.P2DE_E_UNSUPPORTED),
+    ("upper_bound_mode", 5, S.P2DE_E_INVALID), ("Nd", 3, S.P2DE_E_INVALID), ("N", 9, S.P2DE_E_UNSUPPORTED),
+    ("Kx", 0, S.P2DE_E_INVALID), ("gamma", 1.0, S.P2DE_E_INVALID)])
+def test_create_rejects_options(fp2, field, value, code):
+    st, msg = _create(fp2, **{field: value})
+    assert st == code and msg
+
+
+def test_create_rejects_bad_operators_and_mesh(fp2):
+    cfg_fields = dict(Nd=fp2.Nd, N=fp2.N1D - 1, Kx=fp2.Kx, Ky=fp2.Ky, gamma=fp2.gamma, zeta=fp2.zeta, eta=fp2.eta,
+                      POSTOL=fp2.POSTOL, ZEROTOL=fp2.ZEROTOL, CFL=fp2.CFL, dt0=fp2.dt0, t0=fp2.t0, T=fp2.T,
+                      high_order_surface_flux=1)
+
+    def base():
+        keep = {n: np.ascontiguousarray(getattr(fp2, n), dtype=np.float64).copy()
+                for n in ("Srsh_db", "Srs0", "Brs", "MinvVhT", "MinvVfT", "Vf", "Vf_low", "wq", "Jq", "GJh", "Ival")}
+        keep.update({n: np.ascontiguousarray(getattr(fp2, n), dtype=np.int64).copy()
+                     for n in ("Srsh_nnz", "Srs0_nnz", "fq2q", "q2fq", "mapP", "mapI", "mapO")})
+        return keep
+
+    cases = []
+    k = base(); k["mapP"][3] = 10 ** 9; cases.append((k, S.P2DE_E_INVALID, "mapP"))
+    k = base(); k["fq2q"][0] = 99; cases.append((k, S.P2DE_E_INVALID, "fq2q"))
+    k = base(); k["Jq"][1] *= 2.0; cases.append((k, S.P2DE_E_UNSUPPORTED, "Jq"))
+    k = base(); k["Srsh_nnz"] = k["Srsh_nnz"][:-2]; cases.append((k, S.P2DE_E_UNSUPPORTED, "Srsh_nnz"))
+    k = base(); k["Vf"][fp2.Nfp * 3 + 1] = 0.5; cases.append((k, S.P2DE_E_UNSUPPORTED, "Vf"))
+    k = base(); k["MinvVhT"][1] = 0.25; cases.append((k, S.P2DE_E_UNSUPPORTED, "MinvVhT"))
+    K, Nh = fp2.K, fp2.Nh
+    k = base(); k["GJh"][1 * Nh * K] = 0.1; cases.append((k, S.P2DE_E_UNSUPPORTED, "metric"))
+    for keep, code, what in cases:
+        st, msg = _create_raw(fp2, S._Config(**cfg_fields), keep)
+        assert st == code, (what, st, msg)
+        assert msg
+
+
+def test_null_arguments():
+    L = S.lib()
+    h = C.c_void_p()
+    assert L.p2de_create(None, None, None, None, C.byref(h)) == S.P2DE_E_INVALID
+    assert L.p2de_set_state(None, None) == S.P2DE_E_INVALID
+    assert L.p2de_rhs(None, 0.0, 0.0, 1, None, None) == S.P2DE_E_INVALID
+    assert L.p2de_run(None, 1, None) == S.P2DE_E_INVALID
+    assert L.p2de_num_dofs(None) == 0
