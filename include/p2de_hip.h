@@ -178,6 +178,16 @@ int p2de_synchronize(p2de_ctx *ctx);
 int p2de_set_kernel_timing(p2de_ctx *ctx, int on);
 int p2de_get_kernel_timing(p2de_ctx *ctx, double ms[3], int64_t count[3]);
 
+/* Launch shapes of the element kernel (elements per workgroup E, threads per workgroup NT).
+ * All shapes give bit-identical results.  p2de_autotune times each shape on `steps`
+ * SSP-RK3 steps of the current device state (state and time are restored afterwards),
+ * selects the fastest and, if cache_path is not NULL, records the choice there keyed by
+ * (Nd, N, K); a later p2de_autotune with the same key reads the cache instead of timing. */
+int p2de_num_variants(const p2de_ctx *ctx);
+int p2de_set_variant(p2de_ctx *ctx, int variant);
+int p2de_get_variant(const p2de_ctx *ctx, int *variant, int *E, int *NT);
+int p2de_autotune(p2de_ctx *ctx, int steps, const char *cache_path, int *chosen);
+
 /* Number of degrees of freedom (nodes) K*Nq, and the device address of the state
  * (double[K][Nq][Nc]) for callers that keep data on the GPU. */
 int64_t p2de_num_dofs(const p2de_ctx *ctx);

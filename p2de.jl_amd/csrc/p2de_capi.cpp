@@ -7,7 +7,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <map>
 #include <new>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -38,12 +41,15 @@ struct p2de_ctx {
     double* d_Ival = nullptr;
     double* d_U[2] = {nullptr, nullptr};
     double *d_rhsUA = nullptr, *d_lA = nullptr, *d_dfA = nullptr, *d_rhsU = nullptr;
+    double* d_ut = nullptr;                // projected face states of d_U[0]
+    bool ut_valid = false;                 // d_ut matches d_U[0]
     double *d_dbg_rhsH = nullptr, *d_dbg_rhsL = nullptr, *d_dbg_L = nullptr;
     StepScalars* d_sc = nullptr;
     unsigned long long* d_keys = nullptr;  // [min rho key, min p key, count]
     StepScalars* h_sc = nullptr;           // pinned
     unsigned long long* h_keys = nullptr;  // pinned
     hipStream_t stream = nullptr;
+    int variantA = 0;                      // phase-A launch shape (autotuned)
     bool state_set = false;
     bool debug = false;
     bool timing = false;
@@ -93,7 +99,7 @@ void free_ctx(p2de_ctx* c) {
     }
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
     void* dptrs[] = {c->d_tab, c->d_geo, c->d_nb_node, c->d_nb_face, c->d_bc, c->d_Ival, c->d_U[0], c->d_U[1],
-                     c->d_rhsUA, c->d_lA, c->d_dfA, c->d_rhsU, c->d_dbg_rhsH, c->d_dbg_rhsL, c->d_dbg_L,
+                     c->d_rhsUA, c->d_lA, c->d_dfA, c->d_rhsU, c->d_ut, c->d_dbg_rhsH, c->d_dbg_rhsL, c->d_dbg_L,
                      c->d_sc, c->d_keys};
     for (void* p : dptrs)
         if (p) (void)hipFree(p);
@@ -449,7 +455,8 @@ int p2de_create(const p2de_config* cfg, const p2de_operators* ops, const p2de_ge
             (st = dalloc(c, &c->d_bc, (size_t)K * Nfp)) || (st = dalloc(c, &c->d_Ival, (size_t)std::max(bc->nI, 1) * Nc)) ||
             (st = dalloc(c, &c->d_U[0], nU)) || (st = dalloc(c, &c->d_U[1], nU)) || (st = dalloc(c, &c->d_rhsUA, nU)) ||
             (st = dalloc(c, &c->d_lA, (size_t)K * Nfp)) || (st = dalloc(c, &c->d_dfA, (size_t)K * Nfp * Nc)) ||
-            (st = dalloc(c, &c->d_rhsU, nU)) || (st = dalloc(c, &c->d_sc, 1)) || (st = dalloc(c, &c->d_keys, 3)))
+            (st = dalloc(c, &c->d_rhsU, nU)) || (st = dalloc(c, &c->d_ut, (size_t)K * Nfp * Nc)) ||
+            (st = dalloc(c, &c->d_sc, 1)) || (st = dalloc(c, &c->d_keys, 3)))
             return bail(st);
         if (hipHostMalloc((void**)&c->h_sc, sizeof(StepScalars)) != hipSuccess ||
             hipHostMalloc((void**)&c->h_keys, 3 * sizeof(unsigned long long)) != hipSuccess)
@@ -493,6 +500,7 @@ int p2de_set_state(p2de_ctx* ctx, const double* Uq) {
                                hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     ctx->state_set = true;
+    ctx->ut_valid = false;
     return P2DE_OK;
 }
 
@@ -525,6 +533,7 @@ namespace {
 PhaseAArgs argsA(p2de_ctx* c, const double* U, int stage1) {
     PhaseAArgs a{};
     a.U = U;
+    a.ut = c->d_ut;
     a.T = c->d_tab;
     a.geo = c->d_geo;
     a.nb_node = c->d_nb_node;
@@ -567,8 +576,19 @@ PhaseBArgs argsB(p2de_ctx* c, int mode, const double* W, const double* Uin, doub
     b.Uout = Uout;
     b.rhsU = c->d_rhsU;
     b.dbg_L = c->debug ? c->d_dbg_L : nullptr;
+    b.ut = c->d_ut;
     b.finalize = finalize;
     return b;
+}
+
+// Projected face states of d_U[0] when they are not already current.
+int ensure_face_states(p2de_ctx* c) {
+    if (c->ut_valid) return P2DE_OK;
+    LAUNCHCHK(c, timed(c, 2, [&] {
+                  return launch_face_state(c->Nd, c->N1D, c->d_U[0], c->d_tab, c->K, c->cfg.gamma, c->d_ut, c->stream);
+              }));
+    c->ut_valid = true;
+    return P2DE_OK;
 }
 
 int enqueue_step(p2de_ctx* c) {
@@ -576,15 +596,17 @@ int enqueue_step(p2de_ctx* c) {
     const p2de_config& g = c->cfg;
     double* A = c->d_U[0];
     double* B = c->d_U[1];
+    int st = ensure_face_states(c);
+    if (st) return st;
     LAUNCHCHK(c, timed(c, 2, [&] { return launch_step_init(c->d_sc, g.CFL, g.dt0, g.T, s); }));
     // stage 1: U1 = W + dt*rhsU (SSPRK33.jl:32-33)
-    LAUNCHCHK(c, timed(c, 0, [&] { return launch_phaseA(c->Nd, c->N1D, argsA(c, A, 1), s); }));
+    LAUNCHCHK(c, timed(c, 0, [&] { return launch_phaseA(c->Nd, c->N1D, c->variantA, argsA(c, A, 1), s); }));
     LAUNCHCHK(c, timed(c, 1, [&] { return launch_phaseB(c->Nd, c->N1D, argsB(c, MODE_STAGE1, A, A, B, 0), s); }));
     // stage 2: U2 = 3/4 W + 1/4 (U1 + dt*rhsU) (SSPRK33.jl:34-36)
-    LAUNCHCHK(c, timed(c, 0, [&] { return launch_phaseA(c->Nd, c->N1D, argsA(c, B, 0), s); }));
+    LAUNCHCHK(c, timed(c, 0, [&] { return launch_phaseA(c->Nd, c->N1D, c->variantA, argsA(c, B, 0), s); }));
     LAUNCHCHK(c, timed(c, 1, [&] { return launch_phaseB(c->Nd, c->N1D, argsB(c, MODE_STAGE2, A, B, B, 0), s); }));
     // stage 3: U^{n+1} = 1/3 W + 2/3 (U2 + dt*rhsU) (SSPRK33.jl:37-39), t += dt
-    LAUNCHCHK(c, timed(c, 0, [&] { return launch_phaseA(c->Nd, c->N1D, argsA(c, B, 0), s); }));
+    LAUNCHCHK(c, timed(c, 0, [&] { return launch_phaseA(c->Nd, c->N1D, c->variantA, argsA(c, B, 0), s); }));
     LAUNCHCHK(c, timed(c, 1, [&] { return launch_phaseB(c->Nd, c->N1D, argsB(c, MODE_STAGE3, A, B, A, 1), s); }));
     return P2DE_OK;
 }
@@ -621,7 +643,9 @@ int p2de_rhs(p2de_ctx* ctx, double t, double dt, int nstage, double* dt_out, dou
     // the stage-1 CFL reduction starts from min(CFL*dt0, T - t) (low_order_graph_viscosity.jl:230)
     const double cap = nstage == 1 ? jmin_h(g.CFL * g.dt0, g.T - t) : dt;
     LAUNCHCHK(ctx, timed(ctx, 2, [&] { return launch_rhs_init(ctx->d_sc, dt, cap, s); }));
-    LAUNCHCHK(ctx, timed(ctx, 0, [&] { return launch_phaseA(ctx->Nd, ctx->N1D, argsA(ctx, ctx->d_U[0], nstage == 1), s); }));
+    int st0 = ensure_face_states(ctx);
+    if (st0) return st0;
+    LAUNCHCHK(ctx, timed(ctx, 0, [&] { return launch_phaseA(ctx->Nd, ctx->N1D, ctx->variantA, argsA(ctx, ctx->d_U[0], nstage == 1), s); }));
     LAUNCHCHK(ctx, timed(ctx, 1, [&] {
                   return launch_phaseB(ctx->Nd, ctx->N1D, argsB(ctx, MODE_RHS, ctx->d_U[0], ctx->d_U[0], nullptr, 0), s);
               }));
@@ -780,6 +804,116 @@ int p2de_get_kernel_timing(p2de_ctx* ctx, double ms[3], int64_t count[3]) {
         ms[k] = ctx->time_ms[k];
         count[k] = ctx->time_count[k];
     }
+    return P2DE_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+std::string tune_key(const p2de_ctx* c) {
+    std::ostringstream o;
+    o << "Nd" << c->Nd << "_N" << (c->N1D - 1) << "_K" << c->K;
+    return o.str();
+}
+
+// Tuning cache: one "key variant" pair per line.
+std::map<std::string, int> read_cache(const char* path) {
+    std::map<std::string, int> m;
+    if (!path) return m;
+    std::ifstream f(path);
+    std::string k;
+    int v;
+    while (f >> k >> v) m[k] = v;
+    return m;
+}
+
+void write_cache(const char* path, const std::map<std::string, int>& m) {
+    if (!path) return;
+    std::ofstream f(path, std::ios::trunc);
+    for (auto& kv : m) f << kv.first << " " << kv.second << "\n";
+}
+
+}  // namespace
+
+extern "C" {
+
+int p2de_num_variants(const p2de_ctx* ctx) { return ctx ? num_variants_A(ctx->Nd, ctx->N1D) : 0; }
+
+int p2de_set_variant(p2de_ctx* ctx, int variant) {
+    if (!ctx) return fail(ctx, P2DE_E_INVALID, "NULL context");
+    if (variant < 0 || variant >= num_variants_A(ctx->Nd, ctx->N1D)) return fail(ctx, P2DE_E_INVALID, "variant out of range");
+    ctx->variantA = variant;
+    return P2DE_OK;
+}
+
+int p2de_get_variant(const p2de_ctx* ctx, int* variant, int* E, int* NT) {
+    if (!ctx) return P2DE_E_INVALID;
+    int e = 0, nt = 0;
+    variant_shape_A(ctx->Nd, ctx->N1D, ctx->variantA, &e, &nt);
+    if (variant) *variant = ctx->variantA;
+    if (E) *E = e;
+    if (NT) *NT = nt;
+    return P2DE_OK;
+}
+
+int p2de_autotune(p2de_ctx* ctx, int steps, const char* cache_path, int* chosen) {
+    if (!ctx) return fail(ctx, P2DE_E_INVALID, "NULL context");
+    if (!ctx->state_set) return fail(ctx, P2DE_E_STATE, "no state uploaded (p2de_set_state)");
+    const std::string key = tune_key(ctx);
+    auto cache = read_cache(cache_path);
+    const int nv = num_variants_A(ctx->Nd, ctx->N1D);
+    auto it = cache.find(key);
+    if (it != cache.end() && it->second >= 0 && it->second < nv) {
+        ctx->variantA = it->second;
+        if (chosen) *chosen = ctx->variantA;
+        return P2DE_OK;
+    }
+    // save state and time, time each variant, restore
+    const size_t nU = (size_t)ctx->K * ctx->Nq * ctx->Nc;
+    double* saved = nullptr;
+    int st = dalloc(ctx, &saved, nU);
+    if (st) return st;
+    HIPCHK(ctx, hipMemcpyAsync(saved, ctx->d_U[0], nU * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    if ((st = read_scalars(ctx))) return st;
+    const StepScalars keep = *ctx->h_sc;
+    const bool timing_was = ctx->timing;
+    int best = 0;
+    double best_ms = 1e300;
+    for (int v = 0; v < nv; v++) {
+        ctx->variantA = v;
+        HIPCHK(ctx, hipMemcpyAsync(ctx->d_U[0], saved, nU * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        ctx->ut_valid = false;
+        *ctx->h_sc = keep;
+        HIPCHK(ctx, hipMemcpyAsync(ctx->d_sc, ctx->h_sc, sizeof(StepScalars), hipMemcpyHostToDevice, ctx->stream));
+        if ((st = enqueue_step(ctx))) return st;   // warm-up
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        harvest_timing(ctx);
+        ctx->timing = true;
+        for (int k = 0; k < 3; k++) { ctx->time_ms[k] = 0; ctx->time_count[k] = 0; }
+        for (int i = 0; i < std::max(steps, 1); i++)
+            if ((st = enqueue_step(ctx))) return st;
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        harvest_timing(ctx);
+        const double ms = ctx->time_ms[0] / std::max<int64_t>(ctx->time_count[0], 1);
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = v;
+        }
+        ctx->timing = false;
+    }
+    for (int k = 0; k < 3; k++) { ctx->time_ms[k] = 0; ctx->time_count[k] = 0; }
+    ctx->timing = timing_was;
+    ctx->variantA = best;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_U[0], saved, nU * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    ctx->ut_valid = false;
+    *ctx->h_sc = keep;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_sc, ctx->h_sc, sizeof(StepScalars), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    (void)hipFree(saved);
+    cache[key] = best;
+    write_cache(cache_path, cache);
+    if (chosen) *chosen = best;
     return P2DE_OK;
 }
 

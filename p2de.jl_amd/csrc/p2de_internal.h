@@ -7,6 +7,7 @@
 //   lA         double[K][Nfp]              limiter parameter of each element-boundary
 //                                          subcell face (one per face node)
 //   dfA        double[K][Nfp][Nc]          f_bar_H - f_bar_L at those faces
+//   ut         double[K][Nfp][Nc]          projected face states u(v(U)) of the current state
 //   geo        Geo[K]                      per-element metric (constant inside an element)
 //   nb_node    int32[K][Nfp]               kP*Nq + fq2q[iP] of the mapP target
 //   nb_face    int32[K][Nfp]               kP*Nfp + iP of the mapP target
@@ -84,6 +85,7 @@ struct StepScalars {
 
 struct PhaseAArgs {
     const double* U;
+    const double* ut;             // projected face states u_tilde, double[K][Nfp][Nc]
     const ElemTables* T;
     const Geo* geo;
     const int* nb_node;
@@ -121,12 +123,17 @@ struct PhaseBArgs {
     double* Uout;                 // stage output (may alias Uin or W elementwise)
     double* rhsU;                 // MODE_RHS output
     double* dbg_L;
+    double* ut;                   // projected face states of Uout (stage modes)
     int finalize;                 // stage 3: advance t
 };
 
 // launchers (rhs_kernels.hip); return hipError_t as int
-int launch_phaseA(int Nd, int N1D, const PhaseAArgs& a, void* stream);
+int launch_phaseA(int Nd, int N1D, int variant, const PhaseAArgs& a, void* stream);
+int num_variants_A(int Nd, int N1D);
+int variant_shape_A(int Nd, int N1D, int variant, int* E, int* NT);
 int launch_phaseB(int Nd, int N1D, const PhaseBArgs& a, void* stream);
+int launch_face_state(int Nd, int N1D, const double* U, const ElemTables* T, int K, double gamma, double* ut,
+                      void* stream);
 int launch_step_init(StepScalars* sc, double CFL, double dt0, double T, void* stream);
 int launch_rhs_init(StepScalars* sc, double dt_lim, double dt_cap, void* stream);
 int launch_positivity(const double* U, int64_t nodes, int Nd, double gamma, unsigned long long* keys,

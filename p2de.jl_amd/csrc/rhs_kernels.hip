@@ -3,14 +3,16 @@
 //
 // One stage = two launches:
 //   phase A  (element-local; a workgroup owns E elements, node data staged in LDS)
-//     entropy projection of the face nodes, low-order graph-viscosity RHS, high-order
-//     flux-differencing RHS, subcell flux potentials and the limiter parameters of every
-//     subcell face with the element's own data; writes the element RHS plus, for each
-//     element-boundary subcell face, its parameter and f_bar_H - f_bar_L; at stage 1 it
-//     also reduces the CFL time step (global atomic min).
+//     low-order graph-viscosity RHS, high-order flux-differencing RHS, subcell flux
+//     potentials and the limiter parameters of every subcell face with the element's own
+//     data; writes the element RHS plus, for each element-boundary subcell face, its
+//     parameter and f_bar_H - f_bar_L; at stage 1 it also reduces the CFL time step
+//     (global atomic min).
 //   phase B  (node-parallel) symmetrises the element-boundary parameters with the
-//     neighbour (min), corrects the adjacent nodes' RHS, applies the RK stage and checks
-//     positivity of the new state.
+//     neighbour (min), corrects the adjacent nodes' RHS, applies the RK stage, checks
+//     positivity of the new state and forms its projected face states u(v(u)) once per
+//     boundary node for the next phase A (face_state_kernel does the same for a state
+//     set from the host).
 // The reference runs the same work as ~27 threaded loops per stage over HBM-sized
 // intermediates (src/dg/rhs/*.jl, src/dg/limiter/subcell.jl); the formulas and their
 // operation order are kept (built with -ffp-contract=off) so the FP64 result matches the
@@ -33,17 +35,6 @@ struct Sz {
     static constexpr int NLINE = DIM == 1 ? 1 : 2 * N1D;
     static constexpr int NLS = NQ + N1D;  // first dimension of L_local (State.jl:21)
 };
-
-// elements per workgroup and workgroup size
-template <int DIM, int N1D> struct Cfg;
-template <> struct Cfg<1, 2> { static constexpr int E = 32, NT = 64; };
-template <> struct Cfg<1, 3> { static constexpr int E = 21, NT = 64; };
-template <> struct Cfg<1, 4> { static constexpr int E = 16, NT = 64; };
-template <> struct Cfg<1, 5> { static constexpr int E = 12, NT = 64; };
-template <> struct Cfg<2, 2> { static constexpr int E = 16, NT = 64; };
-template <> struct Cfg<2, 3> { static constexpr int E = 7, NT = 64; };
-template <> struct Cfg<2, 4> { static constexpr int E = 4, NT = 64; };
-template <> struct Cfg<2, 5> { static constexpr int E = 2, NT = 64; };
 
 __device__ __forceinline__ double bits_to_double(unsigned long long b) { return __longlong_as_double((long long)b); }
 __device__ __forceinline__ unsigned long long double_to_bits(double d) { return (unsigned long long)__double_as_longlong(d); }
@@ -126,33 +117,74 @@ __device__ __forceinline__ double limiting_param(const double* uL, const double*
     return l;
 }
 
+// LDS of one workgroup.  Arrays whose lifetimes do not overlap share storage:
+// node/face primitives and fluxes (phases 1-3) with the directional RHS and u_L (phases 4-5),
+// pair results (phases 3-4) with the subcell flux potentials and parameters (phase 5).
 template <int DIM, int N1D, int E>
 struct SmemA {
     using S = Sz<DIM, N1D>;
+    static constexpr int NFB = S::NLINE * (N1D + 1);  // subcell faces per element
     double U[E][S::NQ][S::NC];
-    Prim<DIM> np[E][S::NQ];
-    double nF[E][S::NQ][DIM * S::NC];
-    Prim<DIM> fp[E][S::NFP];
     double BFH[E][S::NFP][S::NC];
     double BFL[E][S::NFP][S::NC];
     double lamB[E][S::NFP];
     union {
         struct {
+            Prim<DIM> np[E][S::NQ];
+            double nF[E][S::NQ][DIM * S::NC];
+            Prim<DIM> fp[E][S::NFP];
+        } g1;
+        struct {
+            double RH[E][S::NQ][DIM][S::NC];
+            double RL[E][S::NQ][DIM][S::NC];
+            double UL[E][S::NQ][S::NC];
+        } g2;
+    } a;
+    union {
+        struct {
             double PH[E][S::NPH][S::NC];
             double PL[E][S::NPL][S::NC + 1];
         } p;
-        double RU[E][S::NQ][DIM][S::NC];
+        struct {
+            double fH[E][NFB][S::NC];
+            double fL[E][NFB][S::NC];
+            double l[E][NFB];
+        } fb;
     } x;
-    double RH[E][S::NQ][DIM][S::NC];
-    double RL[E][S::NQ][DIM][S::NC];
-    double UL[E][S::NQ][S::NC];
     double red[64];
 };
+
+// node of subcell line `line` at position s (0..N1D-1); lines are x lines (sj = line) then
+// y lines (si = line - N1D) in 2D, a single line in 1D
+template <int DIM, int N1D>
+__device__ __forceinline__ int line_node(int line, int s) {
+    if (DIM == 1) return s;
+    return line < N1D ? s + N1D * line : (line - N1D) + N1D * s;
+}
+
+// face node at the element-boundary end of a subcell line (s = 0 or s = N1D)
+template <int DIM, int N1D>
+__device__ __forceinline__ int line_face(int line, int s) {
+    if (DIM == 1) return s == 0 ? 0 : 1;
+    if (line < N1D) return s == 0 ? line : N1D + line;
+    return s == 0 ? 2 * N1D + (line - N1D) : 3 * N1D + (line - N1D);
+}
+
+// entropy_projection_face_node! (rhs.jl:84-94) for an LGL face node:
+// v_tilde = sum_j (1*Vf + 0*Vf_low)[f,j] vq_j with the row's single nonzero, u_tilde = u(v_tilde)
+template <int DIM>
+__device__ __forceinline__ void project_face(double g, double coef, const double* V, double* ut) {
+    constexpr int NC = DIM + 2;
+    double Vt[NC];
+    for (int c = 0; c < NC; c++) Vt[c] = 0.0 + coef * V[c];
+    Eq<DIM>::u_of_v(g, Vt, ut);
+}
 
 template <int DIM, int N1D, int E, int NT>
 __global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
     using S = Sz<DIM, N1D>;
-    constexpr int NQ = S::NQ, NFP = S::NFP, NC = S::NC, NPH = S::NPH, NPL = S::NPL;
+    constexpr int NQ = S::NQ, NFP = S::NFP, NC = S::NC, NPH = S::NPH, NPL = S::NPL, NLINE = S::NLINE;
+    constexpr int NFB = NLINE * (N1D + 1);
     __shared__ SmemA<DIM, N1D, E> sm;
     if (a.sc->done) return;
     const int tid = threadIdx.x;
@@ -175,11 +207,12 @@ __global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
         const int e = w / NQ, i = w - e * NQ;
         if (k0 + e >= K) continue;
         const double* u = sm.U[e][i];
-        make_prim<DIM>(g, u, sm.np[e][i]);          // u_tilde = Uq at volume nodes (rhs.jl:79-82)
-        Eq<DIM>::flux(g, u, sm.nF[e][i]);           // update_wavespeed_and_inviscid_flux! :79-82
+        make_prim<DIM>(g, u, sm.a.g1.np[e][i]);          // u_tilde = Uq at volume nodes (rhs.jl:79-82)
+        Eq<DIM>::flux(g, u, sm.a.g1.nF[e][i]);           // update_wavespeed_and_inviscid_flux! :79-82
     }
 
-    // ---- phase 2: face nodes (entropy projection, LF coefficients, both surface fluxes)
+    // ---- phase 2: face nodes (LF coefficients, both surface fluxes).  The projected face
+    //      states u_tilde were formed once per node by the kernel that wrote the state.
     for (int w = tid; w < E * NFP; w += NT) {
         const int e = w / NFP, f = w - e * NFP;
         const int k = k0 + e;
@@ -189,27 +222,22 @@ __global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
         const Geo G = a.geo[k];
         double B[2], nrm, n[2];
         face_normal<DIM>(G, T.Br[f], T.Bs[f], B, nrm, n);
-
-        // entropy_projection_face_node! (rhs.jl:84-94): v_tilde = sum_j (1*Vf + 0*Vf_low)[f,j] vq_j
-        double V[NC], ut[NC];
-        Eq<DIM>::v_of_u(g, Uq, V);
-        for (int c = 0; c < NC; c++) V[c] = 0.0 + T.vf_coef[f] * V[c];
-        Eq<DIM>::u_of_v(g, V, ut);
-        make_prim<DIM>(g, ut, sm.fp[e][f]);
+        const int fi = k * NFP + f;
+        double ut[NC];
+        for (int c = 0; c < NC; c++) ut[c] = a.ut[(size_t)fi * NC + c];
+        make_prim<DIM>(g, ut, sm.a.g1.fp[e][f]);
 
         // mapP target (calculate_primitive_variables! :54-69; get_uP_and_enforce_BC! :101-108)
-        const int fi = k * NFP + f;
         const int nbn = a.nb_node[fi], nbf = a.nb_face[fi];
         const int kP = nbf / NFP, iP = nbf - kP * NFP;
-        double Ut[NC];
-        for (int c = 0; c < NC; c++) Ut[c] = a.U[(size_t)nbn * NC + c];
+        double Ut[NC], utP[NC];
+        for (int c = 0; c < NC; c++) {
+            Ut[c] = a.U[(size_t)nbn * NC + c];
+            utP[c] = a.ut[(size_t)nbf * NC + c];
+        }
         const Geo GP = a.geo[kP];
         double BP[2], nrmP, nP[2];
         face_normal<DIM>(GP, T.Br[iP], T.Bs[iP], BP, nrmP, nP);
-        double VP[NC], utP[NC];
-        Eq<DIM>::v_of_u(g, Ut, VP);
-        for (int c = 0; c < NC; c++) VP[c] = 0.0 + T.vf_coef[iP] * VP[c];
-        Eq<DIM>::u_of_v(g, VP, utP);
 
         // calculate_interface_dissipation_coeff! (flux_differencing.jl:90-114) + enforce_BC! (:116-151)
         const double lam = Eq<DIM>::wavespeed(g, ut, n);
@@ -250,7 +278,7 @@ __global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
         const double lB = 0.5 * nrm * jmax(wsf, wsP);
         Eq<DIM>::flux(g, uPL, FP);
         for (int c = 0; c < NC; c++) {
-            double fs = 0.5 * (sm.nF[e][q][ld * NC + c] + FP[ld * NC + c]);
+            double fs = 0.5 * (sm.a.g1.nF[e][q][ld * NC + c] + FP[ld * NC + c]);
             double BF = B[ld] * fs;
             double lf = lB * (uPL[c] - Uq[c]);
             sm.BFL[e][f][c] = BF - lf;
@@ -265,8 +293,8 @@ __global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
         const int k = k0 + e;
         if (k >= K) continue;
         const int i = T.hp_i[p], j = T.hp_j[p], dir = T.hp_dir[p];
-        const Prim<DIM>& Pi = i < NQ ? sm.np[e][i] : sm.fp[e][i - NQ];
-        const Prim<DIM>& Pj = j < NQ ? sm.np[e][j] : sm.fp[e][j - NQ];
+        const Prim<DIM>& Pi = i < NQ ? sm.a.g1.np[e][i] : sm.a.g1.fp[e][i - NQ];
+        const Prim<DIM>& Pj = j < NQ ? sm.a.g1.np[e][j] : sm.a.g1.fp[e][j - NQ];
         const Geo G = a.geo[k];
         double Sd;
         if (DIM == 1) Sd = G.rxJ * T.hp_cr[p];
@@ -301,7 +329,7 @@ __global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
         const double ws = jmax(Eq<DIM>::wavespeed(g, ui, nij), Eq<DIM>::wavespeed(g, uj, nji));
         const double lam = nrm * ws;
         for (int c = 0; c < NC; c++) {
-            double Favg = 0.5 * (sm.nF[e][i][dir * NC + c] + sm.nF[e][j][dir * NC + c]);
+            double Favg = 0.5 * (sm.a.g1.nF[e][i][dir * NC + c] + sm.a.g1.nF[e][j][dir * NC + c]);
             double visc = lam * (uj[c] - ui[c]);
             sm.x.p.PL[e][p][c] = 2.0 * S0[dir] * Favg - visc;
         }
@@ -392,11 +420,11 @@ __global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
             rhsL[c] = s;
         }
         // u_L = Uq + dt*rhsL (subcell.jl:269)
-        for (int c = 0; c < NC; c++) sm.UL[e][i][c] = sm.U[e][i][c] + dt_lim * rhsL[c];
+        for (int c = 0; c < NC; c++) sm.a.g2.UL[e][i][c] = sm.U[e][i][c] + dt_lim * rhsL[c];
         for (int d = 0; d < DIM; d++)
             for (int c = 0; c < NC; c++) {
-                sm.RH[e][i][d][c] = rH[d][c];
-                sm.RL[e][i][d][c] = rL[d][c];
+                sm.a.g2.RH[e][i][d][c] = rH[d][c];
+                sm.a.g2.RL[e][i][d][c] = rL[d][c];
             }
         if (a.dbg_rhsH) {
             for (int c = 0; c < NC; c++) {
@@ -422,88 +450,129 @@ __global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
         atomicMin(&a.sc->dt_bits, double_to_bits(m));
     }
 
-    // ---- phase 5: subcell lines: f_bar (subcell.jl:163-206), limiter (:248-349),
-    //      f_bar_lim (:841-876) and the line part of rhsU (:894-924)
-    const double zeta = a.zeta, ZEROTOL = a.ZEROTOL;
-    for (int w = tid; w < E * S::NLINE; w += NT) {
-        const int e = w / S::NLINE, line = w - e * S::NLINE;
+    // ---- phase 5a: subcell flux potentials along every line (accumulate_f_bar! subcell.jl:163-206)
+    for (int w = tid; w < E * NLINE; w += NT) {
+        const int e = w / NLINE, line = w - e * NLINE;
         const int k = k0 + e;
         if (k >= K) continue;
-        const Geo G = a.geo[k];
+        const double J = a.geo[k].J;
         const int d = (DIM == 1 || line < N1D) ? 0 : 1;
-        const int o = d == 0 ? line : line - N1D;    // sj for x lines, si for y lines
-        const double fac = 2.0 * DIM;                 // 2 in 1D, 4 in 2D
-        double fH[NC], fL[NC], fprev[NC];
-        const int f0 = DIM == 1 ? 0 : (d == 0 ? o : 2 * N1D + o);
+        const int f0 = line_face<DIM, N1D>(line, 0);
+        const int base = line * (N1D + 1);
+        double fH[NC], fL[NC];
         for (int c = 0; c < NC; c++) {
             fH[c] = sm.BFH[e][f0][c];
             fL[c] = sm.BFL[e][f0][c];
+            sm.x.fb.fH[e][base][c] = fH[c];
+            sm.x.fb.fL[e][base][c] = fL[c];
         }
-        for (int s = 0; s <= N1D; s++) {
-            if (s > 0) {
-                const int q = DIM == 1 ? s - 1 : (d == 0 ? (s - 1) + N1D * o : o + N1D * (s - 1));
-                const double wJ = T.wq[q] * G.J;
-                for (int c = 0; c < NC; c++) {
-                    fH[c] = fH[c] + wJ * sm.RH[e][q][d][c];
-                    fL[c] = fL[c] + wJ * sm.RL[e][q][d][c];
-                }
+        for (int s = 1; s <= N1D; s++) {
+            const int q = line_node<DIM, N1D>(line, s - 1);
+            const double wJ = T.wq[q] * J;
+            for (int c = 0; c < NC; c++) {
+                fH[c] = fH[c] + wJ * sm.a.g2.RH[e][q][d][c];
+                fL[c] = fL[c] + wJ * sm.a.g2.RL[e][q][d][c];
+                sm.x.fb.fH[e][base + s][c] = fH[c];
+                sm.x.fb.fL[e][base + s][c] = fL[c];
             }
-            double dfb[NC], P[NC];
-            for (int c = 0; c < NC; c++) dfb[c] = fH[c] - fL[c];
-            double l = 1.0;
-            if (s < N1D) {   // node on the right of (above) the subcell face
-                const int q = DIM == 1 ? s : (d == 0 ? s + N1D * o : o + N1D * s);
-                const double wJ = T.wq[q] * G.J;
-                for (int c = 0; c < NC; c++) P[c] = -fac * dt_lim * dfb[c] / wJ;
-                l = jmin(l, limiting_param<DIM>(sm.UL[e][q], P, zeta, ZEROTOL, a.upper_mode));
-            }
-            if (s > 0) {     // node on the left of (below) the subcell face
-                const int q = DIM == 1 ? s - 1 : (d == 0 ? (s - 1) + N1D * o : o + N1D * (s - 1));
-                const double wJ = T.wq[q] * G.J;
-                for (int c = 0; c < NC; c++) P[c] = fac * dt_lim * dfb[c] / wJ;
-                l = jmin(l, limiting_param<DIM>(sm.UL[e][q], P, zeta, ZEROTOL, a.upper_mode));
-            }
-            l = jmin(l, 1.0);   // blending factor of NoShockCapture (shock_capture.jl:111-114)
-            double flim[NC];
-            for (int c = 0; c < NC; c++) flim[c] = l * fH[c] + (1.0 - l) * fL[c];
-            if (s > 0) {
-                const int q = DIM == 1 ? s - 1 : (d == 0 ? (s - 1) + N1D * o : o + N1D * (s - 1));
-                const double wJ = T.wq[q] * G.J;
-                for (int c = 0; c < NC; c++) sm.x.RU[e][q][d][c] = (flim[c] - fprev[c]) / wJ;
-            }
-            for (int c = 0; c < NC; c++) fprev[c] = flim[c];
-            if (s == 0 || s == N1D) {
-                const int f = DIM == 1 ? (s == 0 ? 0 : 1)
-                                       : (d == 0 ? (s == 0 ? o : N1D + o) : (s == 0 ? 2 * N1D + o : 3 * N1D + o));
-                a.lA[(size_t)k * NFP + f] = l;
-                for (int c = 0; c < NC; c++) a.dfA[((size_t)k * NFP + f) * NC + c] = dfb[c];
-            }
-            if (a.dbg_L) {
-                const int idx = DIM == 1 ? s : (d == 0 ? s + (N1D + 1) * o : o + N1D * s);
-                a.dbg_L[idx + (size_t)S::NLS * (d + DIM * (size_t)k)] = l;
-            }
-        }
-        if (DIM == 1 && a.dbg_L) {
-            for (int idx = N1D + 1; idx < S::NLS; idx++) a.dbg_L[idx + (size_t)S::NLS * k] = 1.0;
         }
     }
     __syncthreads();
 
-    // ---- phase 6: rhsU with the element's own limiter parameters
-    {
-        const int nel = min(E, K - k0);
-        double* dst = a.rhsUA + (size_t)k0 * NQ * NC;
-        for (int w = tid; w < nel * NQ * NC; w += NT) {
-            const int e = w / (NQ * NC), r = w - e * NQ * NC, i = r / NC, c = r - i * NC;
-            double s = sm.x.RU[e][i][0][c];
-            if (DIM == 2) s = s + sm.x.RU[e][i][DIM - 1][c];
-            dst[w] = s;
+    // ---- phase 5b: limiter parameter of every subcell face (subcell_bound_limiter! :248-349)
+    const double zeta = a.zeta, ZEROTOL = a.ZEROTOL;
+    constexpr double fac = 2.0 * DIM;                      // 2 in 1D, 4 in 2D
+    for (int w = tid; w < E * NFB; w += NT) {
+        const int e = w / NFB, r = w - e * NFB;
+        const int line = r / (N1D + 1), s = r - line * (N1D + 1);
+        const int k = k0 + e;
+        if (k >= K) continue;
+        const double J = a.geo[k].J;
+        double dfb[NC], P[NC];
+        for (int c = 0; c < NC; c++) dfb[c] = sm.x.fb.fH[e][r][c] - sm.x.fb.fL[e][r][c];
+        double l = 1.0;
+        if (s < N1D) {   // node on the right of (above) the subcell face
+            const int q = line_node<DIM, N1D>(line, s);
+            const double wJ = T.wq[q] * J;
+            for (int c = 0; c < NC; c++) P[c] = -fac * dt_lim * dfb[c] / wJ;
+            l = jmin(l, limiting_param<DIM>(sm.a.g2.UL[e][q], P, zeta, ZEROTOL, a.upper_mode));
         }
+        if (s > 0) {     // node on the left of (below) the subcell face
+            const int q = line_node<DIM, N1D>(line, s - 1);
+            const double wJ = T.wq[q] * J;
+            for (int c = 0; c < NC; c++) P[c] = fac * dt_lim * dfb[c] / wJ;
+            l = jmin(l, limiting_param<DIM>(sm.a.g2.UL[e][q], P, zeta, ZEROTOL, a.upper_mode));
+        }
+        l = jmin(l, 1.0);   // blending factor of NoShockCapture (shock_capture.jl:111-114)
+        sm.x.fb.l[e][r] = l;
+        if (s == 0 || s == N1D) {
+            const int f = line_face<DIM, N1D>(line, s);
+            a.lA[(size_t)k * NFP + f] = l;
+            for (int c = 0; c < NC; c++) a.dfA[((size_t)k * NFP + f) * NC + c] = dfb[c];
+        }
+        if (a.dbg_L) {
+            const int d = (DIM == 1 || line < N1D) ? 0 : 1;
+            const int o = d == 0 ? line : line - N1D;
+            const int idx = DIM == 1 ? s : (d == 0 ? s + (N1D + 1) * o : o + N1D * s);
+            a.dbg_L[idx + (size_t)S::NLS * (d + DIM * (size_t)k)] = l;
+            if (DIM == 1 && s == N1D)
+                for (int x = N1D + 1; x < S::NLS; x++) a.dbg_L[x + (size_t)S::NLS * k] = 1.0;
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 5c: rhsU with the element's own parameters (accumulate_f_bar_limited! :841-876,
+    //      apply_subcell_limiter! :894-924)
+    for (int w = tid; w < E * NQ; w += NT) {
+        const int e = w / NQ, i = w - e * NQ;
+        const int k = k0 + e;
+        if (k >= K) continue;
+        const double wJ = T.wq[i] * a.geo[k].J;
+        const int si = DIM == 1 ? i : i % N1D, sj = DIM == 1 ? 0 : i / N1D;
+        double rx[NC], ry[NC];
+        {
+            const int rl = (DIM == 1 ? 0 : sj) * (N1D + 1) + si;   // left subcell face of the node
+            const double lL = sm.x.fb.l[e][rl], lR = sm.x.fb.l[e][rl + 1];
+            for (int c = 0; c < NC; c++) {
+                const double fl = lL * sm.x.fb.fH[e][rl][c] + (1.0 - lL) * sm.x.fb.fL[e][rl][c];
+                const double fr = lR * sm.x.fb.fH[e][rl + 1][c] + (1.0 - lR) * sm.x.fb.fL[e][rl + 1][c];
+                rx[c] = (fr - fl) / wJ;
+            }
+        }
+        if (DIM == 2) {
+            const int rb = (N1D + si) * (N1D + 1) + sj;            // bottom subcell face of the node
+            const double lB = sm.x.fb.l[e][rb], lT = sm.x.fb.l[e][rb + 1];
+            for (int c = 0; c < NC; c++) {
+                const double fb = lB * sm.x.fb.fH[e][rb][c] + (1.0 - lB) * sm.x.fb.fL[e][rb][c];
+                const double ft = lT * sm.x.fb.fH[e][rb + 1][c] + (1.0 - lT) * sm.x.fb.fL[e][rb + 1][c];
+                ry[c] = (ft - fb) / wJ;
+            }
+        }
+        for (int c = 0; c < NC; c++) a.rhsUA[((size_t)k * NQ + i) * NC + c] = DIM == 2 ? rx[c] + ry[c] : rx[c];
     }
 }
 
+// Projected face states u_tilde of every face node from the state (entropy projection of
+// the face nodes, rhs.jl:84-94), for states that did not come out of phase B.
+template <int DIM, int N1D>
+__global__ __launch_bounds__(256) void face_state_kernel(const double* U, const ElemTables* T, int K, double gamma,
+                                                         double* ut) {
+    using S = Sz<DIM, N1D>;
+    constexpr int NQ = S::NQ, NFP = S::NFP, NC = S::NC;
+    const size_t fi = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (fi >= (size_t)K * NFP) return;
+    const int k = (int)(fi / NFP), f = (int)(fi - (size_t)k * NFP);
+    const double* u = U + ((size_t)k * NQ + T->fq2q[f]) * NC;
+    double V[NC], uu[NC], o[NC];
+    for (int c = 0; c < NC; c++) uu[c] = u[c];
+    Eq<DIM>::v_of_u(gamma, uu, V);
+    project_face<DIM>(gamma, T->vf_coef[f], V, o);
+    for (int c = 0; c < NC; c++) ut[fi * NC + c] = o[c];
+}
+
 // Symmetrise the element-boundary parameters (subcell.jl:405-456), correct rhsU at the
-// adjacent nodes, then the RK stage (SSPRK33.jl:33-39) and the positivity check.
+// adjacent nodes, then the RK stage (SSPRK33.jl:33-39), the positivity check and the
+// projected face states of the new state.
 template <int DIM, int N1D>
 __global__ __launch_bounds__(256) void phaseB_kernel(PhaseBArgs a) {
     using S = Sz<DIM, N1D>;
@@ -587,6 +656,19 @@ __global__ __launch_bounds__(256) void phaseB_kernel(PhaseBArgs a) {
         atomicAdd(&a.sc->viol, 1ull);
         atomicMin(&a.sc->first_bad, ((unsigned long long)a.mode << 40) | (unsigned long long)gidx);
     }
+    // projected face states of the new state, once per node (corner nodes feed two faces)
+    if (T.nf[i] > 0) {
+        double V[NC], o[NC];
+        Eq<DIM>::v_of_u(a.gamma, un, V);
+        double coef_prev = 0.0;
+        for (int m = 0; m < T.nf[i]; m++) {
+            const int f = T.nface[i][m];
+            const double coef = T.vf_coef[f];
+            if (m == 0 || coef != coef_prev) project_face<DIM>(a.gamma, coef, V, o);
+            coef_prev = coef;
+            for (int c = 0; c < NC; c++) a.ut[((size_t)k * NFP + f) * NC + c] = o[c];
+        }
+    }
 }
 
 __global__ void step_init_kernel(StepScalars* sc, double CFL, double dt0, double T) {
@@ -642,9 +724,8 @@ __global__ void positivity_kernel(const double* U, int64_t nodes, double gamma, 
 }
 
 // --------------------------------------------------------------------------- launchers
-template <int DIM, int N1D>
+template <int DIM, int N1D, int E, int NT>
 static int launchA(const PhaseAArgs& a, hipStream_t s) {
-    constexpr int E = Cfg<DIM, N1D>::E, NT = Cfg<DIM, N1D>::NT;
     const int blocks = (a.K + E - 1) / E;
     hipLaunchKernelGGL((phaseA_kernel<DIM, N1D, E, NT>), dim3(blocks), dim3(NT), 0, s, a);
     return (int)hipGetLastError();
@@ -657,24 +738,58 @@ static int launchB(const PhaseBArgs& a, hipStream_t s) {
     return (int)hipGetLastError();
 }
 
-int launch_phaseA(int Nd, int N1D, const PhaseAArgs& a, void* stream) {
-    hipStream_t s = (hipStream_t)stream;
-    if (Nd == 1) {
-        switch (N1D) {
-            case 2: return launchA<1, 2>(a, s);
-            case 3: return launchA<1, 3>(a, s);
-            case 4: return launchA<1, 4>(a, s);
-            case 5: return launchA<1, 5>(a, s);
-        }
-    } else {
-        switch (N1D) {
-            case 2: return launchA<2, 2>(a, s);
-            case 3: return launchA<2, 3>(a, s);
-            case 4: return launchA<2, 4>(a, s);
-            case 5: return launchA<2, 5>(a, s);
-        }
+// Launch shapes of phase A (elements per workgroup E, threads NT).  All variants compute
+// bit-identical results (the gather order does not depend on the shape); the autotuner
+// picks one per (Nd, N).  The first entry is the default.
+struct VariantA {
+    int E, NT;
+    int (*launch)(const PhaseAArgs&, hipStream_t);
+};
+#define VA(D, N, E, T) {E, T, &launchA<D, N, E, T>}
+static const VariantA kV1_2[] = {VA(1, 2, 32, 64), VA(1, 2, 32, 128)};
+static const VariantA kV1_3[] = {VA(1, 3, 21, 64), VA(1, 3, 21, 128)};
+static const VariantA kV1_4[] = {VA(1, 4, 16, 64), VA(1, 4, 16, 128)};
+static const VariantA kV1_5[] = {VA(1, 5, 12, 64), VA(1, 5, 12, 128)};
+static const VariantA kV2_2[] = {VA(2, 2, 16, 64), VA(2, 2, 16, 128), VA(2, 2, 32, 128), VA(2, 2, 32, 256)};
+static const VariantA kV2_3[] = {VA(2, 3, 7, 64), VA(2, 3, 7, 128), VA(2, 3, 14, 128), VA(2, 3, 14, 256), VA(2, 3, 4, 64)};
+static const VariantA kV2_4[] = {VA(2, 4, 4, 64), VA(2, 4, 4, 128), VA(2, 4, 4, 256), VA(2, 4, 8, 128),
+                                 VA(2, 4, 8, 256), VA(2, 4, 2, 64), VA(2, 4, 2, 128)};
+static const VariantA kV2_5[] = {VA(2, 5, 2, 64), VA(2, 5, 2, 128), VA(2, 5, 4, 128), VA(2, 5, 4, 256), VA(2, 5, 1, 64)};
+#undef VA
+
+static const VariantA* variants(int Nd, int N1D, int* n) {
+#define CASE(D, N, arr)                                    \
+    if (Nd == D && N1D == N) {                             \
+        *n = (int)(sizeof(arr) / sizeof(arr[0]));          \
+        return arr;                                        \
     }
-    return (int)hipErrorInvalidValue;
+    CASE(1, 2, kV1_2) CASE(1, 3, kV1_3) CASE(1, 4, kV1_4) CASE(1, 5, kV1_5)
+    CASE(2, 2, kV2_2) CASE(2, 3, kV2_3) CASE(2, 4, kV2_4) CASE(2, 5, kV2_5)
+#undef CASE
+    *n = 0;
+    return nullptr;
+}
+
+int num_variants_A(int Nd, int N1D) {
+    int n = 0;
+    variants(Nd, N1D, &n);
+    return n;
+}
+
+int variant_shape_A(int Nd, int N1D, int v, int* E, int* NT) {
+    int n = 0;
+    const VariantA* t = variants(Nd, N1D, &n);
+    if (!t || v < 0 || v >= n) return -1;
+    *E = t[v].E;
+    *NT = t[v].NT;
+    return 0;
+}
+
+int launch_phaseA(int Nd, int N1D, int variant, const PhaseAArgs& a, void* stream) {
+    int n = 0;
+    const VariantA* t = variants(Nd, N1D, &n);
+    if (!t || variant < 0 || variant >= n) return (int)hipErrorInvalidValue;
+    return t[variant].launch(a, (hipStream_t)stream);
 }
 
 int launch_phaseB(int Nd, int N1D, const PhaseBArgs& a, void* stream) {
@@ -692,6 +807,34 @@ int launch_phaseB(int Nd, int N1D, const PhaseBArgs& a, void* stream) {
             case 3: return launchB<2, 3>(a, s);
             case 4: return launchB<2, 4>(a, s);
             case 5: return launchB<2, 5>(a, s);
+        }
+    }
+    return (int)hipErrorInvalidValue;
+}
+
+template <int DIM, int N1D>
+static int launchF(const double* U, const ElemTables* T, int K, double gamma, double* ut, hipStream_t s) {
+    const size_t n = (size_t)K * Sz<DIM, N1D>::NFP;
+    hipLaunchKernelGGL((face_state_kernel<DIM, N1D>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, U, T, K, gamma, ut);
+    return (int)hipGetLastError();
+}
+
+int launch_face_state(int Nd, int N1D, const double* U, const ElemTables* T, int K, double gamma, double* ut,
+                      void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (Nd == 1) {
+        switch (N1D) {
+            case 2: return launchF<1, 2>(U, T, K, gamma, ut, s);
+            case 3: return launchF<1, 3>(U, T, K, gamma, ut, s);
+            case 4: return launchF<1, 4>(U, T, K, gamma, ut, s);
+            case 5: return launchF<1, 5>(U, T, K, gamma, ut, s);
+        }
+    } else {
+        switch (N1D) {
+            case 2: return launchF<2, 2>(U, T, K, gamma, ut, s);
+            case 3: return launchF<2, 3>(U, T, K, gamma, ut, s);
+            case 4: return launchF<2, 4>(U, T, K, gamma, ut, s);
+            case 5: return launchF<2, 5>(U, T, K, gamma, ut, s);
         }
     }
     return (int)hipErrorInvalidValue;

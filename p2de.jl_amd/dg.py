@@ -83,18 +83,8 @@ def make_periodic(md: MeshData) -> MeshData:
         mapP[1, K - 1] = 0
         return replace(md, mapP=mapP + 1, boundary=np.zeros_like(md.boundary))
     Kx, Ky = Kdims
-    mapP = md.mapP.copy() - 1
-    for k in range(Kx * Ky):
-        ex, ey = k % Kx, k // Kx
-        for m in range(N1D):
-            if ex == 0:
-                mapP[0 * N1D + m, k] = 1 * N1D + m + Nfp * ((Kx - 1) + ey * Kx)
-            if ex == Kx - 1:
-                mapP[1 * N1D + m, k] = 0 * N1D + m + Nfp * (0 + ey * Kx)
-            if ey == 0:
-                mapP[2 * N1D + m, k] = 3 * N1D + m + Nfp * (ex + (Ky - 1) * Kx)
-            if ey == Ky - 1:
-                mapP[3 * N1D + m, k] = 2 * N1D + m + Nfp * (ex + 0 * Kx)
+    from .operators import quad_connectivity
+    mapP, _ = quad_connectivity(Kx, Ky, N1D, True, True)
     return replace(md, mapP=mapP + 1, boundary=np.zeros_like(md.boundary))
 
 

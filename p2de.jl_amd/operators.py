@@ -337,26 +337,35 @@ def uniform_mesh(rd: RefElem, Kdims, xL, xR, periodic) -> MeshData:
     sxJh = np.zeros((Nh, K))
     ryJh = np.zeros((Nh, K))
     syJh = np.full((Nh, K), xr)
-    mapP = np.zeros((Nfp, K), dtype=np.int64)
-    bnd = np.zeros((Nfp, K), dtype=bool)
-    for k in range(K):
-        i, j = ex[k], ey[k]
-        nbr = {}
-        nbr[0] = ((i - 1) % Kx + j * Kx) if (i > 0 or px) else None       # left -> right face
-        nbr[1] = ((i + 1) % Kx + j * Kx) if (i < Kx - 1 or px) else None   # right -> left face
-        nbr[2] = (i + ((j - 1) % Ky) * Kx) if (j > 0 or py) else None     # bottom -> top face
-        nbr[3] = (i + ((j + 1) % Ky) * Kx) if (j < Ky - 1 or py) else None  # top -> bottom face
-        opp = {0: 1, 1: 0, 2: 3, 3: 2}
-        for face in range(4):
-            for m in range(N1D):
-                f = face * N1D + m
-                kP = nbr[face]
-                if kP is None:
-                    mapP[f, k] = f + Nfp * k
-                    bnd[f, k] = True
-                else:
-                    mapP[f, k] = opp[face] * N1D + m + Nfp * kP
+    mapP, bnd = quad_connectivity(Kx, Ky, N1D, px, py)
     return MeshData(2, (Kx, Ky), xq, yq, J, (rxJh, sxJh, ryJh, syJh), mapP + 1, bnd, (hx, hy))
+
+
+def quad_connectivity(Kx: int, Ky: int, N1D: int, px: bool, py: bool):
+    """0-based mapP (Nfp, K) of a Kx x Ky quad grid (elements x-fastest) and the boundary
+    mask.  Face f = face*N1D + m: faces left/right/bottom/top; node m of a face matches node
+    m of the opposite face of the neighbour; a boundary face node maps to itself."""
+    K = Kx * Ky
+    Nfp = 4 * N1D
+    k = np.arange(K)
+    ex, ey = k % Kx, k // Kx
+    m = np.arange(N1D)
+    mapP = np.empty((Nfp, K), dtype=np.int64)
+    bnd = np.zeros((Nfp, K), dtype=bool)
+    specs = [  # (face, opposite face, neighbour element, exists)
+        (0, 1, (ex - 1) % Kx + ey * Kx, (ex > 0) | px),
+        (1, 0, (ex + 1) % Kx + ey * Kx, (ex < Kx - 1) | px),
+        (2, 3, ex + ((ey - 1) % Ky) * Kx, (ey > 0) | py),
+        (3, 2, ex + ((ey + 1) % Ky) * Kx, (ey < Ky - 1) | py),
+    ]
+    for face, opp, nbr, ok in specs:
+        f = face * N1D + m[:, None]
+        target = opp * N1D + m[:, None] + Nfp * nbr[None, :]
+        self_ = f + Nfp * k[None, :]
+        okb = np.broadcast_to(np.asarray(ok)[None, :], (N1D, K))
+        mapP[face * N1D:(face + 1) * N1D, :] = np.where(okb, target, self_)
+        bnd[face * N1D:(face + 1) * N1D, :] = ~okb
+    return mapP, bnd
 
 
 @dataclass

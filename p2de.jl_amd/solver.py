@@ -98,6 +98,10 @@ def lib():
         L.p2de_synchronize.argtypes = [vp]
         L.p2de_set_kernel_timing.argtypes = [vp, C.c_int]
         L.p2de_get_kernel_timing.argtypes = [vp, _dp, _ip]
+        L.p2de_num_variants.argtypes = [vp]
+        L.p2de_set_variant.argtypes = [vp, C.c_int]
+        L.p2de_get_variant.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.p2de_autotune.argtypes = [vp, C.c_int, C.c_char_p, C.POINTER(C.c_int)]
         L.p2de_num_dofs.argtypes = [vp]
         L.p2de_num_dofs.restype = C.c_int64
         L.p2de_device_state.argtypes = [vp, C.POINTER(vp)]
@@ -115,6 +119,7 @@ EXPORTED_SYMBOLS = (
     "p2de_get_field", "p2de_ssprk3_step", "p2de_run", "p2de_set_time", "p2de_get_time",
     "p2de_check_positivity", "p2de_enqueue_steps", "p2de_synchronize", "p2de_set_kernel_timing",
     "p2de_get_kernel_timing", "p2de_num_dofs", "p2de_device_state", "p2de_last_error", "p2de_status_string",
+    "p2de_num_variants", "p2de_set_variant", "p2de_get_variant", "p2de_autotune",
 )
 
 
@@ -246,6 +251,22 @@ class DeviceSolver:
         cnt = (C.c_int64 * 3)()
         self._check(lib().p2de_get_kernel_timing(self.h, ms, cnt))
         return list(ms), list(cnt)
+
+    def num_variants(self) -> int:
+        return int(lib().p2de_num_variants(self.h))
+
+    def set_variant(self, v: int):
+        self._check(lib().p2de_set_variant(self.h, int(v)))
+
+    def variant(self):
+        v, e, nt = C.c_int(), C.c_int(), C.c_int()
+        self._check(lib().p2de_get_variant(self.h, C.byref(v), C.byref(e), C.byref(nt)))
+        return v.value, e.value, nt.value
+
+    def autotune(self, steps: int = 5, cache_path: str = None) -> int:
+        ch = C.c_int(-1)
+        self._check(lib().p2de_autotune(self.h, int(steps), cache_path.encode() if cache_path else None, C.byref(ch)))
+        return ch.value
 
     def num_dofs(self) -> int:
         return int(lib().p2de_num_dofs(self.h))

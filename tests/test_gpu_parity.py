@@ -116,3 +116,34 @@ def test_ssprk3_steps_parity(case):
         rep = dev.check_positivity()
         assert rep.violations == 0 and rep.min_rho > 0 and rep.min_p > 0
     assert abs(t_gpu - t_cpu) <= 1e-14
+
+
+@pytest.mark.parametrize("case", [("vortex_2d", dict(K=8)), ("vortex_2d", dict(K=5, N=4)), ("vortex_2d", dict(K=6, N=2)),
+                                  ("vortex_2d", dict(K=6, N=1)), ("sod_1d", dict())],
+                         ids=["N3", "N4", "N2", "N1", "sod"])
+def test_launch_variants_bit_identical_and_autotune(case, tmp_path):
+    """Every launch shape of the element kernel gives the same bits; the autotuner picks
+    one, restores state and time, and reuses its cache."""
+    from p2de_jl_amd.solver import DeviceSolver
+    name, kw = case
+    solver, state, sp, fp = problem(name, **kw)
+    print(f"case {name} {kw}", flush=True)
+    dev = DeviceSolver(fp)
+    U0 = state.preallocation.Uq
+    out = []
+    for v in range(dev.num_variants()):
+        dev.set_variant(v)
+        dev.set_state(U0)
+        t, dt = dev.step(0.0)
+        t, dt = dev.step(t)
+        out.append(dev.get_state())
+        print(f"  variant {dev.variant()}: t={t:.6e}", flush=True)
+    for U in out[1:]:
+        assert np.array_equal(U, out[0])
+    dev.set_state(U0)
+    dev.set_time(0.0)
+    cache = tmp_path / "tune.txt"
+    v = dev.autotune(2, str(cache))
+    assert 0 <= v < dev.num_variants() and cache.exists()
+    assert np.array_equal(dev.get_state(), U0) and dev.get_time()[0] == 0.0
+    assert dev.autotune(2, str(cache)) == v
