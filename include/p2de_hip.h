@@ -40,7 +40,10 @@ enum { P2DE_NO_ENTROPY_PROJ_LIMITER = 0, P2DE_NODEWISE_SCALED_EXTRAPOLATION = 1 
 enum { P2DE_SUBCELL_LIMITER = 0, P2DE_ZHANG_SHU_LIMITER = 1 };
 enum { P2DE_POSITIVITY_BOUND = 0 };
 enum { P2DE_NO_SHOCK_CAPTURE = 0 };
-enum { P2DE_PRECISION_FP64 = 0 };
+/* FP64: the drop-in, matching the reference's FP64 arithmetic.  MIXED (2D only): FP32 flux
+ * differencing, surface and low-order fluxes and element assembly; FP64 node pressures,
+ * subcell flux potentials, limiter bounds and coefficients, RK stages and state. */
+enum { P2DE_PRECISION_FP64 = 0, P2DE_PRECISION_MIXED = 1 };
 /* Upper density bound of the subcell limiter.  REFERENCE: as limiter_utils.jl:67
  * destructures the 5-value bound tuple of subcell.jl:299 into 4 names, so
  * Urho = Lphi (0 for PositivityBound).  INTENDED: Urho = Urhoe = Inf. */
@@ -63,7 +66,7 @@ typedef struct {
     int bound;                    /* P2DE_POSITIVITY_BOUND */
     int shockcapture;             /* P2DE_NO_SHOCK_CAPTURE */
     int upper_bound_mode;         /* P2DE_UPPER_BOUND_* */
-    int precision;                /* P2DE_PRECISION_FP64 */
+    int precision;                /* P2DE_PRECISION_* */
 } p2de_config;
 
 /* Operators{DIM} (Solver.jl:192-212), sizes from SizeData (Solver.jl:173-183). */
@@ -192,6 +195,50 @@ int p2de_autotune(p2de_ctx *ctx, int steps, const char *cache_path, int *chosen)
  * (double[K][Nq][Nc]) for callers that keep data on the GPU. */
 int64_t p2de_num_dofs(const p2de_ctx *ctx);
 int p2de_device_state(p2de_ctx *ctx, void **dptr);
+
+/* ------------------------------------------------------------------ batched solves
+ * Many independent 2D solves with their own mesh sizes, degrees (N = 1..4), times and time
+ * steps, advanced together in precision P2DE_PRECISION_MIXED.  This has no counterpart in
+ * the reference, which runs one solve per Julia process (SSPRK33.jl:1-62 per instance); each
+ * instance is described exactly as for p2de_create (arrays of n descriptors), and instances
+ * of equal N must pass the same operators.  Per SSP-RK3 step one CFL launch and three fused
+ * stage launches (RHS, limiter, stage update) cover every element of every instance; the
+ * launches of a step are captured once in a hipGraph and replayed.  Instance i's state is
+ * Array{SVector{4,Float64},2}(Nq_i, K_i) as for p2de_set_state. */
+typedef struct p2de_batch p2de_batch;
+int p2de_batch_create(int n, const p2de_config *cfg, const p2de_operators *ops, const p2de_geometry *geom,
+                      const p2de_bcdata *bc, p2de_batch **out);
+void p2de_batch_destroy(p2de_batch *b);
+int p2de_batch_num_instances(const p2de_batch *b);
+int64_t p2de_batch_num_dofs(const p2de_batch *b, int inst);          /* inst < 0: all instances */
+int p2de_batch_set_state(p2de_batch *b, int inst, const double *Uq);
+int p2de_batch_get_state(p2de_batch *b, int inst, double *Uq);
+int p2de_batch_device_state(p2de_batch *b, int inst, void **dptr);
+int p2de_batch_set_time(p2de_batch *b, int inst, double t);          /* inst < 0: all instances */
+int p2de_batch_get_time(p2de_batch *b, int inst, double *t, double *last_dt, int *steps);
+/* rhs! of one instance (as p2de_rhs); the other instances are untouched. */
+int p2de_batch_rhs(p2de_batch *b, int inst, double t, double dt, int nstage, double *dt_out, double *rhsU);
+int p2de_batch_set_debug(p2de_batch *b, int on);                     /* keep rhsH, rhsL of p2de_batch_rhs */
+int p2de_batch_get_field(p2de_batch *b, int inst, int field, double *out);   /* RHSU, RHSH, RHSL */
+/* Queue `steps` SSP-RK3 steps of every instance not yet at its T (no host synchronisation). */
+int p2de_batch_enqueue_steps(p2de_batch *b, int steps);
+int p2de_batch_synchronize(p2de_batch *b);
+/* Step until every instance reaches its T or max_steps steps (max_steps <= 0: no limit). */
+int p2de_batch_run(p2de_batch *b, int max_steps, int *steps_taken);
+int p2de_batch_check_positivity(p2de_batch *b, int inst, p2de_pos_report *rep);
+/* HIP-event timing per launch kind: 0 = fused stage, 1 = CFL pass, 2 = bookkeeping.  While
+ * on, steps are launched directly instead of through the graph. */
+int p2de_batch_set_kernel_timing(p2de_batch *b, int on);
+int p2de_batch_get_kernel_timing(p2de_batch *b, double ms[3], int64_t count[3]);
+int p2de_batch_set_graph(p2de_batch *b, int on);                     /* default on */
+/* Launch shapes (threads NT, elements per tile E[N+1] for each degree) that cover the
+ * batch's degrees; p2de_batch_autotune times each on `steps` steps (state and times are
+ * restored) and keeps the fastest, caching the choice in cache_path when not NULL. */
+int p2de_batch_num_variants(const p2de_batch *b);
+int p2de_batch_set_variant(p2de_batch *b, int v);
+int p2de_batch_get_variant(const p2de_batch *b, int *v, int *NT, int *E /* [6] */);
+int p2de_batch_autotune(p2de_batch *b, int steps, const char *cache_path, int *chosen);
+const char *p2de_batch_last_error(const p2de_batch *b);
 
 const char *p2de_last_error(const p2de_ctx *ctx);
 const char *p2de_status_string(int status);

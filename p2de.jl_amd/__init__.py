@@ -7,14 +7,17 @@ Host-side mirror of the reference interface:
 * ``initialize_DG`` and the state containers (``dg``), as in src/dg/init.jl:44-60;
 * ``rhs`` / ``SSP33`` (``solver``), the drop-ins for ``rhs!`` (src/dg/rhs/rhs.jl:5-13)
   and ``SSP33!`` (src/timestepping/SSPRK33.jl:1-62), which run on the GPU through the
-  C-ABI library ``libp2de_hip.so`` (``include/p2de_hip.h``).
+  C-ABI library ``libp2de_hip.so`` (``include/p2de_hip.h``);
+* ``BatchSolver`` (``solver``), many independent 2D solves advanced together in mixed
+  precision (FP32 bulk, FP64 limiter and RK stages), and the batched workload
+  (``configs.batched_instances``).
 
 Import it with ``__graft_entry__.load_package()``: the directory name is not a Python
 identifier, so it is registered as ``p2de_jl_amd``.
 """
-from . import types, operators, dg, configs  # noqa: F401
+from . import types, operators, dg, configs, timestep  # noqa: F401
 from .types import *  # noqa: F401,F403
 from .dg import (initialize_DG, make_periodic, primitive_to_conservative, flat_problem,  # noqa: F401
                  Solver, State, StateParam, TimeParam)
 from . import solver  # noqa: F401,E402
-from .solver import DeviceSolver, P2DEError, SSP33, rhs  # noqa: F401,E402
+from .solver import BatchSolver, DeviceSolver, P2DEError, SSP33, rhs  # noqa: F401,E402

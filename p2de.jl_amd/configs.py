@@ -11,6 +11,9 @@ Problems (γ = 1.4, ζ = 0.1, POSTOL = 1e-14, ZEROTOL = 5e-16 unless stated):
                          transmissive boundary.
 * ``kelvin_helmholtz``-- periodic [-1,1]^2, 512x512, N=4 (examples/2D/kelvin-helmholtz.jl:9-18).
 * ``smoke_vortex``    -- the reference's own smoke problem (test/test_smoke.jl:6-20,117-129).
+* ``periodic_wave``   -- one instance of the batched workload: periodic [0,1]^2, density
+                         1 + 0.5 sin(2 pi kx x) sin(2 pi ky y), constant velocity, p = 1, with an
+                         optional near-vacuum Gaussian dip (``batched_instances`` draws 48).
 
 ``dt0`` for the first four is ``0.9 * dt_pos(U0) / CFL`` where ``dt_pos`` is the
 low-order positivity time step of the initial state, recorded once (so the CFL cap
@@ -166,12 +169,95 @@ def smoke_vortex(N=3, K=5, T=2e-2, CFL=1.0, dt0=1e-2):
     return _param(N, (K, K), (0.0, 0.0), (10.0, 10.0), T, CFL, dt0, 2, zeta=0.1, eta=0.5), smoke_ic, _periodic_bc
 
 
+# ------------------------------------------------------------------------------- batched workload
+MASK64 = (1 << 64) - 1
+
+
+class SplitMix64:
+    """splitmix64 (Steele, Lea & Flood 2014): the seeded stream the batched workload is drawn from."""
+
+    def __init__(self, seed: int):
+        self.state = seed & MASK64
+
+    def next(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & MASK64
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        return z ^ (z >> 31)
+
+    def integer(self, lo: int, hi: int) -> int:
+        """uniform in {lo, ..., hi}"""
+        return lo + self.next() % (hi - lo + 1)
+
+    def uniform(self, lo: float = 0.0, hi: float = 1.0) -> float:
+        return lo + (hi - lo) * ((self.next() >> 11) * 2.0 ** -53)
+
+
+def periodic_wave_ic(param, x, y, kx=1, ky=1, u0=0.0, v0=0.0, dip=None, sigma=0.05):
+    """rho = 1 + 0.5 sin(2 pi kx x) sin(2 pi ky y), velocity (u0, v0), p = 1; with ``dip = (x0, y0)``
+    the density is blended towards 1e-6 with the Gaussian weight g = exp(-|x - x0|^2 / (2 sigma^2))
+    (nearest periodic image): rho - (rho - 1e-6) g, which is 1e-6 at x0 and positive everywhere
+    (subtracting the fixed amount (rho(x0) - 1e-6) g instead leaves negative densities where
+    the background falls faster than the Gaussian)."""
+    g = param.equation.gamma
+    rho = 1.0 + 0.5 * np.sin(2 * math.pi * kx * x) * np.sin(2 * math.pi * ky * y)
+    if dip is not None:
+        x0, y0 = dip
+        dx = x - x0 - np.round(x - x0)
+        dy = y - y0 - np.round(y - y0)
+        w = np.exp(-(dx * dx + dy * dy) / (2 * sigma * sigma))
+        rho = rho - (rho - 1e-6) * w
+    return primitive_to_conservative(g, (rho, np.full_like(x, u0), np.full_like(x, v0), np.ones_like(x)))
+
+
+def periodic_wave(Kx=32, Ky=32, N=3, kx=1, ky=1, u0=0.0, v0=0.0, dip=None, T=1.0, CFL=0.5, dt0=1e-4):
+    param = _param(N, (Kx, Ky), (0.0, 0.0), (1.0, 1.0), T, CFL, dt0, 2)
+    return param, (lambda prm, x, y: periodic_wave_ic(prm, x, y, kx, ky, u0, v0, dip)), _periodic_bc
+
+
+def batched_draws(count: int = 48, seed: int = 0):
+    """The instance parameters of the batched workload, drawn in this order per instance:
+    Kx, Ky in {32..160}; N in {2, 3, 4}; kx, ky in {1..4}; u0, v0 in [-1, 1]; a dip with
+    probability 0.1 at (x0, y0) uniform in [0, 1)^2."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(count):
+        d = dict(Kx=rng.integer(32, 160), Ky=rng.integer(32, 160), N=rng.integer(2, 4),
+                 kx=rng.integer(1, 4), ky=rng.integer(1, 4), u0=rng.uniform(-1.0, 1.0), v0=rng.uniform(-1.0, 1.0))
+        if rng.uniform() < 0.1:
+            d["dip"] = (rng.uniform(), rng.uniform())
+        out.append(d)
+    return out
+
+
+def batched_instances(count: int = 48, seed: int = 0, T: float = 1.0, CFL: float = 0.5, scale: float = 1.0):
+    """``(solver, state, state_param)`` of every instance of the batched workload, each with
+    dt0 = 0.9 * dt_pos(U0) / CFL from the host CFL estimate.  ``scale`` shrinks the element
+    counts (for quick tests; 1.0 is the workload)."""
+    from .dg import flat_problem
+    from .timestep import cfl_dt
+    out = []
+    for d in batched_draws(count, seed):
+        kw = dict(d)
+        kw["Kx"] = max(2, int(round(kw["Kx"] * scale)))
+        kw["Ky"] = max(2, int(round(kw["Ky"] * scale)))
+        param, ic, bc = periodic_wave(T=T, CFL=CFL, **kw)
+        solver, state, sp = initialize_DG(param, ic, bc)
+        dt = cfl_dt(flat_problem(solver, sp), state.preallocation.Uq)
+        param = replace(param, timestepping_param=replace(param.timestepping_param, dt0=0.9 * dt / CFL))
+        solver = replace(solver, param=param)
+        out.append((solver, state, sp))
+    return out
+
+
 CONFIGS: Dict[str, Callable] = {
     "sod_1d": sod_1d,
     "vortex_2d": vortex_2d,
     "riemann4_2d": riemann4_2d,
     "kelvin_helmholtz": kelvin_helmholtz,
     "smoke_vortex": smoke_vortex,
+    "periodic_wave": periodic_wave,
 }
 
 

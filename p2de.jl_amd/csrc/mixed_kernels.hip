@@ -1,0 +1,884 @@
+// mixed_kernels.hip -- mixed-precision (FP32 bulk, FP64 limiter and RK) fused stage kernels of
+// the limited ESDG scheme on 2D LGL quads, for gfx950.
+//
+// Per SSP-RK3 step: init (per instance: done flag, dt cap, queue reset), the CFL pass
+// (global min of the low-order time step per instance, low_order_graph_viscosity.jl:222-243),
+// three fused stage launches (low- and high-order RHS, subcell limiter, RK stage, positivity
+// check) and a post kernel (t += dt).  Each launch is a persistent workgroup pool: a
+// workgroup takes tiles of E_N elements from the queue chunk of its XCD (blockIdx % 8, then
+// the other chunks), so neighbouring elements share an L2.
+//
+// Algebra that the LGL collocation makes exact, used here instead of computing roundoff:
+//  * the entropy projection of a face node is u(v(u_q)) = u_q (rhs.jl:84-94 with Vf a
+//    selection), so the high-order face state is the node state;
+//  * then the high- and low-order surface fluxes are the same expression at interior faces
+//    (flux_differencing.jl:262-272 and low_order_graph_viscosity.jl:184-198 with equal states
+//    and wave speeds), so f_bar_H - f_bar_L = 0 on element-boundary subcell faces and the
+//    limiter symmetrisation across elements (subcell.jl:418-456) changes nothing;
+//  * a face-volume pair (Nq+f, fq2q[f]) enters MinvVhT*QF1 as (a_h - a_v)*S*f(u) = 0 when
+//    MinvVhT[i,Nq+f] == MinvVhT[i,i] (checked on the host), so those pairs are not formed;
+//  * f_bar on the far end of a line is -BF of the far face (telescoping), which is stored
+//    exactly so that the element-interface fluxes are conservative to FP64 roundoff.
+// The limited update is assembled from the same FP64 flux potentials the limiter saw, so
+// rho >= zeta*rho_L and rhoe >= zeta*rhoe_L hold to FP64 roundoff exactly as in the FP64 path.
+#include <hip/hip_runtime.h>
+
+#include "euler_device.h"
+#include "p2de_internal.h"
+#include "p2de_mixed.h"
+
+namespace p2de {
+namespace {
+
+__device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float sqrtf_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// FP64 reciprocal: hardware estimate plus two Newton steps (about 1 ulp)
+__device__ __forceinline__ double rcp64(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+__device__ __forceinline__ double bits_to_d(unsigned long long b) { return __longlong_as_double((long long)b); }
+__device__ __forceinline__ unsigned long long d_to_bits(double d) { return (unsigned long long)__double_as_longlong(d); }
+
+// FP32 node state for the pair fluxes.  wsx/wsy are the wave speeds of wavespeed_estimate
+// (compressible_Navier_Stokes.jl:48-62) along x and y: the rotated 1D state (rho, m.n, E)
+// keeps the tangential kinetic energy inside its "internal" energy.
+struct NodeP {
+    float rho, u, v, p, beta, E, wsx, wsy;
+};
+
+// pfun / betafun / wavespeed with the cancellation-prone energy differences in FP64
+__device__ __forceinline__ NodeP node_prims(double g, const double* U) {
+    const double ri = rcp64(U[0]);
+    const double u = U[1] * ri, v = U[2] * ri;
+    const double kx = 0.5 * U[1] * u, ky = 0.5 * U[2] * v;
+    const double gm1 = g - 1.0;
+    const double p = gm1 * (U[3] - (kx + ky));
+    const double px = gm1 * (U[3] - kx), py = gm1 * (U[3] - ky);
+    NodeP P;
+    P.rho = (float)U[0];
+    P.u = (float)u;
+    P.v = (float)v;
+    P.p = (float)p;
+    P.E = (float)U[3];
+    P.beta = 0.5f * P.rho * rcpf(P.p);
+    const float gri = (float)(g * ri);
+    P.wsx = fabsf(P.u) + sqrtf_fast(gri * (float)px);
+    P.wsy = fabsf(P.v) + sqrtf_fast(gri * (float)py);
+    return P;
+}
+
+// Euler flux in direction d (fluxes, compressible_Navier_Stokes.jl:175-194)
+__device__ __forceinline__ void flux_dir(const NodeP& P, int d, float F[4]) {
+    const float un = d == 0 ? P.u : P.v;
+    const float m = P.rho * un;
+    F[0] = m;
+    F[1] = fmaf(m, P.u, d == 0 ? P.p : 0.f);
+    F[2] = fmaf(m, P.v, d == 0 ? 0.f : P.p);
+    F[3] = un * (P.E + P.p);
+}
+
+// Logarithmic mean in ratio form: ln(b/a) = 2 atanh(f), f = (b-a)/(b+a), so the mean is
+// (a+b) / (2 (1 + f^2/3 + f^4/5 + f^6/7 + ...)) (Ismail & Roe 2009).  Its relative accuracy
+// does not depend on the magnitude of a and b (near-vacuum densities and betas included);
+// the series (truncation < 1.2e-9) is used for f^2 < 1e-2, the logarithm of the ratio above.
+// The reference's series (compressible_Navier_Stokes.jl:307-321) differs from the exact
+// mean by < 1.2e-9 relative, below FP32 resolution.
+__device__ __forceinline__ float logmean_f(float a, float b) {
+    const float s = a + b, d = b - a;
+    const float f = d * rcpf(s);
+    const float u = f * f;
+    const float F = fmaf(u, fmaf(u, fmaf(u, 1.f / 7.f, 0.2f), 1.f / 3.f), 1.f);
+    const float ser = 0.5f * s * rcpf(F);
+    const float lg = d * rcpf(__logf(b * rcpf(a)));
+    return u < 1e-2f ? ser : lg;
+}
+
+// Chandrashekar flux fS (compressible_Navier_Stokes.jl:196-249), components of direction d
+__device__ __forceinline__ void fS_f(float g2m1, const NodeP& L, const NodeP& R, int d, float F[4]) {
+    const float rl = logmean_f(L.rho, R.rho), bl = logmean_f(L.beta, R.beta);
+    const float ra = 0.5f * (L.rho + R.rho), ua = 0.5f * (L.u + R.u), va = 0.5f * (L.v + R.v);
+    const float un = fmaf(L.u, R.u, L.v * R.v);
+    const float pa = ra * rcpf(L.beta + R.beta);
+    const float f4 = fmaf(0.5f * rl, un, fmaf(rl, rcpf(g2m1 * bl), pa));
+    const float vn = d == 0 ? ua : va;
+    const float m = rl * vn;
+    F[0] = m;
+    F[1] = fmaf(m, ua, d == 0 ? pa : 0.f);
+    F[2] = fmaf(m, va, d == 0 ? 0.f : pa);
+    F[3] = f4 * vn;
+}
+
+// limiting_param(::SubcellLimiter, ::PositivityBound) (limiter_utils.jl:10-14, 26-40, 52-90) in
+// FP64 with precomputed bounds Lrho = zeta*rho(uL), Lrhoe = zeta*rhoe(uL).  upper_mode 0
+// keeps the reference's bound tuple destructuring (Urho = 0, limiter_utils.jl:67).
+__device__ __forceinline__ double lim_param(const double* uL, double Lrho, double Lrhoe, const double* P,
+                                            double ZEROTOL, int upper_mode) {
+    double l = 1.0;
+    const double r0 = uL[0] + P[0];
+    if (r0 < Lrho) l = fmax((Lrho - uL[0]) * rcp64(P[0]), 0.0);
+    if (upper_mode == 0 && r0 > 0.0) l = fmin(l, fmax(-uL[0] * rcp64(P[0]), 0.0));
+    const double a = P[0] * P[3] - 0.5 * (P[1] * P[1] + P[2] * P[2]);
+    const double b = uL[3] * P[0] + uL[0] * P[3] - uL[1] * P[1] - uL[2] * P[2] - P[0] * Lrhoe;
+    const double c = uL[3] * uL[0] - 0.5 * (uL[1] * uL[1] + uL[2] * uL[2]) - uL[0] * Lrhoe;
+    const double disc = b * b - 4.0 * a * c;
+    // a == 0 gives non-finite roots that fail every test of rhoe_quadratic_solve: l stays 1
+    if (disc >= 0.0 && a != 0.0) {
+        const double sq = sqrt(disc), i2a = rcp64(2.0 * a);
+        const double x1 = (-b + sq) * i2a, x2 = (-b - sq) * i2a;
+        double q = 1.0;
+        if (x1 > ZEROTOL && x2 > ZEROTOL) q = fmin(x1, x2);
+        else if (x1 > ZEROTOL && x2 < -ZEROTOL) q = x1;
+        else if (x2 > ZEROTOL && x1 < -ZEROTOL) q = x2;
+        l = fmin(l, q);
+    }
+    return l;
+}
+
+template <int N1D>
+__device__ __forceinline__ int lnode(int d, int line, int s) {
+    return d == 0 ? s + N1D * line : line + N1D * s;
+}
+// volume node of face node f (faces r=-1, r=+1, s=-1, s=+1)
+template <int N1D>
+__device__ __forceinline__ int face_node(int f) {
+    const int face = f / N1D, m = f - face * N1D;
+    return face == 0 ? m * N1D : face == 1 ? m * N1D + N1D - 1 : face == 2 ? m : (N1D - 1) * N1D + m;
+}
+// a of pair index q = a(a-1)/2 + b, b < a
+__device__ __forceinline__ void pair_ab(int q, int& a, int& b) {
+    a = 1;
+    while (q >= a) {
+        q -= a;
+        a++;
+    }
+    b = q;
+}
+
+template <int N1D, int E>
+struct SmemM {
+    static constexpr int NQ = N1D * N1D, NFP = 4 * N1D;
+    static constexpr int NPV = MixTab<N1D>::NPV, NPL = MixTab<N1D>::NPL, NFB = 2 * N1D * (N1D + 1);
+    double U[E][NQ][4];
+    double geo[E][4];            // J, rxJ, syJ, 1/J
+    float BF[E][NFP][4];         // B*f* - LF (high order == low order at interior faces)
+    float BFL[E][NFP][4];        // low-order surface term (differs at inflow faces)
+    float RH[E][NQ][2][4];       // rhsxyH
+    float RL[E][NQ][2][4];       // rhsxyL
+    union {
+        struct {
+            NodeP np[E][NQ];
+            float PH[E][NPV][4];
+            float PL[E][NPL][4];
+        } a;
+        struct {
+            double fL[E][NFB][4];    // low-order subcell flux potentials
+            double df[E][NFB][4];    // f_bar_H - f_bar_L
+            double l[E][NFB];
+            double UL[E][NQ][4];
+            double Lrhoe[E][NQ];
+        } b;
+    } x;
+};
+
+template <int N1D, int E>
+struct SmemDt {
+    static constexpr int NQ = N1D * N1D;
+    float ws[E][NQ][2];
+    double geo[E][3];
+    double red[16];
+};
+
+template <int N1D, int E, int NT>
+__device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, const MixTab<N1D>& T, int k0,
+                                           int nel, SmemM<N1D, E>& sm) {
+    using Sm = SmemM<N1D, E>;
+    constexpr int NQ = Sm::NQ, NFP = Sm::NFP, NPV = Sm::NPV, NPL = Sm::NPL, NFB = Sm::NFB;
+    constexpr int NP1 = MixTab<N1D>::NP1;
+    const int tid = threadIdx.x;
+    const double gd = I.gamma;
+    const float g2m1 = (float)(2.0 * (gd - 1.0));
+    const double* Uin = a.Uin + I.node_off * 4;
+    StepScalars* sc = I.sc;
+
+    // ---- 0: element states and metric -> LDS
+    {
+        const double* src = Uin + (size_t)k0 * NQ * 4;
+        double* dst = &sm.U[0][0][0];
+        for (int w = tid; w < nel * NQ * 4; w += NT) dst[w] = src[w];
+        for (int w = tid; w < nel; w += NT) {
+            const Geo G = I.geo[k0 + w];
+            sm.geo[w][0] = G.J;
+            sm.geo[w][1] = G.rxJ;
+            sm.geo[w][2] = G.syJ;
+            sm.geo[w][3] = 1.0 / G.J;
+        }
+    }
+    __syncthreads();
+
+    // ---- 1: node primitives (FP64 energy differences -> FP32)
+#pragma unroll 1
+    for (int w = tid; w < E * NQ; w += NT) {
+        const int e = w / NQ, i = w - e * NQ;
+        if (e >= nel) continue;
+        sm.x.a.np[e][i] = node_prims(gd, sm.U[e][i]);
+    }
+    __syncthreads();
+
+    // ---- 2: surface fluxes at the face nodes (flux_differencing.jl:90-151, 223-272;
+    //      low_order_graph_viscosity.jl:51-124, 175-204)
+#pragma unroll 1
+    for (int w = tid; w < E * NFP; w += NT) {
+        const int e = w / NFP, f = w - e * NFP;
+        if (e >= nel) continue;
+        const int k = k0 + e;
+        const int ld = f < 2 * N1D ? 0 : 1;
+        const int q = face_node<N1D>(f);
+        const NodeP P = sm.x.a.np[e][q];
+        const size_t fi = (size_t)k * NFP + f;
+        const int bcc = I.bc[fi];
+        const int nb = I.nb_node[fi];
+        double UP[4];
+        for (int c = 0; c < 4; c++) UP[c] = Uin[(size_t)nb * 4 + c];
+        const NodeP PP = node_prims(gd, UP);
+        // |B| from both elements' metric, so that the two sides of an interface compute exactly
+        // negated fluxes (the per-element metrics agree to FP64 roundoff, which FP32 rounding
+        // could otherwise turn into a one-ulp mismatch and a conservation defect)
+        const Geo& GP = I.geo[nb / NQ];
+        const double gsym = 0.5 * (sm.geo[e][1 + ld] + (ld == 0 ? GP.rxJ : GP.syJ));
+        const float Bmag = (float)gsym * fabsf(T.Bf[f]);
+        const float Bf = T.Bf[f] < 0.f ? -Bmag : Bmag;
+        const float LF = 0.5f * Bmag * fmaxf(ld == 0 ? P.wsx : P.wsy, ld == 0 ? PP.wsx : PP.wsy);
+        float F[4], FP[4];
+        flux_dir(P, ld, F);
+        const double* Uq = sm.U[e][q];
+        if (bcc == 0) {
+            flux_dir(PP, ld, FP);
+            for (int c = 0; c < 4; c++) {
+                const float v = fmaf(Bf, 0.5f * (F[c] + FP[c]), -LF * (float)(UP[c] - Uq[c]));
+                sm.BF[e][f][c] = v;
+                sm.BFL[e][f][c] = v;
+            }
+        } else if (bcc < 0) {   // outflow: uP = own node, no dissipation
+            for (int c = 0; c < 4; c++) {
+                const float v = Bf * F[c];
+                sm.BF[e][f][c] = v;
+                sm.BFL[e][f][c] = v;
+            }
+        } else {                // inflow: uP = Ival; the low-order flux keeps its dissipation
+            double UI[4];
+            for (int c = 0; c < 4; c++) UI[c] = I.Ival[(size_t)(bcc - 1) * 4 + c];
+            const NodeP PI = node_prims(gd, UI);
+            flux_dir(PI, ld, FP);
+            for (int c = 0; c < 4; c++) {
+                const float v = Bf * (0.5f * (F[c] + FP[c]));
+                sm.BF[e][f][c] = v;
+                sm.BFL[e][f][c] = v - LF * (float)(UI[c] - Uq[c]);
+            }
+        }
+    }
+
+    // ---- 3a: high-order volume pairs (flux_differencing_volume!, flux_differencing.jl:164-211)
+#pragma unroll 1
+    for (int w = tid; w < E * NPV; w += NT) {
+        const int e = w / NPV, p = w - e * NPV;
+        if (e >= nel) continue;
+        const int d = p / (N1D * NP1), r = p - d * (N1D * NP1), line = r / NP1;
+        int aa, bb;
+        pair_ab(r - line * NP1, aa, bb);
+        const int i = lnode<N1D>(d, line, aa), j = lnode<N1D>(d, line, bb);
+        const float Sd = (float)sm.geo[e][1 + d] * T.cH[p];
+        float F[4];
+        fS_f(g2m1, sm.x.a.np[e][i], sm.x.a.np[e][j], d, F);
+        for (int c = 0; c < 4; c++) sm.x.a.PH[e][p][c] = Sd * F[c];
+    }
+    // ---- 3b: low-order line pairs (accumulate_low_order_rhs_volume!, :139-173)
+#pragma unroll 1
+    for (int w = tid; w < E * NPL; w += NT) {
+        const int e = w / NPL, p = w - e * NPL;
+        if (e >= nel) continue;
+        const int d = p / (N1D * (N1D - 1)), r = p - d * (N1D * (N1D - 1)), line = r / (N1D - 1);
+        const int s = r - line * (N1D - 1);
+        const int i = lnode<N1D>(d, line, s + 1), j = lnode<N1D>(d, line, s);
+        const float S0 = (float)sm.geo[e][1 + d] * T.cL[p];
+        const NodeP& Pi = sm.x.a.np[e][i];
+        const NodeP& Pj = sm.x.a.np[e][j];
+        const float lam = fabsf(S0) * fmaxf(d == 0 ? Pi.wsx : Pi.wsy, d == 0 ? Pj.wsx : Pj.wsy);
+        float Fi[4], Fj[4];
+        flux_dir(Pi, d, Fi);
+        flux_dir(Pj, d, Fj);
+        for (int c = 0; c < 4; c++)
+            sm.x.a.PL[e][p][c] = fmaf(S0, Fi[c] + Fj[c], -lam * (float)(sm.U[e][j][c] - sm.U[e][i][c]));
+    }
+    __syncthreads();
+
+    // ---- 4: node assembly (assemble_rhs!, flux_differencing.jl:331-361; low-order surface and
+    //      mass scaling, low_order_graph_viscosity.jl:175-220)
+#pragma unroll 1
+    for (int w = tid; w < E * NQ; w += NT) {
+        const int e = w / NQ, i = w - e * NQ;
+        if (e >= nel) continue;
+        const int ia = i % N1D, ib = i / N1D;
+        float Qx[4] = {0.f, 0.f, 0.f, 0.f}, Qy[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int o = 0; o < N1D; o++) {
+            if (o == ia) continue;
+            const int hi = ia > o ? ia : o, lo = ia > o ? o : ia;
+            const float sg = ia > o ? 1.f : -1.f;
+            const int p = ib * NP1 + hi * (hi - 1) / 2 + lo;
+            for (int c = 0; c < 4; c++) Qx[c] = fmaf(sg, sm.x.a.PH[e][p][c], Qx[c]);
+        }
+        for (int o = 0; o < N1D; o++) {
+            if (o == ib) continue;
+            const int hi = ib > o ? ib : o, lo = ib > o ? o : ib;
+            const float sg = ib > o ? 1.f : -1.f;
+            const int p = N1D * NP1 + ia * NP1 + hi * (hi - 1) / 2 + lo;
+            for (int c = 0; c < 4; c++) Qy[c] = fmaf(sg, sm.x.a.PH[e][p][c], Qy[c]);
+        }
+        const float av = T.a_v[i];
+        float hx[4], hy[4];
+        for (int c = 0; c < 4; c++) {
+            hx[c] = av * Qx[c];
+            hy[c] = av * Qy[c];
+        }
+        // low order: -Q0F1 of the line neighbours
+        float lx[4] = {0.f, 0.f, 0.f, 0.f}, ly[4] = {0.f, 0.f, 0.f, 0.f};
+        constexpr int LW = N1D - 1;
+        if (ia > 0)
+            for (int c = 0; c < 4; c++) lx[c] -= sm.x.a.PL[e][ib * LW + ia - 1][c];
+        if (ia < N1D - 1)
+            for (int c = 0; c < 4; c++) lx[c] += sm.x.a.PL[e][ib * LW + ia][c];
+        if (ib > 0)
+            for (int c = 0; c < 4; c++) ly[c] -= sm.x.a.PL[e][N1D * LW + ia * LW + ib - 1][c];
+        if (ib < N1D - 1)
+            for (int c = 0; c < 4; c++) ly[c] += sm.x.a.PL[e][N1D * LW + ia * LW + ib][c];
+        // face terms (at most one r-face and one s-face per node)
+        const int fx = ia == 0 ? ib : ia == N1D - 1 ? N1D + ib : -1;
+        const int fy = ib == 0 ? 2 * N1D + ia : ib == N1D - 1 ? 3 * N1D + ia : -1;
+        if (fx >= 0) {
+            const float af = T.a_fx[i];
+            for (int c = 0; c < 4; c++) {
+                hx[c] = fmaf(af, sm.BF[e][fx][c], hx[c]);
+                lx[c] -= sm.BFL[e][fx][c];
+            }
+        }
+        if (fy >= 0) {
+            const float af = T.a_fy[i];
+            for (int c = 0; c < 4; c++) {
+                hy[c] = fmaf(af, sm.BF[e][fy][c], hy[c]);
+                ly[c] -= sm.BFL[e][fy][c];
+            }
+        }
+        const float nJ = -(float)sm.geo[e][3];
+        const float iwJ = (float)(sm.geo[e][3] / T.wq[i]);
+        for (int c = 0; c < 4; c++) {
+            sm.RH[e][i][0][c] = hx[c] * nJ;
+            sm.RH[e][i][1][c] = hy[c] * nJ;
+            sm.RL[e][i][0][c] = lx[c] * iwJ;
+            sm.RL[e][i][1][c] = ly[c] * iwJ;
+        }
+        if (a.dbg_rhsH) {
+            const size_t o = ((size_t)I.node_off + (size_t)(k0 + e) * NQ + i) * 4;
+            for (int c = 0; c < 4; c++) {
+                a.dbg_rhsH[o + c] = (double)(hx[c] * nJ) + (double)(hy[c] * nJ);
+                a.dbg_rhsL[o + c] = (double)(lx[c] * iwJ) + (double)(ly[c] * iwJ);
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- 5a: subcell flux potentials in FP64 (accumulate_f_bar!, subcell.jl:163-206)
+#pragma unroll 1
+    for (int w = tid; w < E * 2 * N1D; w += NT) {
+        const int e = w / (2 * N1D), L = w - e * (2 * N1D);
+        if (e >= nel) continue;
+        const int d = L / N1D, line = L - d * N1D;
+        const double J = sm.geo[e][0];
+        const int f0 = d == 0 ? line : 2 * N1D + line;
+        const int f1 = d == 0 ? N1D + line : 3 * N1D + line;
+        const int base = L * (N1D + 1);
+        double fh[4], fl[4];
+        for (int c = 0; c < 4; c++) {
+            fh[c] = (double)sm.BF[e][f0][c];
+            fl[c] = (double)sm.BFL[e][f0][c];
+            sm.x.b.fL[e][base][c] = fl[c];
+            sm.x.b.df[e][base][c] = fh[c] - fl[c];
+        }
+        for (int s = 1; s < N1D; s++) {
+            const int q = lnode<N1D>(d, line, s - 1);
+            const double wJ = T.wq[q] * J;
+            for (int c = 0; c < 4; c++) {
+                fh[c] = fma(wJ, (double)sm.RH[e][q][d][c], fh[c]);
+                fl[c] = fma(wJ, (double)sm.RL[e][q][d][c], fl[c]);
+                sm.x.b.fL[e][base + s][c] = fl[c];
+                sm.x.b.df[e][base + s][c] = fh[c] - fl[c];
+            }
+        }
+        for (int c = 0; c < 4; c++) {
+            const double h = -(double)sm.BF[e][f1][c], lo = -(double)sm.BFL[e][f1][c];
+            sm.x.b.fL[e][base + N1D][c] = lo;
+            sm.x.b.df[e][base + N1D][c] = h - lo;
+        }
+    }
+    __syncthreads();
+
+    // ---- 5u: u_L from the low-order potentials and the bounds (subcell.jl:269, 370-387)
+    const double dt = bits_to_d(sc->dt_bits);
+    const double dt_lim = (a.mode == MODE_STAGE2 || a.mode == MODE_STAGE3) ? dt : sc->dt_lim;
+    const double zeta = I.zeta;
+#pragma unroll 1
+    for (int w = tid; w < E * NQ; w += NT) {
+        const int e = w / NQ, i = w - e * NQ;
+        if (e >= nel) continue;
+        const int ia = i % N1D, ib = i / N1D;
+        const int rx = ib * (N1D + 1) + ia, ry = (N1D + ia) * (N1D + 1) + ib;
+        const double iwJ = sm.geo[e][3] * rcp64(T.wq[i]);
+        double uL[4];
+        for (int c = 0; c < 4; c++) {
+            const double r = (sm.x.b.fL[e][rx + 1][c] - sm.x.b.fL[e][rx][c]) +
+                             (sm.x.b.fL[e][ry + 1][c] - sm.x.b.fL[e][ry][c]);
+            uL[c] = sm.U[e][i][c] + dt_lim * (r * iwJ);
+            sm.x.b.UL[e][i][c] = uL[c];
+        }
+        sm.x.b.Lrhoe[e][i] = zeta * (uL[3] - 0.5 * (uL[1] * uL[1] + uL[2] * uL[2]) * rcp64(uL[0]));
+    }
+    __syncthreads();
+
+    // ---- 5b: limiter parameter of every subcell face (subcell_bound_limiter!, subcell.jl:248-349)
+    const double fac_dt = 4.0 * dt_lim;
+#pragma unroll 1
+    for (int w = tid; w < E * NFB; w += NT) {
+        const int e = w / NFB, r = w - e * NFB;
+        if (e >= nel) continue;
+        const int L = r / (N1D + 1), s = r - L * (N1D + 1);
+        const int d = L / N1D, line = L - d * N1D;
+        double l = 1.0;
+        double P[4];
+        if (s < N1D) {   // node above/right of the face
+            const int q = lnode<N1D>(d, line, s);
+            const double sc_ = -fac_dt * sm.geo[e][3] * rcp64(T.wq[q]);
+            for (int c = 0; c < 4; c++) P[c] = sc_ * sm.x.b.df[e][r][c];
+            const double* uL = sm.x.b.UL[e][q];
+            l = fmin(l, lim_param(uL, zeta * uL[0], sm.x.b.Lrhoe[e][q], P, I.ZEROTOL, I.upper_mode));
+        }
+        if (s > 0) {     // node below/left of the face
+            const int q = lnode<N1D>(d, line, s - 1);
+            const double sc_ = fac_dt * sm.geo[e][3] * rcp64(T.wq[q]);
+            for (int c = 0; c < 4; c++) P[c] = sc_ * sm.x.b.df[e][r][c];
+            const double* uL = sm.x.b.UL[e][q];
+            l = fmin(l, lim_param(uL, zeta * uL[0], sm.x.b.Lrhoe[e][q], P, I.ZEROTOL, I.upper_mode));
+        }
+        sm.x.b.l[e][r] = l;
+    }
+    __syncthreads();
+
+    // ---- 5c: limited RHS (accumulate_f_bar_limited!, apply_subcell_limiter!, subcell.jl:826-924),
+    //      RK stage (SSPRK33.jl:30-39) and positivity of the new state
+    const size_t ib0 = (size_t)I.node_off * 4;
+#pragma unroll 1
+    for (int w = tid; w < E * NQ; w += NT) {
+        const int e = w / NQ, i = w - e * NQ;
+        if (e >= nel) continue;
+        const int ia = i % N1D, ib = i / N1D;
+        const int rx = ib * (N1D + 1) + ia, ry = (N1D + ia) * (N1D + 1) + ib;
+        const double iwJ = sm.geo[e][3] * rcp64(T.wq[i]);
+        const double lx0 = sm.x.b.l[e][rx], lx1 = sm.x.b.l[e][rx + 1];
+        const double ly0 = sm.x.b.l[e][ry], ly1 = sm.x.b.l[e][ry + 1];
+        double r[4];
+        for (int c = 0; c < 4; c++) {
+            const double gx0 = fma(lx0, sm.x.b.df[e][rx][c], sm.x.b.fL[e][rx][c]);
+            const double gx1 = fma(lx1, sm.x.b.df[e][rx + 1][c], sm.x.b.fL[e][rx + 1][c]);
+            const double gy0 = fma(ly0, sm.x.b.df[e][ry][c], sm.x.b.fL[e][ry][c]);
+            const double gy1 = fma(ly1, sm.x.b.df[e][ry + 1][c], sm.x.b.fL[e][ry + 1][c]);
+            r[c] = ((gx1 - gx0) + (gy1 - gy0)) * iwJ;
+        }
+        const size_t o = ib0 + ((size_t)(k0 + e) * NQ + i) * 4;
+        if (a.mode == MODE_RHS) {
+            for (int c = 0; c < 4; c++) a.rhsU[o + c] = r[c];
+            continue;
+        }
+        const double* u = sm.U[e][i];
+        double un[4];
+        if (a.mode == MODE_STAGE1) {
+            for (int c = 0; c < 4; c++) un[c] = fma(dt, r[c], u[c]);
+        } else {
+            const double wa = a.mode == MODE_STAGE2 ? 3.0 / 4.0 : 1.0 / 3.0;
+            const double wb = a.mode == MODE_STAGE2 ? 1.0 / 4.0 : 2.0 / 3.0;
+            for (int c = 0; c < 4; c++) un[c] = fma(wa, a.W[o + c], wb * fma(dt, r[c], u[c]));
+        }
+        for (int c = 0; c < 4; c++) a.Uout[o + c] = un[c];
+        const double pr = (gd - 1.0) * (un[3] - 0.5 * (un[1] * un[1] + un[2] * un[2]) / un[0]);
+        if (!(un[0] > 0.0) || !(pr > 0.0)) {
+            atomicAdd(&sc->viol, 1ull);
+            atomicMin(&sc->first_bad, ((unsigned long long)a.mode << 40) |
+                                          (unsigned long long)((size_t)(k0 + e) * NQ + i));
+        }
+    }
+}
+
+// CFL time step of the low-order scheme (calculate_lambda_and_low_order_CFL!,
+// low_order_graph_viscosity.jl:222-285): per node CFL*wq*J/2 / (sum_j lambda_ij + sum_f lambda_B).
+template <int N1D, int E, int NT>
+__device__ __forceinline__ void dt_tile(const MixArgs& a, const MixInst& I, const MixTab<N1D>& T, int k0, int nel,
+                                        SmemDt<N1D, E>& sm) {
+    constexpr int NQ = N1D * N1D, NFP = 4 * N1D, LW = N1D - 1;
+    const int tid = threadIdx.x;
+    const double gd = I.gamma;
+    const double* Uin = a.Uin + I.node_off * 4;
+#pragma unroll 1
+    for (int w = tid; w < E * NQ; w += NT) {
+        const int e = w / NQ, i = w - e * NQ;
+        if (e >= nel) continue;
+        double U[4];
+        for (int c = 0; c < 4; c++) U[c] = Uin[((size_t)(k0 + e) * NQ + i) * 4 + c];
+        const NodeP P = node_prims(gd, U);
+        sm.ws[e][i][0] = P.wsx;
+        sm.ws[e][i][1] = P.wsy;
+    }
+    for (int w = tid; w < nel; w += NT) {
+        const Geo G = I.geo[k0 + w];
+        sm.geo[w][0] = G.J;
+        sm.geo[w][1] = G.rxJ;
+        sm.geo[w][2] = G.syJ;
+    }
+    __syncthreads();
+    double dtmin = INFINITY;
+#pragma unroll 1
+    for (int w = tid; w < E * NQ; w += NT) {
+        const int e = w / NQ, i = w - e * NQ;
+        if (e >= nel) continue;
+        const int k = k0 + e;
+        const int ia = i % N1D, ib = i / N1D;
+        const float rx = (float)sm.geo[e][1], sy = (float)sm.geo[e][2];
+        const float wx = sm.ws[e][i][0], wy = sm.ws[e][i][1];
+        float lam = 0.f;
+        if (ia > 0) lam += fabsf(rx * T.cL[ib * LW + ia - 1]) * fmaxf(wx, sm.ws[e][i - 1][0]);
+        if (ia < N1D - 1) lam += fabsf(rx * T.cL[ib * LW + ia]) * fmaxf(wx, sm.ws[e][i + 1][0]);
+        if (ib > 0) lam += fabsf(sy * T.cL[N1D * LW + ia * LW + ib - 1]) * fmaxf(wy, sm.ws[e][i - N1D][1]);
+        if (ib < N1D - 1) lam += fabsf(sy * T.cL[N1D * LW + ia * LW + ib]) * fmaxf(wy, sm.ws[e][i + N1D][1]);
+        const int fx = ia == 0 ? ib : ia == N1D - 1 ? N1D + ib : -1;
+        const int fy = ib == 0 ? 2 * N1D + ia : ib == N1D - 1 ? 3 * N1D + ia : -1;
+        for (int m = 0; m < 2; m++) {
+            const int f = m == 0 ? fx : fy;
+            if (f < 0) continue;
+            const int nb = I.nb_node[(size_t)k * NFP + f];
+            double UP[4];
+            for (int c = 0; c < 4; c++) UP[c] = Uin[(size_t)nb * 4 + c];
+            const NodeP PP = node_prims(gd, UP);
+            const float Bf = (m == 0 ? rx : sy) * T.Bf[f];
+            lam += 0.5f * fabsf(Bf) * fmaxf(m == 0 ? wx : wy, m == 0 ? PP.wsx : PP.wsy);
+        }
+        const double dti = I.CFL * 0.5 * T.wq[i] * sm.geo[e][0] / (double)lam;
+        dtmin = dti < dtmin ? dti : dtmin;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(dtmin, off, 64);
+        dtmin = o < dtmin ? o : dtmin;
+    }
+    if ((tid & 63) == 0) sm.red[tid >> 6] = dtmin;
+    __syncthreads();
+    if (tid == 0) {
+        double m = sm.red[0];
+        for (int x = 1; x < NT / 64; x++) m = sm.red[x] < m ? sm.red[x] : m;
+        if (m < INFINITY) atomicMin(&I.sc->dt_bits, d_to_bits(m));
+    }
+}
+
+template <int A, int B>
+struct cmax {
+    static constexpr int v = A > B ? A : B;
+};
+
+template <int N1D, int E>
+constexpr int stage_bytes() {
+    return E > 0 ? (int)sizeof(SmemM<N1D, (E > 0 ? E : 1)>) : 0;
+}
+template <int N1D, int E>
+constexpr int dt_bytes() {
+    return E > 0 ? (int)sizeof(SmemDt<N1D, (E > 0 ? E : 1)>) : 0;
+}
+template <int N1D, int E>
+constexpr int tab_bytes() {
+    return E > 0 ? (int)sizeof(MixTab<N1D>) : 0;
+}
+
+// Copy the tables of the compiled degrees to LDS (at fixed offsets), then drain the queue.
+template <int E2, int E3, int E4, int E5>
+struct Layout {
+    static constexpr int T2 = 0;
+    static constexpr int T3 = T2 + (tab_bytes<2, E2>() + 15) / 16 * 16;
+    static constexpr int T4 = T3 + (tab_bytes<3, E3>() + 15) / 16 * 16;
+    static constexpr int T5 = T4 + (tab_bytes<4, E4>() + 15) / 16 * 16;
+    static constexpr int TEND = T5 + (tab_bytes<5, E5>() + 15) / 16 * 16;
+    static constexpr int SMAX = cmax<cmax<stage_bytes<2, E2>(), stage_bytes<3, E3>()>::v,
+                                     cmax<stage_bytes<4, E4>(), stage_bytes<5, E5>()>::v>::v;
+    static constexpr int DMAX = cmax<cmax<dt_bytes<2, E2>(), dt_bytes<3, E3>()>::v,
+                                     cmax<dt_bytes<4, E4>(), dt_bytes<5, E5>()>::v>::v;
+};
+
+template <int N1D, int EN, int NT>
+__device__ __forceinline__ void load_tab(const MixArgs& a, unsigned char* dst) {
+    // degrees compiled into the variant but absent from the batch have no table
+    if (EN == 0 || a.tab[N1D] == nullptr) return;
+    const int n = (int)sizeof(MixTab<N1D>) / 4;
+    const unsigned* src = (const unsigned*)a.tab[N1D];
+    unsigned* d = (unsigned*)dst;
+    for (int w = threadIdx.x; w < n; w += NT) d[w] = src[w];
+}
+
+// next tile of this workgroup's queue chunk, then of the other chunks; -1 when all are drained
+__device__ __forceinline__ int next_tile(const MixArgs& a, int& xo, int* s_tile) {
+    if (threadIdx.x == 0) {
+        int t = -1;
+        const int x0 = blockIdx.x & (MIX_CHUNKS - 1);
+        while (xo < MIX_CHUNKS) {
+            const int c = (x0 + xo) & (MIX_CHUNKS - 1);
+            const int lo = (int)((int64_t)a.ntiles * c / MIX_CHUNKS);
+            const int hi = (int)((int64_t)a.ntiles * (c + 1) / MIX_CHUNKS);
+            const int v = atomicAdd(&a.queue[c], 1);
+            if (lo + v < hi) {
+                t = lo + v;
+                break;
+            }
+            xo++;
+        }
+        *s_tile = t;
+    }
+    __syncthreads();
+    const int t = __builtin_amdgcn_readfirstlane(*s_tile);
+    __syncthreads();
+    return t;
+}
+
+template <int NT, int E2, int E3, int E4, int E5>
+__global__ __launch_bounds__(NT) void mix_stage_kernel(MixArgs a) {
+    using LY = Layout<E2, E3, E4, E5>;
+    __shared__ __align__(16) unsigned char tabs[LY::TEND > 0 ? LY::TEND : 16];
+    __shared__ __align__(16) unsigned char raw[LY::SMAX];
+    __shared__ int s_tile;
+    load_tab<2, E2, NT>(a, tabs + LY::T2);
+    load_tab<3, E3, NT>(a, tabs + LY::T3);
+    load_tab<4, E4, NT>(a, tabs + LY::T4);
+    load_tab<5, E5, NT>(a, tabs + LY::T5);
+    __syncthreads();
+    int xo = 0;
+    for (;;) {
+        const int t = next_tile(a, xo, &s_tile);
+        if (t < 0) break;
+        const MixTile tl = a.tiles[t];
+        if (a.only >= 0 && tl.inst != a.only) continue;
+        const MixInst& I = a.inst[tl.inst];
+        if (I.sc->done) continue;
+        switch (tl.n1d) {
+            case 2:
+                if constexpr (E2 > 0)
+                    stage_tile<2, E2, NT>(a, I, *(const MixTab<2>*)(tabs + LY::T2), tl.k0, tl.nel,
+                                          *(SmemM<2, (E2 > 0 ? E2 : 1)>*)raw);
+                break;
+            case 3:
+                if constexpr (E3 > 0)
+                    stage_tile<3, E3, NT>(a, I, *(const MixTab<3>*)(tabs + LY::T3), tl.k0, tl.nel,
+                                          *(SmemM<3, (E3 > 0 ? E3 : 1)>*)raw);
+                break;
+            case 4:
+                if constexpr (E4 > 0)
+                    stage_tile<4, E4, NT>(a, I, *(const MixTab<4>*)(tabs + LY::T4), tl.k0, tl.nel,
+                                          *(SmemM<4, (E4 > 0 ? E4 : 1)>*)raw);
+                break;
+            case 5:
+                if constexpr (E5 > 0)
+                    stage_tile<5, E5, NT>(a, I, *(const MixTab<5>*)(tabs + LY::T5), tl.k0, tl.nel,
+                                          *(SmemM<5, (E5 > 0 ? E5 : 1)>*)raw);
+                break;
+        }
+    }
+}
+
+template <int NT, int E2, int E3, int E4, int E5>
+__global__ __launch_bounds__(NT) void mix_dt_kernel(MixArgs a) {
+    using LY = Layout<E2, E3, E4, E5>;
+    __shared__ __align__(16) unsigned char tabs[LY::TEND > 0 ? LY::TEND : 16];
+    __shared__ __align__(16) unsigned char raw[LY::DMAX];
+    __shared__ int s_tile;
+    load_tab<2, E2, NT>(a, tabs + LY::T2);
+    load_tab<3, E3, NT>(a, tabs + LY::T3);
+    load_tab<4, E4, NT>(a, tabs + LY::T4);
+    load_tab<5, E5, NT>(a, tabs + LY::T5);
+    __syncthreads();
+    int xo = 0;
+    for (;;) {
+        const int t = next_tile(a, xo, &s_tile);
+        if (t < 0) break;
+        const MixTile tl = a.tiles[t];
+        if (a.only >= 0 && tl.inst != a.only) continue;
+        const MixInst& I = a.inst[tl.inst];
+        if (I.sc->done) continue;
+        switch (tl.n1d) {
+            case 2:
+                if constexpr (E2 > 0)
+                    dt_tile<2, E2, NT>(a, I, *(const MixTab<2>*)(tabs + LY::T2), tl.k0, tl.nel,
+                                       *(SmemDt<2, (E2 > 0 ? E2 : 1)>*)raw);
+                break;
+            case 3:
+                if constexpr (E3 > 0)
+                    dt_tile<3, E3, NT>(a, I, *(const MixTab<3>*)(tabs + LY::T3), tl.k0, tl.nel,
+                                       *(SmemDt<3, (E3 > 0 ? E3 : 1)>*)raw);
+                break;
+            case 4:
+                if constexpr (E4 > 0)
+                    dt_tile<4, E4, NT>(a, I, *(const MixTab<4>*)(tabs + LY::T4), tl.k0, tl.nel,
+                                       *(SmemDt<4, (E4 > 0 ? E4 : 1)>*)raw);
+                break;
+            case 5:
+                if constexpr (E5 > 0)
+                    dt_tile<5, E5, NT>(a, I, *(const MixTab<5>*)(tabs + LY::T5), tl.k0, tl.nel,
+                                       *(SmemDt<5, (E5 > 0 ? E5 : 1)>*)raw);
+                break;
+        }
+        __syncthreads();   // the reduction slots are reused by the next tile
+    }
+}
+
+// per instance: step start (SSPRK33.jl:30) or RHS-evaluation set-up; queue counters to zero
+__global__ void mix_init_kernel(const MixInst* inst, int n, int* queue, int nqueue, int mode, int only,
+                                double dt_lim, double dt_cap) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < nqueue) queue[g] = 0;
+    if (g >= n) return;
+    StepScalars* sc = inst[g].sc;
+    if (mode == 0) {
+        const double t = sc->t;
+        if (!(t < inst[g].T)) {
+            sc->done = 1;
+            return;
+        }
+        sc->done = 0;
+        const double dt = jmin(inst[g].CFL * inst[g].dt0, inst[g].T - t);
+        sc->dt_lim = dt;
+        sc->dt_bits = d_to_bits(dt);
+    } else if (g == only) {
+        sc->done = 0;
+        sc->dt_lim = dt_lim;
+        sc->dt_bits = d_to_bits(dt_cap);
+    }
+}
+
+__global__ void mix_post_kernel(const MixInst* inst, int n) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    StepScalars* sc = inst[g].sc;
+    if (sc->done) return;
+    sc->t = sc->t + bits_to_d(sc->dt_bits);
+    sc->steps = sc->steps + 1;
+}
+
+__device__ __forceinline__ unsigned long long order_key_m(double v) {
+    unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__global__ void mix_positivity_kernel(const double* U, int64_t nodes, double gamma, unsigned long long* keys,
+                                      unsigned long long* cnt) {
+    const size_t gidx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double mr = INFINITY, mp = INFINITY;
+    unsigned long long bad = 0;
+    if (gidx < (size_t)nodes) {
+        const double* u = U + gidx * 4;
+        const double pr = (gamma - 1.0) * (u[3] - 0.5 * (u[1] * u[1] + u[2] * u[2]) / u[0]);
+        mr = u[0];
+        mp = pr;
+        bad = (!(u[0] > 0.0) || !(pr > 0.0)) ? 1ull : 0ull;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o1 = __shfl_xor(mr, off, 64), o2 = __shfl_xor(mp, off, 64);
+        mr = o1 < mr ? o1 : mr;
+        mp = o2 < mp ? o2 : mp;
+        bad += __shfl_xor(bad, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&keys[0], order_key_m(mr));
+        atomicMin(&keys[1], order_key_m(mp));
+        if (bad) atomicAdd(cnt, bad);
+    }
+}
+
+// ------------------------------------------------------------------------------ variants
+struct MixVariant {
+    int NT, E[MIX_MAX_N1D + 1];
+    const void* stage;
+    const void* dt;
+};
+
+#define MV(NT, E2, E3, E4, E5)                                                            \
+    {NT, {0, 0, E2, E3, E4, E5}, (const void*)&mix_stage_kernel<NT, E2, E3, E4, E5>,     \
+     (const void*)&mix_dt_kernel<NT, E2, E3, E4, E5>}
+
+// single-degree shapes first (one degree per problem), then all-degree shapes for batches
+const MixVariant kMix[] = {
+    MV(128, 0, 0, 4, 0), MV(64, 0, 0, 4, 0), MV(128, 0, 0, 8, 0), MV(256, 0, 0, 8, 0), MV(64, 0, 0, 2, 0),
+    MV(128, 0, 0, 0, 2), MV(64, 0, 0, 0, 2), MV(128, 0, 0, 0, 4), MV(256, 0, 0, 0, 4),
+    MV(128, 0, 8, 0, 0), MV(64, 0, 4, 0, 0), MV(256, 0, 16, 0, 0),
+    MV(128, 16, 0, 0, 0), MV(64, 8, 0, 0, 0),
+    MV(128, 0, 8, 4, 2), MV(64, 0, 4, 2, 1), MV(256, 0, 16, 8, 4), MV(128, 16, 8, 4, 2),
+};
+#undef MV
+constexpr int kNumMix = (int)(sizeof(kMix) / sizeof(kMix[0]));
+
+}  // namespace
+
+int mix_num_variants() { return kNumMix; }
+
+int mix_variant_shape(int v, int* NT, int E[MIX_MAX_N1D + 1]) {
+    if (v < 0 || v >= kNumMix) return -1;
+    *NT = kMix[v].NT;
+    for (int n = 0; n <= MIX_MAX_N1D; n++) E[n] = kMix[v].E[n];
+    return 0;
+}
+
+int mix_blocks_per_cu(int v, int* stage_blocks, int* dt_blocks) {
+    if (v < 0 || v >= kNumMix) return (int)hipErrorInvalidValue;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(stage_blocks, kMix[v].stage, kMix[v].NT, 0);
+    if (e != hipSuccess) return (int)e;
+    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(dt_blocks, kMix[v].dt, kMix[v].NT, 0);
+}
+
+int mix_launch_stage(int v, const MixArgs& a, int grid, void* stream) {
+    if (v < 0 || v >= kNumMix || grid < 1) return (int)hipErrorInvalidValue;
+    MixArgs arg = a;
+    void* params[] = {&arg};
+    return (int)hipLaunchKernel(kMix[v].stage, dim3(grid), dim3(kMix[v].NT), params, 0, (hipStream_t)stream);
+}
+
+int mix_launch_dt(int v, const MixArgs& a, int grid, void* stream) {
+    if (v < 0 || v >= kNumMix || grid < 1) return (int)hipErrorInvalidValue;
+    MixArgs arg = a;
+    void* params[] = {&arg};
+    return (int)hipLaunchKernel(kMix[v].dt, dim3(grid), dim3(kMix[v].NT), params, 0, (hipStream_t)stream);
+}
+
+int mix_launch_init(const MixInst* inst, int n, int* queue, int nqueue, int mode, int only, double dt_lim,
+                    double dt_cap, void* stream) {
+    const int m = n > nqueue ? n : nqueue;
+    hipLaunchKernelGGL(mix_init_kernel, dim3((m + 255) / 256), dim3(256), 0, (hipStream_t)stream, inst, n, queue,
+                       nqueue, mode, only, dt_lim, dt_cap);
+    return (int)hipGetLastError();
+}
+
+int mix_launch_post(const MixInst* inst, int n, void* stream) {
+    hipLaunchKernelGGL(mix_post_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, inst, n);
+    return (int)hipGetLastError();
+}
+
+int mix_launch_positivity(const double* U, int64_t nodes, double gamma, unsigned long long* keys,
+                          unsigned long long* cnt, void* stream) {
+    const int blocks = (int)((nodes + 255) / 256);
+    hipLaunchKernelGGL(mix_positivity_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, U, nodes, gamma,
+                       keys, cnt);
+    return (int)hipGetLastError();
+}
+
+}  // namespace p2de

@@ -16,6 +16,7 @@
 
 #include "p2de_hip.h"
 #include "p2de_internal.h"
+#include "p2de_setup.h"
 
 using namespace p2de;
 
@@ -109,14 +110,6 @@ void free_ctx(p2de_ctx* c) {
     delete c;
 }
 
-int64_t ipow(int a, int b) {
-    int64_t r = 1;
-    for (int i = 0; i < b; i++) r *= a;
-    return r;
-}
-
-// Column-major access helpers.
-inline double cm(const double* A, int64_t ld, int64_t i, int64_t j) { return A[i + ld * j]; }
 
 hipEvent_t take_event(p2de_ctx* c) {
     if (!c->event_pool.empty()) {
@@ -152,164 +145,6 @@ void harvest_timing(p2de_ctx* c) {
         c->event_pool.push_back(l.b);
     }
     c->launches.clear();
-}
-
-int build_tables(p2de_ctx* c, const p2de_operators* ops) {
-    ElemTables& T = c->tab;
-    std::memset(&T, 0, sizeof(T));
-    const int Nd = c->Nd, N1D = c->N1D, Nq = c->Nq, Nfp = c->Nfp, Nh = c->Nh;
-    T.Nd = Nd;
-    T.Nc = c->Nc;
-    T.N1D = N1D;
-    T.Nq = Nq;
-    T.Nfp = Nfp;
-    T.Nh = Nh;
-    const int64_t nph_expect = Nd == 1 ? N1D * (N1D - 1) / 2 + 2 : (int64_t)N1D * N1D * (N1D - 1) + Nfp;
-    const int64_t npl_expect = Nd == 1 ? N1D - 1 : 2 * N1D * (N1D - 1);
-    if (ops->nnz_h != nph_expect)
-        return fail(c, P2DE_E_UNSUPPORTED, "Srsh_nnz has " + std::to_string(ops->nnz_h) + " pairs, the Lobatto operator has " +
-                                               std::to_string(nph_expect));
-    if (ops->nnz_0 != npl_expect)
-        return fail(c, P2DE_E_UNSUPPORTED, "Srs0_nnz has " + std::to_string(ops->nnz_0) + " pairs, expected " +
-                                               std::to_string(npl_expect));
-    T.nph = (int)nph_expect;
-    T.npl = (int)npl_expect;
-
-    // fq2q and the tensor-product face layout the subcell limiter assumes (subcell.jl:173-206)
-    for (int f = 0; f < Nfp; f++) {
-        const int64_t q1 = ops->fq2q[f];
-        if (q1 < 1 || q1 > Nq) return fail(c, P2DE_E_INVALID, "fq2q out of range");
-        const int q = (int)q1 - 1;
-        int expect;
-        if (Nd == 1) {
-            expect = f == 0 ? 0 : N1D - 1;
-        } else {
-            const int face = f / N1D, m = f % N1D;
-            expect = face == 0 ? m * N1D : face == 1 ? m * N1D + N1D - 1 : face == 2 ? m : (N1D - 1) * N1D + m;
-        }
-        if (q != expect) return fail(c, P2DE_E_UNSUPPORTED, "fq2q does not follow the tensor-product face layout");
-        T.fq2q[f] = (uint8_t)q;
-        T.ld[f] = (int8_t)((Nd == 1 || f < 2 * N1D) ? 0 : 1);   // rhs_utils.jl:84-88
-        if (Nd == 1) T.bsign[f] = f == 0 ? -1 : 1;
-        else T.bsign[f] = (f < N1D || (f >= 2 * N1D && f < 3 * N1D)) ? -1 : 1;
-        T.Br[f] = cm(ops->Brs, Nfp, f, f);
-        T.Bs[f] = Nd == 2 ? cm(ops->Brs + (int64_t)Nfp * Nfp, Nfp, f, f) : 0.0;
-        if (Nd == 2) {
-            const bool xface = f < 2 * N1D;
-            if (xface ? (T.Br[f] == 0.0 || T.Bs[f] != 0.0) : (T.Bs[f] == 0.0 || T.Br[f] != 0.0))
-                return fail(c, P2DE_E_UNSUPPORTED, "Brs does not have one nonzero direction per face node");
-        } else if (T.Br[f] == 0.0) {
-            return fail(c, P2DE_E_UNSUPPORTED, "Br has a zero face entry");
-        }
-        // entropy projection row (rhs.jl:90-92 with l = 1): a single nonzero at fq2q[f]
-        for (int j = 0; j < Nq; j++) {
-            const double vf = cm(ops->Vf, Nfp, f, j);
-            const double coef = 1.0 * vf + (1.0 - 1.0) * cm(ops->Vf_low, Nfp, f, j);
-            if (j == q) {
-                if (coef == 0.0) return fail(c, P2DE_E_UNSUPPORTED, "Vf is zero at fq2q");
-                T.vf_coef[f] = coef;
-            } else if (vf != 0.0) {
-                return fail(c, P2DE_E_UNSUPPORTED, "Vf is not a face-node selection (Gauss basis is not implemented)");
-            }
-        }
-    }
-    // node data and the assembly structure of MinvVhT / MinvVfT (flux_differencing.jl:331-361)
-    for (int i = 0; i < Nq; i++) {
-        T.wq[i] = ops->wq[i];
-        if (!(T.wq[i] > 0.0)) return fail(c, P2DE_E_INVALID, "wq must be positive");
-        for (int j = 0; j < Nq; j++)
-            if (j != i && cm(ops->MinvVhT, Nq, i, j) != 0.0)
-                return fail(c, P2DE_E_UNSUPPORTED, "MinvVhT volume block is not diagonal");
-        T.a_v[i] = cm(ops->MinvVhT, Nq, i, i);
-        int n = 0;
-        for (int f = 0; f < Nfp; f++) {
-            const double ah = cm(ops->MinvVhT, Nq, i, Nq + f), af = cm(ops->MinvVfT, Nq, i, f);
-            if (T.fq2q[f] == i) {
-                if (n >= 2) return fail(c, P2DE_E_UNSUPPORTED, "more than two faces at a node");
-                T.nface[i][n] = (int8_t)f;
-                T.a_h[i][n] = ah;
-                T.a_f[i][n] = af;
-                n++;
-            } else if (ah != 0.0 || af != 0.0) {
-                return fail(c, P2DE_E_UNSUPPORTED, "MinvVhT/MinvVfT couple a node to a face not at that node");
-            }
-        }
-        T.nf[i] = (int8_t)n;
-        int m = 0;
-        for (int s = 0; s < 2; s++) {
-            const int64_t f1 = ops->q2fq[i + (int64_t)Nq * s];
-            if (f1 == 0) continue;
-            if (f1 < 1 || f1 > Nfp) return fail(c, P2DE_E_INVALID, "q2fq out of range");
-            T.q2fq[i][m++] = (int8_t)(f1 - 1);
-        }
-        T.nq2fq[i] = (int8_t)m;
-    }
-    // high-order pairs
-    std::vector<std::vector<std::pair<int, int>>> hl(Nh);
-    for (int p = 0; p < T.nph; p++) {
-        const int64_t i1 = ops->Srsh_nnz[2 * p], j1 = ops->Srsh_nnz[2 * p + 1];
-        if (i1 < 1 || i1 > Nh || j1 < 1 || j1 > Nh || i1 <= j1) return fail(c, P2DE_E_INVALID, "Srsh_nnz entry out of range");
-        const int i = (int)i1 - 1, j = (int)j1 - 1;
-        const double cr = cm(ops->Srsh_db, Nh, i, j);
-        const double cs = Nd == 2 ? cm(ops->Srsh_db + (int64_t)Nh * Nh, Nh, i, j) : 0.0;
-        if ((cr != 0.0) == (cs != 0.0) && Nd == 2)
-            return fail(c, P2DE_E_UNSUPPORTED, "a Srsh pair acts in both or no directions");
-        if (Nd == 1 && cr == 0.0) return fail(c, P2DE_E_UNSUPPORTED, "zero Srh pair");
-        T.hp_i[p] = (uint8_t)i;
-        T.hp_j[p] = (uint8_t)j;
-        T.hp_dir[p] = (uint8_t)(cr != 0.0 ? 0 : 1);
-        T.hp_cr[p] = cr;
-        T.hp_cs[p] = cs;
-        hl[i].push_back({p, +1});
-        hl[j].push_back({p, -1});
-    }
-    int pos = 0;
-    for (int h = 0; h < Nh; h++) {
-        T.hl_start[h] = (int16_t)pos;
-        for (auto& e : hl[h]) {
-            T.hl_pair[pos] = (int16_t)e.first;
-            T.hl_sign[pos] = (int8_t)e.second;
-            pos++;
-        }
-    }
-    T.hl_start[Nh] = (int16_t)pos;
-    // low-order pairs
-    std::vector<std::vector<std::pair<int, int>>> ll(Nq), cl(Nq);
-    for (int p = 0; p < T.npl; p++) {
-        const int64_t i1 = ops->Srs0_nnz[2 * p], j1 = ops->Srs0_nnz[2 * p + 1];
-        if (i1 < 1 || i1 > Nq || j1 < 1 || j1 > Nq || i1 <= j1) return fail(c, P2DE_E_INVALID, "Srs0_nnz entry out of range");
-        const int i = (int)i1 - 1, j = (int)j1 - 1;
-        const double cr = cm(ops->Srs0, Nq, i, j);
-        const double cs = Nd == 2 ? cm(ops->Srs0 + (int64_t)Nq * Nq, Nq, i, j) : 0.0;
-        if ((cr != 0.0) == (cs != 0.0) && Nd == 2)
-            return fail(c, P2DE_E_UNSUPPORTED, "a Srs0 pair acts in both or no directions");
-        if (Nd == 1 && cr == 0.0) return fail(c, P2DE_E_UNSUPPORTED, "zero Sr0 pair");
-        T.lp_i[p] = (uint8_t)i;
-        T.lp_j[p] = (uint8_t)j;
-        T.lp_dir[p] = (uint8_t)(cr != 0.0 ? 0 : 1);
-        T.lp_cr[p] = cr;
-        T.lp_cs[p] = cs;
-        ll[i].push_back({p, +1});
-        ll[j].push_back({p, -1});
-        cl[i].push_back({j, p});
-        cl[j].push_back({i, p});
-    }
-    pos = 0;
-    int cpos = 0;
-    for (int i = 0; i < Nq; i++) {
-        T.ll_start[i] = (int16_t)pos;
-        for (auto& e : ll[i]) {
-            T.ll_pair[pos] = (int16_t)e.first;
-            T.ll_sign[pos] = (int8_t)e.second;
-            pos++;
-        }
-        std::sort(cl[i].begin(), cl[i].end());
-        T.cl_start[i] = (int16_t)cpos;
-        for (auto& e : cl[i]) T.cl_pair[cpos++] = (int16_t)e.second;
-    }
-    T.ll_start[Nq] = (int16_t)pos;
-    T.cl_start[Nq] = (int16_t)cpos;
-    return P2DE_OK;
 }
 
 }  // namespace
@@ -349,31 +184,14 @@ int p2de_create(const p2de_config* cfg, const p2de_operators* ops, const p2de_ge
     };
     try {
         c->cfg = *cfg;
-        // option set (Solver.jl:1-121); anything else is not implemented here
-        if (cfg->rhs != P2DE_RHS_LIMITED_DG || cfg->low_order_surface_flux != P2DE_LF_ON_NODAL_VAL ||
-            cfg->high_order_surface_flux != P2DE_LF_ON_PROJECTED_VAL ||
-            cfg->high_order_volume_flux != P2DE_CHANDRASHEKAR_FLUX || cfg->basis != P2DE_BASIS_LOBATTO ||
-            cfg->entropyproj_limiter != P2DE_NO_ENTROPY_PROJ_LIMITER || cfg->rhs_limiter != P2DE_SUBCELL_LIMITER ||
-            cfg->bound != P2DE_POSITIVITY_BOUND || cfg->shockcapture != P2DE_NO_SHOCK_CAPTURE ||
-            cfg->precision != P2DE_PRECISION_FP64)
-            return fail(c, P2DE_E_UNSUPPORTED, "option set not implemented"), bail(P2DE_E_UNSUPPORTED);
-        if (cfg->upper_bound_mode != P2DE_UPPER_BOUND_REFERENCE && cfg->upper_bound_mode != P2DE_UPPER_BOUND_INTENDED)
-            return fail(c, P2DE_E_INVALID, "upper_bound_mode"), bail(P2DE_E_INVALID);
-        if (cfg->Nd != 1 && cfg->Nd != 2) return fail(c, P2DE_E_INVALID, "Nd must be 1 or 2"), bail(P2DE_E_INVALID);
-        if (cfg->N < 1 || cfg->N > 4) return fail(c, P2DE_E_UNSUPPORTED, "N must be 1..4"), bail(P2DE_E_UNSUPPORTED);
-        if (cfg->Kx < 1 || cfg->Ky < 1 || (cfg->Nd == 1 && cfg->Ky != 1))
-            return fail(c, P2DE_E_INVALID, "Kx, Ky"), bail(P2DE_E_INVALID);
-        if (!(cfg->gamma > 1.0)) return fail(c, P2DE_E_INVALID, "gamma must exceed 1"), bail(P2DE_E_INVALID);
+        int st = setup_config(cfg, &c->Nq, &c->Nfp, &c->Nh, &c->K, c->err);
+        if (st) return bail(st);
+        if (cfg->precision != P2DE_PRECISION_FP64)
+            return fail(c, P2DE_E_UNSUPPORTED, "p2de_create runs FP64; mixed precision runs through p2de_batch_create"),
+                   bail(P2DE_E_UNSUPPORTED);
         c->Nd = cfg->Nd;
         c->Nc = cfg->Nd + 2;
         c->N1D = cfg->N + 1;
-        c->Nq = (int)ipow(c->N1D, c->Nd);
-        c->Nfp = c->Nd == 1 ? 2 : 4 * c->N1D;
-        c->Nh = c->Nq + c->Nfp;
-        const int64_t K64 = (int64_t)cfg->Kx * cfg->Ky;
-        if (K64 * c->Nq * c->Nc >= (int64_t)1 << 31 || K64 * c->Nfp >= (int64_t)1 << 31)
-            return fail(c, P2DE_E_UNSUPPORTED, "problem too large for 32-bit node indices"), bail(P2DE_E_UNSUPPORTED);
-        c->K = (int)K64;
         if (ops->Nq != c->Nq || ops->Nfp != c->Nfp || ops->Nh != c->Nh)
             return fail(c, P2DE_E_INVALID, "operator sizes do not match Nd and N"), bail(P2DE_E_INVALID);
         const void* req[] = {ops->Srsh_db, ops->Srs0, ops->Srsh_nnz, ops->Srs0_nnz, ops->Brs, ops->MinvVhT,
@@ -381,70 +199,14 @@ int p2de_create(const p2de_config* cfg, const p2de_operators* ops, const p2de_ge
                              bc->mapP};
         for (const void* p : req)
             if (!p) return fail(c, P2DE_E_INVALID, "NULL operator, geometry or mapP array"), bail(P2DE_E_INVALID);
-        int st = build_tables(c, ops);
-        if (st) return bail(st);
-
-        const int K = c->K, Nq = c->Nq, Nfp = c->Nfp, Nh = c->Nh, Nc = c->Nc, N1D = c->N1D;
-        // per-element metric, constant inside the element
-        std::vector<Geo> geo(K);
-        const int NGEO = c->Nd == 1 ? 1 : 4;
-        for (int k = 0; k < K; k++) {
-            const double J = geom->Jq[(int64_t)Nq * k];
-            for (int i = 1; i < Nq; i++)
-                if (geom->Jq[i + (int64_t)Nq * k] != J)
-                    return fail(c, P2DE_E_UNSUPPORTED, "Jq varies inside an element"), bail(P2DE_E_UNSUPPORTED);
-            if (!(J > 0.0)) return fail(c, P2DE_E_INVALID, "Jq must be positive"), bail(P2DE_E_INVALID);
-            double gv[4] = {0, 0, 0, 0};
-            for (int gi = 0; gi < NGEO; gi++) {
-                const double* G = geom->GJh + (int64_t)gi * Nh * K;
-                gv[gi] = G[(int64_t)Nh * k];
-                for (int h = 1; h < Nh; h++)
-                    if (G[h + (int64_t)Nh * k] != gv[gi])
-                        return fail(c, P2DE_E_UNSUPPORTED, "metric terms vary inside an element"), bail(P2DE_E_UNSUPPORTED);
-            }
-            Geo g{J, gv[0], 0.0, 0.0, 0.0};
-            if (c->Nd == 2) {
-                g.sxJ = gv[1];
-                g.ryJ = gv[2];
-                g.syJ = gv[3];
-                if (g.sxJ != 0.0 || g.ryJ != 0.0)
-                    return fail(c, P2DE_E_UNSUPPORTED, "elements must be axis aligned (sxJ = ryJ = 0)"), bail(P2DE_E_UNSUPPORTED);
-            }
-            // the graph-viscosity direction test |Sx0| > 1e-10 (rhs_utils.jl:256) must pick the
-            // direction the pair acts in
-            for (int p = 0; p < c->tab.npl && c->Nd == 2; p++) {
-                const double s0x = g.rxJ * c->tab.lp_cr[p] + g.sxJ * c->tab.lp_cs[p];
-                const int vdir = std::fabs(s0x) > 1e-10 ? 0 : 1;
-                if (vdir != c->tab.lp_dir[p])
-                    return fail(c, P2DE_E_UNSUPPORTED, "graph-viscosity direction test disagrees with the pair direction"),
-                           bail(P2DE_E_UNSUPPORTED);
-            }
-            geo[k] = g;
-        }
-        // face connectivity and boundary codes (StateParam.jl:1-7)
-        std::vector<int> nb_node((size_t)K * Nfp), nb_face((size_t)K * Nfp), bcode((size_t)K * Nfp, 0);
-        for (int k = 0; k < K; k++)
-            for (int f = 0; f < Nfp; f++) {
-                const int64_t m = bc->mapP[f + (int64_t)Nfp * k] - 1;
-                if (m < 0 || m >= (int64_t)Nfp * K) return fail(c, P2DE_E_INVALID, "mapP out of range"), bail(P2DE_E_INVALID);
-                const int kP = (int)(m / Nfp), iP = (int)(m % Nfp);
-                if (c->Nd == 2 && ((f < 2 * N1D) != (iP < 2 * N1D)))
-                    return fail(c, P2DE_E_UNSUPPORTED, "mapP joins an x face to a y face"), bail(P2DE_E_UNSUPPORTED);
-                nb_face[(size_t)k * Nfp + f] = (int)m;
-                nb_node[(size_t)k * Nfp + f] = kP * Nq + c->tab.fq2q[iP];
-            }
-        if (bc->nI > 0 && (!bc->mapI || !bc->Ival)) return fail(c, P2DE_E_INVALID, "mapI/Ival"), bail(P2DE_E_INVALID);
-        if (bc->nO > 0 && !bc->mapO) return fail(c, P2DE_E_INVALID, "mapO"), bail(P2DE_E_INVALID);
-        for (int i = 0; i < bc->nI; i++) {
-            const int64_t m = bc->mapI[i] - 1;
-            if (m < 0 || m >= (int64_t)Nfp * K) return fail(c, P2DE_E_INVALID, "mapI out of range"), bail(P2DE_E_INVALID);
-            bcode[m] = i + 1;
-        }
-        for (int i = 0; i < bc->nO; i++) {   // enforce_BC! applies outflow after inflow
-            const int64_t m = bc->mapO[i] - 1;
-            if (m < 0 || m >= (int64_t)Nfp * K) return fail(c, P2DE_E_INVALID, "mapO out of range"), bail(P2DE_E_INVALID);
-            bcode[m] = -1;
-        }
+        if ((st = setup_tables(c->tab, c->Nd, c->N1D, ops, c->err))) return bail(st);
+        MeshArrays mesh;
+        if ((st = setup_mesh(c->Nd, c->N1D, c->K, c->tab, geom, bc, mesh, c->err))) return bail(st);
+        const std::vector<Geo>& geo = mesh.geo;
+        const std::vector<int>& nb_node = mesh.nb_node;
+        const std::vector<int>& nb_face = mesh.nb_face;
+        const std::vector<int>& bcode = mesh.bcode;
+        const int K = c->K, Nq = c->Nq, Nfp = c->Nfp, Nc = c->Nc;
 
         // device resources (default-priority stream)
         if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)

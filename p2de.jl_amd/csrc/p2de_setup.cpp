@@ -1,0 +1,288 @@
+// p2de_setup.cpp -- validation of the caller's operators and mesh and the tables built from
+// them (see p2de_setup.h).
+#include "p2de_setup.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace p2de {
+namespace {
+
+int fail_s(std::string& err, int code, const std::string& msg) {
+    err = msg;
+    return code;
+}
+
+inline double cm(const double* A, int64_t ld, int64_t i, int64_t j) { return A[i + ld * j]; }
+
+int64_t ipow(int a, int b) {
+    int64_t r = 1;
+    for (int i = 0; i < b; i++) r *= a;
+    return r;
+}
+
+}  // namespace
+
+int setup_config(const p2de_config* cfg, int* Nq, int* Nfp, int* Nh, int* K, std::string& err) {
+    // option set (Solver.jl:1-121); anything else is not implemented here
+    if (cfg->rhs != P2DE_RHS_LIMITED_DG || cfg->low_order_surface_flux != P2DE_LF_ON_NODAL_VAL ||
+        cfg->high_order_surface_flux != P2DE_LF_ON_PROJECTED_VAL ||
+        cfg->high_order_volume_flux != P2DE_CHANDRASHEKAR_FLUX || cfg->basis != P2DE_BASIS_LOBATTO ||
+        cfg->entropyproj_limiter != P2DE_NO_ENTROPY_PROJ_LIMITER || cfg->rhs_limiter != P2DE_SUBCELL_LIMITER ||
+        cfg->bound != P2DE_POSITIVITY_BOUND || cfg->shockcapture != P2DE_NO_SHOCK_CAPTURE)
+        return fail_s(err, P2DE_E_UNSUPPORTED, "option set not implemented");
+    if (cfg->precision != P2DE_PRECISION_FP64 && cfg->precision != P2DE_PRECISION_MIXED)
+        return fail_s(err, P2DE_E_INVALID, "precision");
+    if (cfg->upper_bound_mode != P2DE_UPPER_BOUND_REFERENCE && cfg->upper_bound_mode != P2DE_UPPER_BOUND_INTENDED)
+        return fail_s(err, P2DE_E_INVALID, "upper_bound_mode");
+    if (cfg->Nd != 1 && cfg->Nd != 2) return fail_s(err, P2DE_E_INVALID, "Nd must be 1 or 2");
+    if (cfg->N < 1 || cfg->N > 4) return fail_s(err, P2DE_E_UNSUPPORTED, "N must be 1..4");
+    if (cfg->Kx < 1 || cfg->Ky < 1 || (cfg->Nd == 1 && cfg->Ky != 1)) return fail_s(err, P2DE_E_INVALID, "Kx, Ky");
+    if (!(cfg->gamma > 1.0)) return fail_s(err, P2DE_E_INVALID, "gamma must exceed 1");
+    if (cfg->precision == P2DE_PRECISION_MIXED && cfg->Nd != 2)
+        return fail_s(err, P2DE_E_UNSUPPORTED, "the mixed-precision path is 2D only");
+    const int N1D = cfg->N + 1;
+    *Nq = (int)ipow(N1D, cfg->Nd);
+    *Nfp = cfg->Nd == 1 ? 2 : 4 * N1D;
+    *Nh = *Nq + *Nfp;
+    const int64_t K64 = (int64_t)cfg->Kx * cfg->Ky;
+    if (K64 * *Nq * (cfg->Nd + 2) >= (int64_t)1 << 31 || K64 * *Nfp >= (int64_t)1 << 31)
+        return fail_s(err, P2DE_E_UNSUPPORTED, "problem too large for 32-bit node indices");
+    *K = (int)K64;
+    return P2DE_OK;
+}
+
+int setup_tables(ElemTables& T, int Nd, int N1D, const p2de_operators* ops, std::string& err) {
+    std::memset(&T, 0, sizeof(T));
+    const int Nq = Nd == 1 ? N1D : N1D * N1D, Nfp = Nd == 1 ? 2 : 4 * N1D, Nh = Nq + Nfp;
+    T.Nd = Nd;
+    T.Nc = Nd + 2;
+    T.N1D = N1D;
+    T.Nq = Nq;
+    T.Nfp = Nfp;
+    T.Nh = Nh;
+    const int64_t nph_expect = Nd == 1 ? N1D * (N1D - 1) / 2 + 2 : (int64_t)N1D * N1D * (N1D - 1) + Nfp;
+    const int64_t npl_expect = Nd == 1 ? N1D - 1 : 2 * N1D * (N1D - 1);
+    if (ops->nnz_h != nph_expect)
+        return fail_s(err, P2DE_E_UNSUPPORTED, "Srsh_nnz has " + std::to_string(ops->nnz_h) + " pairs, the Lobatto operator has " +
+                                               std::to_string(nph_expect));
+    if (ops->nnz_0 != npl_expect)
+        return fail_s(err, P2DE_E_UNSUPPORTED, "Srs0_nnz has " + std::to_string(ops->nnz_0) + " pairs, expected " +
+                                               std::to_string(npl_expect));
+    T.nph = (int)nph_expect;
+    T.npl = (int)npl_expect;
+
+    // fq2q and the tensor-product face layout the subcell limiter assumes (subcell.jl:173-206)
+    for (int f = 0; f < Nfp; f++) {
+        const int64_t q1 = ops->fq2q[f];
+        if (q1 < 1 || q1 > Nq) return fail_s(err, P2DE_E_INVALID, "fq2q out of range");
+        const int q = (int)q1 - 1;
+        int expect;
+        if (Nd == 1) {
+            expect = f == 0 ? 0 : N1D - 1;
+        } else {
+            const int face = f / N1D, m = f % N1D;
+            expect = face == 0 ? m * N1D : face == 1 ? m * N1D + N1D - 1 : face == 2 ? m : (N1D - 1) * N1D + m;
+        }
+        if (q != expect) return fail_s(err, P2DE_E_UNSUPPORTED, "fq2q does not follow the tensor-product face layout");
+        T.fq2q[f] = (uint8_t)q;
+        T.ld[f] = (int8_t)((Nd == 1 || f < 2 * N1D) ? 0 : 1);   // rhs_utils.jl:84-88
+        if (Nd == 1) T.bsign[f] = f == 0 ? -1 : 1;
+        else T.bsign[f] = (f < N1D || (f >= 2 * N1D && f < 3 * N1D)) ? -1 : 1;
+        T.Br[f] = cm(ops->Brs, Nfp, f, f);
+        T.Bs[f] = Nd == 2 ? cm(ops->Brs + (int64_t)Nfp * Nfp, Nfp, f, f) : 0.0;
+        if (Nd == 2) {
+            const bool xface = f < 2 * N1D;
+            if (xface ? (T.Br[f] == 0.0 || T.Bs[f] != 0.0) : (T.Bs[f] == 0.0 || T.Br[f] != 0.0))
+                return fail_s(err, P2DE_E_UNSUPPORTED, "Brs does not have one nonzero direction per face node");
+        } else if (T.Br[f] == 0.0) {
+            return fail_s(err, P2DE_E_UNSUPPORTED, "Br has a zero face entry");
+        }
+        // entropy projection row (rhs.jl:90-92 with l = 1): a single nonzero at fq2q[f]
+        for (int j = 0; j < Nq; j++) {
+            const double vf = cm(ops->Vf, Nfp, f, j);
+            const double coef = 1.0 * vf + (1.0 - 1.0) * cm(ops->Vf_low, Nfp, f, j);
+            if (j == q) {
+                if (coef == 0.0) return fail_s(err, P2DE_E_UNSUPPORTED, "Vf is zero at fq2q");
+                T.vf_coef[f] = coef;
+            } else if (vf != 0.0) {
+                return fail_s(err, P2DE_E_UNSUPPORTED, "Vf is not a face-node selection (Gauss basis is not implemented)");
+            }
+        }
+    }
+    // node data and the assembly structure of MinvVhT / MinvVfT (flux_differencing.jl:331-361)
+    for (int i = 0; i < Nq; i++) {
+        T.wq[i] = ops->wq[i];
+        if (!(T.wq[i] > 0.0)) return fail_s(err, P2DE_E_INVALID, "wq must be positive");
+        for (int j = 0; j < Nq; j++)
+            if (j != i && cm(ops->MinvVhT, Nq, i, j) != 0.0)
+                return fail_s(err, P2DE_E_UNSUPPORTED, "MinvVhT volume block is not diagonal");
+        T.a_v[i] = cm(ops->MinvVhT, Nq, i, i);
+        int n = 0;
+        for (int f = 0; f < Nfp; f++) {
+            const double ah = cm(ops->MinvVhT, Nq, i, Nq + f), af = cm(ops->MinvVfT, Nq, i, f);
+            if (T.fq2q[f] == i) {
+                if (n >= 2) return fail_s(err, P2DE_E_UNSUPPORTED, "more than two faces at a node");
+                T.nface[i][n] = (int8_t)f;
+                T.a_h[i][n] = ah;
+                T.a_f[i][n] = af;
+                n++;
+            } else if (ah != 0.0 || af != 0.0) {
+                return fail_s(err, P2DE_E_UNSUPPORTED, "MinvVhT/MinvVfT couple a node to a face not at that node");
+            }
+        }
+        T.nf[i] = (int8_t)n;
+        int m = 0;
+        for (int s = 0; s < 2; s++) {
+            const int64_t f1 = ops->q2fq[i + (int64_t)Nq * s];
+            if (f1 == 0) continue;
+            if (f1 < 1 || f1 > Nfp) return fail_s(err, P2DE_E_INVALID, "q2fq out of range");
+            T.q2fq[i][m++] = (int8_t)(f1 - 1);
+        }
+        T.nq2fq[i] = (int8_t)m;
+    }
+    // high-order pairs
+    std::vector<std::vector<std::pair<int, int>>> hl(Nh);
+    for (int p = 0; p < T.nph; p++) {
+        const int64_t i1 = ops->Srsh_nnz[2 * p], j1 = ops->Srsh_nnz[2 * p + 1];
+        if (i1 < 1 || i1 > Nh || j1 < 1 || j1 > Nh || i1 <= j1) return fail_s(err, P2DE_E_INVALID, "Srsh_nnz entry out of range");
+        const int i = (int)i1 - 1, j = (int)j1 - 1;
+        const double cr = cm(ops->Srsh_db, Nh, i, j);
+        const double cs = Nd == 2 ? cm(ops->Srsh_db + (int64_t)Nh * Nh, Nh, i, j) : 0.0;
+        if ((cr != 0.0) == (cs != 0.0) && Nd == 2)
+            return fail_s(err, P2DE_E_UNSUPPORTED, "a Srsh pair acts in both or no directions");
+        if (Nd == 1 && cr == 0.0) return fail_s(err, P2DE_E_UNSUPPORTED, "zero Srh pair");
+        T.hp_i[p] = (uint8_t)i;
+        T.hp_j[p] = (uint8_t)j;
+        T.hp_dir[p] = (uint8_t)(cr != 0.0 ? 0 : 1);
+        T.hp_cr[p] = cr;
+        T.hp_cs[p] = cs;
+        hl[i].push_back({p, +1});
+        hl[j].push_back({p, -1});
+    }
+    int pos = 0;
+    for (int h = 0; h < Nh; h++) {
+        T.hl_start[h] = (int16_t)pos;
+        for (auto& e : hl[h]) {
+            T.hl_pair[pos] = (int16_t)e.first;
+            T.hl_sign[pos] = (int8_t)e.second;
+            pos++;
+        }
+    }
+    T.hl_start[Nh] = (int16_t)pos;
+    // low-order pairs
+    std::vector<std::vector<std::pair<int, int>>> ll(Nq), cl(Nq);
+    for (int p = 0; p < T.npl; p++) {
+        const int64_t i1 = ops->Srs0_nnz[2 * p], j1 = ops->Srs0_nnz[2 * p + 1];
+        if (i1 < 1 || i1 > Nq || j1 < 1 || j1 > Nq || i1 <= j1) return fail_s(err, P2DE_E_INVALID, "Srs0_nnz entry out of range");
+        const int i = (int)i1 - 1, j = (int)j1 - 1;
+        const double cr = cm(ops->Srs0, Nq, i, j);
+        const double cs = Nd == 2 ? cm(ops->Srs0 + (int64_t)Nq * Nq, Nq, i, j) : 0.0;
+        if ((cr != 0.0) == (cs != 0.0) && Nd == 2)
+            return fail_s(err, P2DE_E_UNSUPPORTED, "a Srs0 pair acts in both or no directions");
+        if (Nd == 1 && cr == 0.0) return fail_s(err, P2DE_E_UNSUPPORTED, "zero Sr0 pair");
+        T.lp_i[p] = (uint8_t)i;
+        T.lp_j[p] = (uint8_t)j;
+        T.lp_dir[p] = (uint8_t)(cr != 0.0 ? 0 : 1);
+        T.lp_cr[p] = cr;
+        T.lp_cs[p] = cs;
+        ll[i].push_back({p, +1});
+        ll[j].push_back({p, -1});
+        cl[i].push_back({j, p});
+        cl[j].push_back({i, p});
+    }
+    pos = 0;
+    int cpos = 0;
+    for (int i = 0; i < Nq; i++) {
+        T.ll_start[i] = (int16_t)pos;
+        for (auto& e : ll[i]) {
+            T.ll_pair[pos] = (int16_t)e.first;
+            T.ll_sign[pos] = (int8_t)e.second;
+            pos++;
+        }
+        std::sort(cl[i].begin(), cl[i].end());
+        T.cl_start[i] = (int16_t)cpos;
+        for (auto& e : cl[i]) T.cl_pair[cpos++] = (int16_t)e.second;
+    }
+    T.ll_start[Nq] = (int16_t)pos;
+    T.cl_start[Nq] = (int16_t)cpos;
+    return P2DE_OK;
+}
+
+int setup_mesh(int Nd, int N1D, int K, const ElemTables& T, const p2de_geometry* geom, const p2de_bcdata* bc,
+               MeshArrays& out, std::string& err) {
+    const int Nq = Nd == 1 ? N1D : N1D * N1D, Nfp = Nd == 1 ? 2 : 4 * N1D, Nh = Nq + Nfp;
+    // per-element metric, constant inside the element
+    std::vector<Geo>& geo = out.geo;
+    geo.assign(K, Geo{});
+    const int NGEO = Nd == 1 ? 1 : 4;
+    for (int k = 0; k < K; k++) {
+        const double J = geom->Jq[(int64_t)Nq * k];
+        for (int i = 1; i < Nq; i++)
+            if (geom->Jq[i + (int64_t)Nq * k] != J)
+                return fail_s(err, P2DE_E_UNSUPPORTED, "Jq varies inside an element");
+        if (!(J > 0.0)) return fail_s(err, P2DE_E_INVALID, "Jq must be positive");
+        double gv[4] = {0, 0, 0, 0};
+        for (int gi = 0; gi < NGEO; gi++) {
+            const double* G = geom->GJh + (int64_t)gi * Nh * K;
+            gv[gi] = G[(int64_t)Nh * k];
+            for (int h = 1; h < Nh; h++)
+                if (G[h + (int64_t)Nh * k] != gv[gi])
+                    return fail_s(err, P2DE_E_UNSUPPORTED, "metric terms vary inside an element");
+        }
+        Geo g{J, gv[0], 0.0, 0.0, 0.0};
+        if (Nd == 2) {
+            g.sxJ = gv[1];
+            g.ryJ = gv[2];
+            g.syJ = gv[3];
+            if (g.sxJ != 0.0 || g.ryJ != 0.0)
+                return fail_s(err, P2DE_E_UNSUPPORTED, "elements must be axis aligned (sxJ = ryJ = 0)");
+        }
+        // the graph-viscosity direction test |Sx0| > 1e-10 (rhs_utils.jl:256) must pick the
+        // direction the pair acts in
+        for (int p = 0; p < T.npl && Nd == 2; p++) {
+            const double s0x = g.rxJ * T.lp_cr[p] + g.sxJ * T.lp_cs[p];
+            const int vdir = std::fabs(s0x) > 1e-10 ? 0 : 1;
+            if (vdir != T.lp_dir[p])
+                return fail_s(err, P2DE_E_UNSUPPORTED, "graph-viscosity direction test disagrees with the pair direction");
+        }
+        geo[k] = g;
+    }
+    // face connectivity and boundary codes (StateParam.jl:1-7)
+    std::vector<int>& nb_node = out.nb_node;
+    std::vector<int>& nb_face = out.nb_face;
+    std::vector<int>& bcode = out.bcode;
+    nb_node.assign((size_t)K * Nfp, 0);
+    nb_face.assign((size_t)K * Nfp, 0);
+    bcode.assign((size_t)K * Nfp, 0);
+    for (int k = 0; k < K; k++)
+        for (int f = 0; f < Nfp; f++) {
+            const int64_t m = bc->mapP[f + (int64_t)Nfp * k] - 1;
+            if (m < 0 || m >= (int64_t)Nfp * K) return fail_s(err, P2DE_E_INVALID, "mapP out of range");
+            const int kP = (int)(m / Nfp), iP = (int)(m % Nfp);
+            if (Nd == 2 && ((f < 2 * N1D) != (iP < 2 * N1D)))
+                return fail_s(err, P2DE_E_UNSUPPORTED, "mapP joins an x face to a y face");
+            nb_face[(size_t)k * Nfp + f] = (int)m;
+            nb_node[(size_t)k * Nfp + f] = kP * Nq + T.fq2q[iP];
+        }
+    if (bc->nI > 0 && (!bc->mapI || !bc->Ival)) return fail_s(err, P2DE_E_INVALID, "mapI/Ival");
+    if (bc->nO > 0 && !bc->mapO) return fail_s(err, P2DE_E_INVALID, "mapO");
+    for (int i = 0; i < bc->nI; i++) {
+        const int64_t m = bc->mapI[i] - 1;
+        if (m < 0 || m >= (int64_t)Nfp * K) return fail_s(err, P2DE_E_INVALID, "mapI out of range");
+        bcode[m] = i + 1;
+    }
+    for (int i = 0; i < bc->nO; i++) {   // enforce_BC! applies outflow after inflow
+        const int64_t m = bc->mapO[i] - 1;
+        if (m < 0 || m >= (int64_t)Nfp * K) return fail_s(err, P2DE_E_INVALID, "mapO out of range");
+        bcode[m] = -1;
+    }
+
+    return P2DE_OK;
+}
+
+}  // namespace p2de

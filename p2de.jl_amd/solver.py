@@ -22,6 +22,7 @@ LIB_PATH = Path(__file__).resolve().parent / "csrc" / "libp2de_hip.so"
 P2DE_OK, P2DE_E_INVALID, P2DE_E_UNSUPPORTED, P2DE_E_HIP, P2DE_E_ALLOC, P2DE_E_STATE, P2DE_E_POSITIVITY = range(7)
 FIELD_RHSU, FIELD_RHSH, FIELD_RHSL, FIELD_L_LOCAL = range(4)
 UPPER_BOUND_REFERENCE, UPPER_BOUND_INTENDED = 0, 1
+PRECISION_FP64, PRECISION_MIXED = 0, 1
 
 
 class P2DEError(RuntimeError):
@@ -102,6 +103,35 @@ def lib():
         L.p2de_set_variant.argtypes = [vp, C.c_int]
         L.p2de_get_variant.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.p2de_autotune.argtypes = [vp, C.c_int, C.c_char_p, C.POINTER(C.c_int)]
+        bp = C.c_void_p
+        L.p2de_batch_create.argtypes = [C.c_int, C.POINTER(_Config), C.POINTER(_Operators), C.POINTER(_Geometry),
+                                        C.POINTER(_BCData), C.POINTER(bp)]
+        L.p2de_batch_destroy.argtypes = [bp]
+        L.p2de_batch_destroy.restype = None
+        L.p2de_batch_num_instances.argtypes = [bp]
+        L.p2de_batch_num_dofs.argtypes = [bp, C.c_int]
+        L.p2de_batch_num_dofs.restype = C.c_int64
+        L.p2de_batch_set_state.argtypes = [bp, C.c_int, _dp]
+        L.p2de_batch_get_state.argtypes = [bp, C.c_int, _dp]
+        L.p2de_batch_device_state.argtypes = [bp, C.c_int, C.POINTER(vp)]
+        L.p2de_batch_set_time.argtypes = [bp, C.c_int, C.c_double]
+        L.p2de_batch_get_time.argtypes = [bp, C.c_int, _dp, _dp, C.POINTER(C.c_int)]
+        L.p2de_batch_rhs.argtypes = [bp, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp]
+        L.p2de_batch_set_debug.argtypes = [bp, C.c_int]
+        L.p2de_batch_get_field.argtypes = [bp, C.c_int, C.c_int, _dp]
+        L.p2de_batch_enqueue_steps.argtypes = [bp, C.c_int]
+        L.p2de_batch_synchronize.argtypes = [bp]
+        L.p2de_batch_run.argtypes = [bp, C.c_int, C.POINTER(C.c_int)]
+        L.p2de_batch_check_positivity.argtypes = [bp, C.c_int, C.POINTER(PosReport)]
+        L.p2de_batch_set_kernel_timing.argtypes = [bp, C.c_int]
+        L.p2de_batch_get_kernel_timing.argtypes = [bp, _dp, _ip]
+        L.p2de_batch_set_graph.argtypes = [bp, C.c_int]
+        L.p2de_batch_num_variants.argtypes = [bp]
+        L.p2de_batch_set_variant.argtypes = [bp, C.c_int]
+        L.p2de_batch_get_variant.argtypes = [bp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.p2de_batch_autotune.argtypes = [bp, C.c_int, C.c_char_p, C.POINTER(C.c_int)]
+        L.p2de_batch_last_error.argtypes = [bp]
+        L.p2de_batch_last_error.restype = C.c_char_p
         L.p2de_num_dofs.argtypes = [vp]
         L.p2de_num_dofs.restype = C.c_int64
         L.p2de_device_state.argtypes = [vp, C.POINTER(vp)]
@@ -120,6 +150,13 @@ EXPORTED_SYMBOLS = (
     "p2de_check_positivity", "p2de_enqueue_steps", "p2de_synchronize", "p2de_set_kernel_timing",
     "p2de_get_kernel_timing", "p2de_num_dofs", "p2de_device_state", "p2de_last_error", "p2de_status_string",
     "p2de_num_variants", "p2de_set_variant", "p2de_get_variant", "p2de_autotune",
+    "p2de_batch_create", "p2de_batch_destroy", "p2de_batch_num_instances", "p2de_batch_num_dofs",
+    "p2de_batch_set_state", "p2de_batch_get_state", "p2de_batch_device_state", "p2de_batch_set_time",
+    "p2de_batch_get_time", "p2de_batch_rhs", "p2de_batch_set_debug", "p2de_batch_get_field",
+    "p2de_batch_enqueue_steps", "p2de_batch_synchronize", "p2de_batch_run", "p2de_batch_check_positivity",
+    "p2de_batch_set_kernel_timing", "p2de_batch_get_kernel_timing", "p2de_batch_set_graph",
+    "p2de_batch_num_variants", "p2de_batch_set_variant", "p2de_batch_get_variant", "p2de_batch_autotune",
+    "p2de_batch_last_error",
 )
 
 
@@ -131,29 +168,37 @@ def _i(a):
     return a.ctypes.data_as(_ip) if a is not None and a.size else None
 
 
+def _descriptors(fp: FlatProblem, upper_bound_mode: int = None, precision: int = None):
+    """The C-ABI descriptors (p2de_config, p2de_operators, p2de_geometry, p2de_bcdata) of a
+    flat problem, plus the arrays they point into (which must outlive the create call)."""
+    mode = fp.upper_bound_mode if upper_bound_mode is None else upper_bound_mode
+    prec = getattr(fp, "precision", PRECISION_FP64) if precision is None else precision
+    cfg = _Config(Nd=fp.Nd, N=fp.N1D - 1, Kx=fp.Kx, Ky=fp.Ky, gamma=fp.gamma, zeta=fp.zeta, eta=fp.eta,
+                  POSTOL=fp.POSTOL, ZEROTOL=fp.ZEROTOL, CFL=fp.CFL, dt0=fp.dt0, t0=fp.t0, T=fp.T,
+                  high_order_surface_flux=1, upper_bound_mode=int(mode), precision=int(prec))
+    keep = {}
+    for n in ("Srsh_db", "Srs0", "Brs", "MinvVhT", "MinvVfT", "Vf", "Vf_low", "wq", "Jq", "GJh", "Ival"):
+        keep[n] = np.ascontiguousarray(getattr(fp, n), dtype=np.float64)
+    for n in ("Srsh_nnz", "Srs0_nnz", "fq2q", "q2fq", "mapP", "mapI", "mapO"):
+        keep[n] = np.ascontiguousarray(getattr(fp, n), dtype=np.int64)
+    ops = _Operators(Nq=fp.Nq, Nfp=fp.Nfp, Nh=fp.Nh, Srsh_db=_d(keep["Srsh_db"]), Srs0=_d(keep["Srs0"]),
+                     nnz_h=len(keep["Srsh_nnz"]) // 2, Srsh_nnz=_i(keep["Srsh_nnz"]),
+                     nnz_0=len(keep["Srs0_nnz"]) // 2, Srs0_nnz=_i(keep["Srs0_nnz"]), Brs=_d(keep["Brs"]),
+                     MinvVhT=_d(keep["MinvVhT"]), MinvVfT=_d(keep["MinvVfT"]), Vf=_d(keep["Vf"]),
+                     Vf_low=_d(keep["Vf_low"]), wq=_d(keep["wq"]), fq2q=_i(keep["fq2q"]), q2fq=_i(keep["q2fq"]))
+    geo = _Geometry(Jq=_d(keep["Jq"]), GJh=_d(keep["GJh"]))
+    bc = _BCData(mapP=_i(keep["mapP"]), nI=len(keep["mapI"]), mapI=_i(keep["mapI"]), Ival=_d(keep["Ival"]),
+                 nO=len(keep["mapO"]), mapO=_i(keep["mapO"]))
+    return cfg, ops, geo, bc, keep
+
+
 class DeviceSolver:
     """One native context; the state lives on the GPU between calls."""
 
-    def __init__(self, fp: FlatProblem, upper_bound_mode: int = None):
+    def __init__(self, fp: FlatProblem, upper_bound_mode: int = None, precision: int = None):
         self.fp = fp
-        mode = fp.upper_bound_mode if upper_bound_mode is None else upper_bound_mode
         self.Nc, self.Nq, self.K = fp.Nc, fp.Nq, fp.K
-        cfg = _Config(Nd=fp.Nd, N=fp.N1D - 1, Kx=fp.Kx, Ky=fp.Ky, gamma=fp.gamma, zeta=fp.zeta, eta=fp.eta,
-                      POSTOL=fp.POSTOL, ZEROTOL=fp.ZEROTOL, CFL=fp.CFL, dt0=fp.dt0, t0=fp.t0, T=fp.T,
-                      high_order_surface_flux=1, upper_bound_mode=int(mode))
-        keep = {}
-        for n in ("Srsh_db", "Srs0", "Brs", "MinvVhT", "MinvVfT", "Vf", "Vf_low", "wq", "Jq", "GJh", "Ival"):
-            keep[n] = np.ascontiguousarray(getattr(fp, n), dtype=np.float64)
-        for n in ("Srsh_nnz", "Srs0_nnz", "fq2q", "q2fq", "mapP", "mapI", "mapO"):
-            keep[n] = np.ascontiguousarray(getattr(fp, n), dtype=np.int64)
-        ops = _Operators(Nq=fp.Nq, Nfp=fp.Nfp, Nh=fp.Nh, Srsh_db=_d(keep["Srsh_db"]), Srs0=_d(keep["Srs0"]),
-                         nnz_h=len(keep["Srsh_nnz"]) // 2, Srsh_nnz=_i(keep["Srsh_nnz"]),
-                         nnz_0=len(keep["Srs0_nnz"]) // 2, Srs0_nnz=_i(keep["Srs0_nnz"]), Brs=_d(keep["Brs"]),
-                         MinvVhT=_d(keep["MinvVhT"]), MinvVfT=_d(keep["MinvVfT"]), Vf=_d(keep["Vf"]),
-                         Vf_low=_d(keep["Vf_low"]), wq=_d(keep["wq"]), fq2q=_i(keep["fq2q"]), q2fq=_i(keep["q2fq"]))
-        geo = _Geometry(Jq=_d(keep["Jq"]), GJh=_d(keep["GJh"]))
-        bc = _BCData(mapP=_i(keep["mapP"]), nI=len(keep["mapI"]), mapI=_i(keep["mapI"]), Ival=_d(keep["Ival"]),
-                     nO=len(keep["mapO"]), mapO=_i(keep["mapO"]))
+        cfg, ops, geo, bc, self._keep = _descriptors(fp, upper_bound_mode, precision)
         h = C.c_void_p()
         st = lib().p2de_create(C.byref(cfg), C.byref(ops), C.byref(geo), C.byref(bc), C.byref(h))
         if st != P2DE_OK:
@@ -270,6 +315,131 @@ class DeviceSolver:
 
     def num_dofs(self) -> int:
         return int(lib().p2de_num_dofs(self.h))
+
+
+class BatchSolver:
+    """Many independent 2D solves in mixed precision, advanced together (``p2de_batch_*``).
+    Instance ``i`` is ``fps[i]``; its state has the shape ``(K_i, Nq_i, 4)``."""
+
+    def __init__(self, fps, upper_bound_mode: int = None):
+        n = len(fps)
+        if n < 1:
+            raise P2DEError(P2DE_E_INVALID, "empty batch")
+        self.fps = list(fps)
+        cfgs, opss, geos, bcs = (_Config * n)(), (_Operators * n)(), (_Geometry * n)(), (_BCData * n)()
+        self._keep = []
+        for i, fp in enumerate(self.fps):
+            cfg, ops, geo, bc, keep = _descriptors(fp, upper_bound_mode, PRECISION_MIXED)
+            cfgs[i], opss[i], geos[i], bcs[i] = cfg, ops, geo, bc
+            self._keep.append(keep)
+        h = C.c_void_p()
+        st = lib().p2de_batch_create(n, cfgs, opss, geos, bcs, C.byref(h))
+        if st != P2DE_OK:
+            raise P2DEError(st, lib().p2de_batch_last_error(None).decode())
+        self.h = h
+        self.n = n
+
+    def _check(self, st):
+        if st != P2DE_OK:
+            raise P2DEError(st, lib().p2de_batch_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().p2de_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def shape(self, i: int):
+        fp = self.fps[i]
+        return (fp.K, fp.Nq, fp.Nc)
+
+    def num_dofs(self, i: int = -1) -> int:
+        return int(lib().p2de_batch_num_dofs(self.h, int(i)))
+
+    def set_state(self, i: int, Uq: np.ndarray):
+        U = np.ascontiguousarray(Uq, dtype=np.float64)
+        if U.size != int(np.prod(self.shape(i))):
+            raise P2DEError(P2DE_E_INVALID, f"state of instance {i} has {U.size} values")
+        self._check(lib().p2de_batch_set_state(self.h, int(i), _d(U)))
+
+    def get_state(self, i: int) -> np.ndarray:
+        out = np.empty(self.shape(i))
+        self._check(lib().p2de_batch_get_state(self.h, int(i), _d(out)))
+        return out
+
+    def set_time(self, t: float, i: int = -1):
+        self._check(lib().p2de_batch_set_time(self.h, int(i), float(t)))
+
+    def get_time(self, i: int):
+        t, dt, n = C.c_double(0.0), C.c_double(0.0), C.c_int(0)
+        self._check(lib().p2de_batch_get_time(self.h, int(i), C.byref(t), C.byref(dt), C.byref(n)))
+        return t.value, dt.value, n.value
+
+    def rhs(self, i: int, t: float, dt: float, nstage: int, want_rhs: bool = True):
+        out = np.empty(self.shape(i)) if want_rhs else None
+        dto = C.c_double(0.0)
+        self._check(lib().p2de_batch_rhs(self.h, int(i), float(t), float(dt), int(nstage), C.byref(dto),
+                                         _d(out) if want_rhs else None))
+        return dto.value, out
+
+    def set_debug(self, on: bool = True):
+        self._check(lib().p2de_batch_set_debug(self.h, 1 if on else 0))
+
+    def field(self, i: int, which: int) -> np.ndarray:
+        out = np.empty(self.shape(i))
+        self._check(lib().p2de_batch_get_field(self.h, int(i), int(which), _d(out)))
+        return out
+
+    def enqueue_steps(self, n: int):
+        self._check(lib().p2de_batch_enqueue_steps(self.h, int(n)))
+
+    def synchronize(self):
+        self._check(lib().p2de_batch_synchronize(self.h))
+
+    def run(self, max_steps: int = 0) -> int:
+        n = C.c_int(0)
+        self._check(lib().p2de_batch_run(self.h, int(max_steps), C.byref(n)))
+        return n.value
+
+    def check_positivity(self, i: int) -> PosReport:
+        r = PosReport()
+        self._check(lib().p2de_batch_check_positivity(self.h, int(i), C.byref(r)))
+        return r
+
+    def set_kernel_timing(self, on: bool):
+        self._check(lib().p2de_batch_set_kernel_timing(self.h, 1 if on else 0))
+
+    def kernel_timing(self):
+        ms = (C.c_double * 3)()
+        cnt = (C.c_int64 * 3)()
+        self._check(lib().p2de_batch_get_kernel_timing(self.h, ms, cnt))
+        return list(ms), list(cnt)
+
+    def set_graph(self, on: bool):
+        self._check(lib().p2de_batch_set_graph(self.h, 1 if on else 0))
+
+    def num_variants(self) -> int:
+        return int(lib().p2de_batch_num_variants(self.h))
+
+    def set_variant(self, v: int):
+        self._check(lib().p2de_batch_set_variant(self.h, int(v)))
+
+    def variant(self):
+        v, nt = C.c_int(), C.c_int()
+        E = (C.c_int * 6)()
+        self._check(lib().p2de_batch_get_variant(self.h, C.byref(v), C.byref(nt), E))
+        return v.value, nt.value, list(E)
+
+    def autotune(self, steps: int = 3, cache_path: str = None) -> int:
+        ch = C.c_int(-1)
+        self._check(lib().p2de_batch_autotune(self.h, int(steps), cache_path.encode() if cache_path else None,
+                                              C.byref(ch)))
+        return ch.value
 
 
 # ------------------------------------------------------------------- reference-shaped API

@@ -74,7 +74,7 @@ def fp2():
 @pytest.mark.parametrize("field,value,code", [
     ("basis", 1, S.P2DE_E_UNSUPPORTED), ("rhs_limiter", 1, S.P2DE_E_UNSUPPORTED),
     ("entropyproj_limiter", 1, S.P2DE_E_UNSUPPORTED), ("shockcapture", 1, S.P2DE_E_UNSUPPORTED),
-    ("high_order_surface_flux", 0, S.P2DE_E_UNSUPPORTED), ("precision", 7, S.P2DE_E_UNSUPPORTED),
+    ("high_order_surface_flux", 0, S.P2DE_E_UNSUPPORTED), ("precision", 7, S.P2DE_E_INVALID),
     ("upper_bound_mode", 5, S.P2DE_E_INVALID), ("Nd", 3, S.P2DE_E_INVALID), ("N", 9, S.P2DE_E_UNSUPPORTED),
     ("Kx", 0, S.P2DE_E_INVALID), ("gamma", 1.0, S.P2DE_E_INVALID)])
 def test_create_rejects_options(fp2, field, value, code):

@@ -1,0 +1,188 @@
+"""GPU tests of the mixed-precision batched path (p2de_batch_*) against the FP64 oracle.
+
+The mixed path computes the same scheme with FP32 flux differencing, surface and low-order
+fluxes and element assembly (FP64 node pressures, subcell flux potentials, limiter and RK
+stages), so it is compared with tolerances that FP32 rounding explains:
+  * rhsH and rhsL are sums of FP32 pair fluxes whose terms are as large as the physical
+    flux times the operator row (|f| sum_j |S_ij| / (wq J)), while the sum can be far
+    smaller (smooth flow, constant pressure): per component the worst node error is
+    bounded by 32 FP32 unit roundoffs (2^-24) of that term scale;
+  * the stage-1 CFL dt: relative 1e-6;
+  * rhsU with the intended bounds (continuous in the data): as rhsH;
+  * positivity (rho > 0, p > 0 in FP64 on the stored state) exactly after every stage;
+  * conservation of every component (periodic, FP64 flux potentials telescoping exactly);
+  * results do not depend on the batch composition, the launch shape or graph replay (bits).
+"""
+import numpy as np
+import pytest
+
+from conftest import PKG, normwise_rel, perturbed_state, problem, rel2
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+EPS32 = 2.0 ** -24
+TOL_TERMS = 32 * EPS32
+
+
+def term_scale(fp, U):
+    """Per component: max_nodes |f_c| * max_i sum_j |S_ij| * metric / min(wq J), the size of the
+    terms each node's RHS sums (Euler fluxes in both directions, LGL operators)."""
+    g = fp.gamma
+    rho, m1, m2, E = (U[..., c] for c in range(4))
+    p = (g - 1) * (E - 0.5 * (m1 * m1 + m2 * m2) / rho)
+    Fx = np.stack([m1, m1 * m1 / rho + p, m1 * m2 / rho, (E + p) * m1 / rho], -1)
+    Fy = np.stack([m2, m1 * m2 / rho, m2 * m2 / rho + p, (E + p) * m2 / rho], -1)
+    Fmax = np.maximum(np.abs(Fx).reshape(-1, 4).max(0), np.abs(Fy).reshape(-1, 4).max(0))
+    Nh, K = fp.Nh, fp.K
+    G = fp.GJh.reshape(-1, K, Nh)[:, :, 0]
+    Sr = np.abs(fp.Srsh_db[:Nh * Nh].reshape(Nh, Nh).T).sum(1).max() * np.abs(G[0]).max()
+    Ss = np.abs(fp.Srsh_db[Nh * Nh:].reshape(Nh, Nh).T).sum(1).max() * np.abs(G[3]).max()
+    wJ = (fp.wq[None, :] * fp.Jq.reshape(K, fp.Nq)).min()
+    return Fmax * max(Sr, Ss) / wJ
+
+
+def term_err(a, b, scale):
+    d = np.abs(np.asarray(a).reshape(-1, 4) - np.asarray(b).reshape(-1, 4)).max(0)
+    return float(np.max(d / scale))
+
+CASES = [
+    ("vortex_2d", dict(K=8)),
+    ("vortex_2d", dict(K=6, N=1)),
+    ("vortex_2d", dict(K=6, N=2)),
+    ("vortex_2d", dict(K=5, N=4)),
+    ("riemann4_2d", dict(K=12)),
+    ("kelvin_helmholtz", dict(K=8)),
+    ("periodic_wave", dict(Kx=12, Ky=10, N=3, kx=2, ky=1, u0=0.3, v0=-0.7, dip=(0.4, 0.6), dt0=2e-5)),
+]
+
+
+def _ids(c):
+    return f"{c[0]}-" + "-".join(f"{k}{v}" for k, v in c[1].items() if k in ("K", "N", "Kx"))
+
+
+def mixed_fp(name, mode=0, **kw):
+    solver, state, sp, fp = problem(name, **kw)
+    fp.upper_bound_mode = mode
+    return solver, state, sp, fp
+
+
+@pytest.mark.parametrize("case", CASES, ids=[_ids(c) for c in CASES])
+@pytest.mark.parametrize("mode", [0, 1], ids=["ref-bounds", "intended-bounds"])
+def test_mixed_rhs_against_oracle(case, mode):
+    from p2de_jl_amd.solver import BatchSolver, FIELD_RHSH, FIELD_RHSL
+    name, kw = case
+    solver, state, sp, fp = mixed_fp(name, mode, **kw)
+    print(f"case {name} {kw} mode {mode}", flush=True)
+    bs = BatchSolver([fp])
+    bs.set_debug(True)
+    orc = Oracle(fp)
+    U0 = state.preallocation.Uq
+    Nc = fp.Nc
+    for trial, U in enumerate([U0, perturbed_state(U0, seed=3)]):
+        bs.set_state(0, U)
+        for nstage, dt in [(1, 0.9 * fp.CFL * fp.dt0), (2, 0.5 * fp.CFL * fp.dt0)]:
+            dt_gpu, r_gpu = bs.rhs(0, 0.0, dt, nstage)
+            dt_cpu = orc.rhs(U, 0.0, dt, nstage)
+            sc = term_scale(fp, U)
+            eH2 = rel2(bs.field(0, FIELD_RHSH), orc.field("rhsH"), Nc)
+            eL2 = rel2(bs.field(0, FIELD_RHSL), orc.field("rhsL"), Nc)
+            eU2 = rel2(r_gpu, orc.field("rhsU"), Nc)
+            eH = term_err(bs.field(0, FIELD_RHSH), orc.field("rhsH"), sc)
+            eL = term_err(bs.field(0, FIELD_RHSL), orc.field("rhsL"), sc)
+            eU = term_err(r_gpu, orc.field("rhsU"), sc)
+            edt = abs(dt_gpu - dt_cpu) / dt_cpu
+            print(f"  trial {trial} stage {nstage}: relative 2-norm rhsH {eH2:.2e} rhsL {eL2:.2e} rhsU {eU2:.2e}; "
+                  f"worst node / term scale: rhsH {eH:.2e} rhsL {eL:.2e} rhsU {eU:.2e}; dt {edt:.2e}", flush=True)
+            assert max(eH, eL) <= TOL_TERMS
+            assert edt <= 1e-6
+            if mode == 1:
+                assert eU <= TOL_TERMS
+
+
+def _instances():
+    """Three instances of different degree; the second has a near-vacuum dip."""
+    out = []
+    for kw in (dict(Kx=10, Ky=7, N=2, kx=1, ky=2, u0=0.5, v0=0.25),
+               dict(Kx=9, Ky=11, N=3, kx=3, ky=1, u0=-0.8, v0=0.6, dip=(0.3, 0.7)),
+               dict(Kx=6, Ky=8, N=4, kx=2, ky=2, u0=0.1, v0=-0.9)):
+        param, ic, bc = PKG.configs.periodic_wave(**kw)
+        solver, state, sp = PKG.initialize_DG(param, ic, bc)
+        fp = PKG.flat_problem(solver, sp)
+        fp.dt0 = 0.9 * PKG.timestep.cfl_dt(fp, state.preallocation.Uq) / fp.CFL
+        out.append((fp, state.preallocation.Uq.copy()))
+    return out
+
+
+def _mass(fp, U):
+    wJ = fp.wq[None, :] * fp.Jq.reshape(fp.K, fp.Nq)
+    return np.einsum("kq,kqc->c", wJ, U)
+
+
+def test_batch_positivity_conservation_and_independence():
+    from p2de_jl_amd.solver import BatchSolver
+    inst = _instances()
+    fps = [fp for fp, _ in inst]
+    bs = BatchSolver(fps)
+    for i, (fp, U) in enumerate(inst):
+        bs.set_state(i, U)
+    bs.set_time(0.0)
+    steps = 20
+    bs.enqueue_steps(steps)
+    bs.synchronize()
+    res = [bs.get_state(i) for i in range(len(inst))]
+    for i, (fp, U) in enumerate(inst):
+        t, dt, n = bs.get_time(i)
+        rep = bs.check_positivity(i)
+        m0, m1 = _mass(fp, U), _mass(fp, res[i])
+        drift = np.max(np.abs(m1 - m0) / np.maximum(np.abs(m0), 1e-300))
+        print(f"  instance {i}: N={fp.N1D - 1} K={fp.K} t={t:.4e} steps={n} violations={rep.violations} "
+              f"min rho {rep.min_rho:.3e} min p {rep.min_p:.3e} mass drift {drift:.2e}", flush=True)
+        assert n == steps and rep.violations == 0 and rep.min_rho > 0 and rep.min_p > 0
+        assert drift <= 1e-12
+    # each instance alone gives the same bits
+    for i, (fp, U) in enumerate(inst):
+        one = BatchSolver([fp])
+        one.set_state(0, U)
+        one.enqueue_steps(steps)
+        one.synchronize()
+        assert np.array_equal(one.get_state(0), res[i]), f"instance {i} differs when run alone"
+    # direct launches (no graph) and every launch shape give the same bits
+    bs.set_graph(False)
+    for v in range(bs.num_variants()):
+        bs.set_variant(v)
+        for i, (fp, U) in enumerate(inst):
+            bs.set_state(i, U)
+        bs.set_time(0.0)
+        bs.enqueue_steps(steps)
+        bs.synchronize()
+        print(f"  variant {bs.variant()}", flush=True)
+        for i in range(len(inst)):
+            assert np.array_equal(bs.get_state(i), res[i])
+
+
+def test_mixed_trajectory_against_fp64():
+    """20 SSP-RK3 steps of the vortex on a 16x16 mesh: mixed against the FP64 drop-in."""
+    from p2de_jl_amd.solver import BatchSolver, DeviceSolver
+    solver, state, sp, fp = problem("vortex_2d", K=16)
+    U0 = state.preallocation.Uq
+    dev = DeviceSolver(fp)
+    dev.set_state(U0)
+    dev.set_time(0.0)
+    dev.enqueue_steps(20)
+    dev.synchronize()
+    U64 = dev.get_state()
+    bs = BatchSolver([fp])
+    bs.set_state(0, U0)
+    bs.set_time(0.0)
+    bs.enqueue_steps(20)
+    bs.synchronize()
+    Um = bs.get_state(0)
+    wJ = fp.wq[None, :] * fp.Jq.reshape(fp.K, fp.Nq)
+    l1 = np.sum(wJ * np.abs(Um[..., 0] - U64[..., 0])) / np.sum(wJ * np.abs(U64[..., 0]))
+    linf = np.max(np.abs(Um[..., 0] - U64[..., 0])) / np.max(np.abs(U64[..., 0]))
+    t64 = dev.get_time()[0]
+    tm = bs.get_time(0)[0]
+    print(f"  density L1 {l1:.2e} Linf {linf:.2e}  t {tm:.10e} vs {t64:.10e}", flush=True)
+    assert abs(tm - t64) <= 1e-6 * t64
+    assert l1 <= 1e-6 and linf <= 1e-5
