@@ -1,12 +1,15 @@
 // mixed_kernels.hip -- mixed-precision (FP32 bulk, FP64 limiter and RK) fused stage kernels of
 // the limited ESDG scheme on 2D LGL quads, for gfx950.
 //
-// Per SSP-RK3 step: init (per instance: done flag, dt cap, queue reset), the CFL pass
+// Per SSP-RK3 step: init (per instance: done flag, dt cap), the CFL pass and its reduction
 // (global min of the low-order time step per instance, low_order_graph_viscosity.jl:222-243),
 // three fused stage launches (low- and high-order RHS, subcell limiter, RK stage, positivity
-// check) and a post kernel (t += dt).  Each launch is a persistent workgroup pool: a
-// workgroup takes tiles of E_N elements from the queue chunk of its XCD (blockIdx % 8, then
-// the other chunks), so neighbouring elements share an L2.
+// check) and a post kernel (t += dt).  Each launch is a persistent workgroup pool: the tile
+// list (E_N elements of one instance per tile) is cut into 8 contiguous chunks, one per XCD,
+// and the workgroups of an XCD (blockIdx % 8) stride through its chunk, so neighbouring
+// elements share an L2.  No global atomics sit on the tile path: device-scope atomics to one
+// address serialise across the XCDs (tens of ns each), so the CFL minimum is written per
+// tile and reduced per instance by a small kernel.
 //
 // Algebra that the LGL collocation makes exact, used here instead of computing roundoff:
 //  * the entropy projection of a face node is u(v(u_q)) = u_q (rhs.jl:84-94 with Vf a
@@ -160,36 +163,69 @@ __device__ __forceinline__ void pair_ab(int q, int& a, int& b) {
     b = q;
 }
 
+// LDS of one tile, component-major (structure of arrays) so that consecutive lanes touch
+// consecutive words: node n = e*NQ + i, face node e*NFP + f, pairs e*NPV + p and e*NPL + p,
+// subcell faces e*NFB + r.  Arrays whose lifetimes do not overlap share storage.
 template <int N1D, int E>
 struct SmemM {
     static constexpr int NQ = N1D * N1D, NFP = 4 * N1D;
     static constexpr int NPV = MixTab<N1D>::NPV, NPL = MixTab<N1D>::NPL, NFB = 2 * N1D * (N1D + 1);
-    double U[E][NQ][4];
+    static constexpr int M = E * NQ, MF = E * NFP, MV = E * NPV, ML = E * NPL, MB = E * NFB;
+    double U[4][M];
     double geo[E][4];            // J, rxJ, syJ, 1/J
-    float BF[E][NFP][4];         // B*f* - LF (high order == low order at interior faces)
-    float BFL[E][NFP][4];        // low-order surface term (differs at inflow faces)
-    float RH[E][NQ][2][4];       // rhsxyH
-    float RL[E][NQ][2][4];       // rhsxyL
+    float BF[4][MF];             // B*f* - LF (high order == low order at interior faces)
+    float BFL[4][MF];            // low-order surface term (differs at inflow faces)
+    float RH[2][4][M];           // rhsxyH
+    float RL[2][4][M];           // rhsxyL
     union {
         struct {
-            NodeP np[E][NQ];
-            float PH[E][NPV][4];
-            float PL[E][NPL][4];
+            float np[8][M];      // NodeP fields
+            float PH[4][MV];
+            float PL[4][ML];
         } a;
         struct {
-            double fL[E][NFB][4];    // low-order subcell flux potentials
-            double df[E][NFB][4];    // f_bar_H - f_bar_L
-            double l[E][NFB];
-            double UL[E][NQ][4];
-            double Lrhoe[E][NQ];
+            double fL[4][MB];    // low-order subcell flux potentials
+            double df[4][MB];    // f_bar_H - f_bar_L
+            double l[MB];
+            double UL[4][M];
+            double Lrhoe[M];
         } b;
     } x;
 };
 
+template <class S>
+__device__ __forceinline__ NodeP ld_np(const S& sm, int n) {
+    NodeP P;
+    P.rho = sm.x.a.np[0][n];
+    P.u = sm.x.a.np[1][n];
+    P.v = sm.x.a.np[2][n];
+    P.p = sm.x.a.np[3][n];
+    P.beta = sm.x.a.np[4][n];
+    P.E = sm.x.a.np[5][n];
+    P.wsx = sm.x.a.np[6][n];
+    P.wsy = sm.x.a.np[7][n];
+    return P;
+}
+template <class S>
+__device__ __forceinline__ void st_np(S& sm, int n, const NodeP& P) {
+    sm.x.a.np[0][n] = P.rho;
+    sm.x.a.np[1][n] = P.u;
+    sm.x.a.np[2][n] = P.v;
+    sm.x.a.np[3][n] = P.p;
+    sm.x.a.np[4][n] = P.beta;
+    sm.x.a.np[5][n] = P.E;
+    sm.x.a.np[6][n] = P.wsx;
+    sm.x.a.np[7][n] = P.wsy;
+}
+template <class S>
+__device__ __forceinline__ float ws_dir(const S& sm, int n, int d) {
+    return sm.x.a.np[6 + d][n];
+}
+
 template <int N1D, int E>
 struct SmemDt {
-    static constexpr int NQ = N1D * N1D;
-    float ws[E][NQ][2];
+    static constexpr int NQ = N1D * N1D, M = E * NQ;
+    float ws[2][M];
     double geo[E][3];
     double red[16];
 };
@@ -199,18 +235,25 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
                                            int nel, SmemM<N1D, E>& sm) {
     using Sm = SmemM<N1D, E>;
     constexpr int NQ = Sm::NQ, NFP = Sm::NFP, NPV = Sm::NPV, NPL = Sm::NPL, NFB = Sm::NFB;
-    constexpr int NP1 = MixTab<N1D>::NP1;
+    constexpr int NP1 = MixTab<N1D>::NP1, LW = N1D - 1;
     const int tid = threadIdx.x;
     const double gd = I.gamma;
     const float g2m1 = (float)(2.0 * (gd - 1.0));
     const double* Uin = a.Uin + I.node_off * 4;
     StepScalars* sc = I.sc;
+    const int mN = nel * NQ;      // valid nodes of the tile
 
-    // ---- 0: element states and metric -> LDS
+    // ---- 0: element states (one node per lane, 32 contiguous bytes) and metric -> LDS
     {
-        const double* src = Uin + (size_t)k0 * NQ * 4;
-        double* dst = &sm.U[0][0][0];
-        for (int w = tid; w < nel * NQ * 4; w += NT) dst[w] = src[w];
+        const double4* src = (const double4*)(Uin + (size_t)k0 * NQ * 4);
+#pragma unroll 1
+        for (int n = tid; n < mN; n += NT) {
+            const double4 v = src[n];
+            sm.U[0][n] = v.x;
+            sm.U[1][n] = v.y;
+            sm.U[2][n] = v.z;
+            sm.U[3][n] = v.w;
+        }
         for (int w = tid; w < nel; w += NT) {
             const Geo G = I.geo[k0 + w];
             sm.geo[w][0] = G.J;
@@ -223,28 +266,26 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
 
     // ---- 1: node primitives (FP64 energy differences -> FP32)
 #pragma unroll 1
-    for (int w = tid; w < E * NQ; w += NT) {
-        const int e = w / NQ, i = w - e * NQ;
-        if (e >= nel) continue;
-        sm.x.a.np[e][i] = node_prims(gd, sm.U[e][i]);
+    for (int n = tid; n < mN; n += NT) {
+        const double U[4] = {sm.U[0][n], sm.U[1][n], sm.U[2][n], sm.U[3][n]};
+        st_np(sm, n, node_prims(gd, U));
     }
     __syncthreads();
 
     // ---- 2: surface fluxes at the face nodes (flux_differencing.jl:90-151, 223-272;
     //      low_order_graph_viscosity.jl:51-124, 175-204)
 #pragma unroll 1
-    for (int w = tid; w < E * NFP; w += NT) {
+    for (int w = tid; w < nel * NFP; w += NT) {
         const int e = w / NFP, f = w - e * NFP;
-        if (e >= nel) continue;
         const int k = k0 + e;
         const int ld = f < 2 * N1D ? 0 : 1;
-        const int q = face_node<N1D>(f);
-        const NodeP P = sm.x.a.np[e][q];
+        const int nq = e * NQ + face_node<N1D>(f);
+        const NodeP P = ld_np(sm, nq);
         const size_t fi = (size_t)k * NFP + f;
         const int bcc = I.bc[fi];
         const int nb = I.nb_node[fi];
-        double UP[4];
-        for (int c = 0; c < 4; c++) UP[c] = Uin[(size_t)nb * 4 + c];
+        const double4 up = ((const double4*)Uin)[nb];
+        const double UP[4] = {up.x, up.y, up.z, up.w};
         const NodeP PP = node_prims(gd, UP);
         // |B| from both elements' metric, so that the two sides of an interface compute exactly
         // negated fluxes (the per-element metrics agree to FP64 roundoff, which FP32 rounding
@@ -256,19 +297,20 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
         const float LF = 0.5f * Bmag * fmaxf(ld == 0 ? P.wsx : P.wsy, ld == 0 ? PP.wsx : PP.wsy);
         float F[4], FP[4];
         flux_dir(P, ld, F);
-        const double* Uq = sm.U[e][q];
+        double Uq[4];
+        for (int c = 0; c < 4; c++) Uq[c] = sm.U[c][nq];
         if (bcc == 0) {
             flux_dir(PP, ld, FP);
             for (int c = 0; c < 4; c++) {
                 const float v = fmaf(Bf, 0.5f * (F[c] + FP[c]), -LF * (float)(UP[c] - Uq[c]));
-                sm.BF[e][f][c] = v;
-                sm.BFL[e][f][c] = v;
+                sm.BF[c][w] = v;
+                sm.BFL[c][w] = v;
             }
         } else if (bcc < 0) {   // outflow: uP = own node, no dissipation
             for (int c = 0; c < 4; c++) {
                 const float v = Bf * F[c];
-                sm.BF[e][f][c] = v;
-                sm.BFL[e][f][c] = v;
+                sm.BF[c][w] = v;
+                sm.BFL[c][w] = v;
             }
         } else {                // inflow: uP = Ival; the low-order flux keeps its dissipation
             double UI[4];
@@ -277,67 +319,64 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
             flux_dir(PI, ld, FP);
             for (int c = 0; c < 4; c++) {
                 const float v = Bf * (0.5f * (F[c] + FP[c]));
-                sm.BF[e][f][c] = v;
-                sm.BFL[e][f][c] = v - LF * (float)(UI[c] - Uq[c]);
+                sm.BF[c][w] = v;
+                sm.BFL[c][w] = v - LF * (float)(UI[c] - Uq[c]);
             }
         }
     }
 
     // ---- 3a: high-order volume pairs (flux_differencing_volume!, flux_differencing.jl:164-211)
 #pragma unroll 1
-    for (int w = tid; w < E * NPV; w += NT) {
+    for (int w = tid; w < nel * NPV; w += NT) {
         const int e = w / NPV, p = w - e * NPV;
-        if (e >= nel) continue;
         const int d = p / (N1D * NP1), r = p - d * (N1D * NP1), line = r / NP1;
         int aa, bb;
         pair_ab(r - line * NP1, aa, bb);
-        const int i = lnode<N1D>(d, line, aa), j = lnode<N1D>(d, line, bb);
+        const int i = e * NQ + lnode<N1D>(d, line, aa), j = e * NQ + lnode<N1D>(d, line, bb);
         const float Sd = (float)sm.geo[e][1 + d] * T.cH[p];
         float F[4];
-        fS_f(g2m1, sm.x.a.np[e][i], sm.x.a.np[e][j], d, F);
-        for (int c = 0; c < 4; c++) sm.x.a.PH[e][p][c] = Sd * F[c];
+        fS_f(g2m1, ld_np(sm, i), ld_np(sm, j), d, F);
+        for (int c = 0; c < 4; c++) sm.x.a.PH[c][w] = Sd * F[c];
     }
     // ---- 3b: low-order line pairs (accumulate_low_order_rhs_volume!, :139-173)
 #pragma unroll 1
-    for (int w = tid; w < E * NPL; w += NT) {
+    for (int w = tid; w < nel * NPL; w += NT) {
         const int e = w / NPL, p = w - e * NPL;
-        if (e >= nel) continue;
-        const int d = p / (N1D * (N1D - 1)), r = p - d * (N1D * (N1D - 1)), line = r / (N1D - 1);
-        const int s = r - line * (N1D - 1);
-        const int i = lnode<N1D>(d, line, s + 1), j = lnode<N1D>(d, line, s);
+        const int d = p / (N1D * LW), r = p - d * (N1D * LW), line = r / LW;
+        const int s = r - line * LW;
+        const int i = e * NQ + lnode<N1D>(d, line, s + 1), j = e * NQ + lnode<N1D>(d, line, s);
         const float S0 = (float)sm.geo[e][1 + d] * T.cL[p];
-        const NodeP& Pi = sm.x.a.np[e][i];
-        const NodeP& Pj = sm.x.a.np[e][j];
+        const NodeP Pi = ld_np(sm, i), Pj = ld_np(sm, j);
         const float lam = fabsf(S0) * fmaxf(d == 0 ? Pi.wsx : Pi.wsy, d == 0 ? Pj.wsx : Pj.wsy);
         float Fi[4], Fj[4];
         flux_dir(Pi, d, Fi);
         flux_dir(Pj, d, Fj);
         for (int c = 0; c < 4; c++)
-            sm.x.a.PL[e][p][c] = fmaf(S0, Fi[c] + Fj[c], -lam * (float)(sm.U[e][j][c] - sm.U[e][i][c]));
+            sm.x.a.PL[c][w] = fmaf(S0, Fi[c] + Fj[c], -lam * (float)(sm.U[c][j] - sm.U[c][i]));
     }
     __syncthreads();
 
     // ---- 4: node assembly (assemble_rhs!, flux_differencing.jl:331-361; low-order surface and
     //      mass scaling, low_order_graph_viscosity.jl:175-220)
 #pragma unroll 1
-    for (int w = tid; w < E * NQ; w += NT) {
-        const int e = w / NQ, i = w - e * NQ;
-        if (e >= nel) continue;
+    for (int n = tid; n < mN; n += NT) {
+        const int e = n / NQ, i = n - e * NQ;
         const int ia = i % N1D, ib = i / N1D;
+        const int pv = e * NPV, pl = e * NPL, fe = e * NFP;
         float Qx[4] = {0.f, 0.f, 0.f, 0.f}, Qy[4] = {0.f, 0.f, 0.f, 0.f};
         for (int o = 0; o < N1D; o++) {
             if (o == ia) continue;
             const int hi = ia > o ? ia : o, lo = ia > o ? o : ia;
             const float sg = ia > o ? 1.f : -1.f;
-            const int p = ib * NP1 + hi * (hi - 1) / 2 + lo;
-            for (int c = 0; c < 4; c++) Qx[c] = fmaf(sg, sm.x.a.PH[e][p][c], Qx[c]);
+            const int p = pv + ib * NP1 + hi * (hi - 1) / 2 + lo;
+            for (int c = 0; c < 4; c++) Qx[c] = fmaf(sg, sm.x.a.PH[c][p], Qx[c]);
         }
         for (int o = 0; o < N1D; o++) {
             if (o == ib) continue;
             const int hi = ib > o ? ib : o, lo = ib > o ? o : ib;
             const float sg = ib > o ? 1.f : -1.f;
-            const int p = N1D * NP1 + ia * NP1 + hi * (hi - 1) / 2 + lo;
-            for (int c = 0; c < 4; c++) Qy[c] = fmaf(sg, sm.x.a.PH[e][p][c], Qy[c]);
+            const int p = pv + N1D * NP1 + ia * NP1 + hi * (hi - 1) / 2 + lo;
+            for (int c = 0; c < 4; c++) Qy[c] = fmaf(sg, sm.x.a.PH[c][p], Qy[c]);
         }
         const float av = T.a_v[i];
         float hx[4], hy[4];
@@ -347,42 +386,41 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
         }
         // low order: -Q0F1 of the line neighbours
         float lx[4] = {0.f, 0.f, 0.f, 0.f}, ly[4] = {0.f, 0.f, 0.f, 0.f};
-        constexpr int LW = N1D - 1;
         if (ia > 0)
-            for (int c = 0; c < 4; c++) lx[c] -= sm.x.a.PL[e][ib * LW + ia - 1][c];
+            for (int c = 0; c < 4; c++) lx[c] -= sm.x.a.PL[c][pl + ib * LW + ia - 1];
         if (ia < N1D - 1)
-            for (int c = 0; c < 4; c++) lx[c] += sm.x.a.PL[e][ib * LW + ia][c];
+            for (int c = 0; c < 4; c++) lx[c] += sm.x.a.PL[c][pl + ib * LW + ia];
         if (ib > 0)
-            for (int c = 0; c < 4; c++) ly[c] -= sm.x.a.PL[e][N1D * LW + ia * LW + ib - 1][c];
+            for (int c = 0; c < 4; c++) ly[c] -= sm.x.a.PL[c][pl + N1D * LW + ia * LW + ib - 1];
         if (ib < N1D - 1)
-            for (int c = 0; c < 4; c++) ly[c] += sm.x.a.PL[e][N1D * LW + ia * LW + ib][c];
+            for (int c = 0; c < 4; c++) ly[c] += sm.x.a.PL[c][pl + N1D * LW + ia * LW + ib];
         // face terms (at most one r-face and one s-face per node)
         const int fx = ia == 0 ? ib : ia == N1D - 1 ? N1D + ib : -1;
         const int fy = ib == 0 ? 2 * N1D + ia : ib == N1D - 1 ? 3 * N1D + ia : -1;
         if (fx >= 0) {
             const float af = T.a_fx[i];
             for (int c = 0; c < 4; c++) {
-                hx[c] = fmaf(af, sm.BF[e][fx][c], hx[c]);
-                lx[c] -= sm.BFL[e][fx][c];
+                hx[c] = fmaf(af, sm.BF[c][fe + fx], hx[c]);
+                lx[c] -= sm.BFL[c][fe + fx];
             }
         }
         if (fy >= 0) {
             const float af = T.a_fy[i];
             for (int c = 0; c < 4; c++) {
-                hy[c] = fmaf(af, sm.BF[e][fy][c], hy[c]);
-                ly[c] -= sm.BFL[e][fy][c];
+                hy[c] = fmaf(af, sm.BF[c][fe + fy], hy[c]);
+                ly[c] -= sm.BFL[c][fe + fy];
             }
         }
         const float nJ = -(float)sm.geo[e][3];
         const float iwJ = (float)(sm.geo[e][3] / T.wq[i]);
         for (int c = 0; c < 4; c++) {
-            sm.RH[e][i][0][c] = hx[c] * nJ;
-            sm.RH[e][i][1][c] = hy[c] * nJ;
-            sm.RL[e][i][0][c] = lx[c] * iwJ;
-            sm.RL[e][i][1][c] = ly[c] * iwJ;
+            sm.RH[0][c][n] = hx[c] * nJ;
+            sm.RH[1][c][n] = hy[c] * nJ;
+            sm.RL[0][c][n] = lx[c] * iwJ;
+            sm.RL[1][c][n] = ly[c] * iwJ;
         }
         if (a.dbg_rhsH) {
-            const size_t o = ((size_t)I.node_off + (size_t)(k0 + e) * NQ + i) * 4;
+            const size_t o = ((size_t)I.node_off + (size_t)k0 * NQ + n) * 4;
             for (int c = 0; c < 4; c++) {
                 a.dbg_rhsH[o + c] = (double)(hx[c] * nJ) + (double)(hy[c] * nJ);
                 a.dbg_rhsL[o + c] = (double)(lx[c] * iwJ) + (double)(ly[c] * iwJ);
@@ -393,35 +431,35 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
 
     // ---- 5a: subcell flux potentials in FP64 (accumulate_f_bar!, subcell.jl:163-206)
 #pragma unroll 1
-    for (int w = tid; w < E * 2 * N1D; w += NT) {
+    for (int w = tid; w < nel * 2 * N1D; w += NT) {
         const int e = w / (2 * N1D), L = w - e * (2 * N1D);
-        if (e >= nel) continue;
         const int d = L / N1D, line = L - d * N1D;
         const double J = sm.geo[e][0];
-        const int f0 = d == 0 ? line : 2 * N1D + line;
-        const int f1 = d == 0 ? N1D + line : 3 * N1D + line;
-        const int base = L * (N1D + 1);
+        const int f0 = e * NFP + (d == 0 ? line : 2 * N1D + line);
+        const int f1 = e * NFP + (d == 0 ? N1D + line : 3 * N1D + line);
+        const int base = e * NFB + L * (N1D + 1);
         double fh[4], fl[4];
         for (int c = 0; c < 4; c++) {
-            fh[c] = (double)sm.BF[e][f0][c];
-            fl[c] = (double)sm.BFL[e][f0][c];
-            sm.x.b.fL[e][base][c] = fl[c];
-            sm.x.b.df[e][base][c] = fh[c] - fl[c];
+            fh[c] = (double)sm.BF[c][f0];
+            fl[c] = (double)sm.BFL[c][f0];
+            sm.x.b.fL[c][base] = fl[c];
+            sm.x.b.df[c][base] = fh[c] - fl[c];
         }
         for (int s = 1; s < N1D; s++) {
             const int q = lnode<N1D>(d, line, s - 1);
+            const int n = e * NQ + q;
             const double wJ = T.wq[q] * J;
             for (int c = 0; c < 4; c++) {
-                fh[c] = fma(wJ, (double)sm.RH[e][q][d][c], fh[c]);
-                fl[c] = fma(wJ, (double)sm.RL[e][q][d][c], fl[c]);
-                sm.x.b.fL[e][base + s][c] = fl[c];
-                sm.x.b.df[e][base + s][c] = fh[c] - fl[c];
+                fh[c] = fma(wJ, (double)sm.RH[d][c][n], fh[c]);
+                fl[c] = fma(wJ, (double)sm.RL[d][c][n], fl[c]);
+                sm.x.b.fL[c][base + s] = fl[c];
+                sm.x.b.df[c][base + s] = fh[c] - fl[c];
             }
         }
         for (int c = 0; c < 4; c++) {
-            const double h = -(double)sm.BF[e][f1][c], lo = -(double)sm.BFL[e][f1][c];
-            sm.x.b.fL[e][base + N1D][c] = lo;
-            sm.x.b.df[e][base + N1D][c] = h - lo;
+            const double h = -(double)sm.BF[c][f1], lo = -(double)sm.BFL[c][f1];
+            sm.x.b.fL[c][base + N1D] = lo;
+            sm.x.b.df[c][base + N1D] = h - lo;
         }
     }
     __syncthreads();
@@ -431,100 +469,108 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
     const double dt_lim = (a.mode == MODE_STAGE2 || a.mode == MODE_STAGE3) ? dt : sc->dt_lim;
     const double zeta = I.zeta;
 #pragma unroll 1
-    for (int w = tid; w < E * NQ; w += NT) {
-        const int e = w / NQ, i = w - e * NQ;
-        if (e >= nel) continue;
+    for (int n = tid; n < mN; n += NT) {
+        const int e = n / NQ, i = n - e * NQ;
         const int ia = i % N1D, ib = i / N1D;
-        const int rx = ib * (N1D + 1) + ia, ry = (N1D + ia) * (N1D + 1) + ib;
+        const int rx = e * NFB + ib * (N1D + 1) + ia, ry = e * NFB + (N1D + ia) * (N1D + 1) + ib;
         const double iwJ = sm.geo[e][3] * rcp64(T.wq[i]);
         double uL[4];
         for (int c = 0; c < 4; c++) {
-            const double r = (sm.x.b.fL[e][rx + 1][c] - sm.x.b.fL[e][rx][c]) +
-                             (sm.x.b.fL[e][ry + 1][c] - sm.x.b.fL[e][ry][c]);
-            uL[c] = sm.U[e][i][c] + dt_lim * (r * iwJ);
-            sm.x.b.UL[e][i][c] = uL[c];
+            const double r = (sm.x.b.fL[c][rx + 1] - sm.x.b.fL[c][rx]) + (sm.x.b.fL[c][ry + 1] - sm.x.b.fL[c][ry]);
+            uL[c] = sm.U[c][n] + dt_lim * (r * iwJ);
+            sm.x.b.UL[c][n] = uL[c];
         }
-        sm.x.b.Lrhoe[e][i] = zeta * (uL[3] - 0.5 * (uL[1] * uL[1] + uL[2] * uL[2]) * rcp64(uL[0]));
+        sm.x.b.Lrhoe[n] = zeta * (uL[3] - 0.5 * (uL[1] * uL[1] + uL[2] * uL[2]) * rcp64(uL[0]));
     }
     __syncthreads();
 
     // ---- 5b: limiter parameter of every subcell face (subcell_bound_limiter!, subcell.jl:248-349)
     const double fac_dt = 4.0 * dt_lim;
 #pragma unroll 1
-    for (int w = tid; w < E * NFB; w += NT) {
+    for (int w = tid; w < nel * NFB; w += NT) {
         const int e = w / NFB, r = w - e * NFB;
-        if (e >= nel) continue;
         const int L = r / (N1D + 1), s = r - L * (N1D + 1);
         const int d = L / N1D, line = L - d * N1D;
         double l = 1.0;
-        double P[4];
+        double P[4], uL[4];
         if (s < N1D) {   // node above/right of the face
-            const int q = lnode<N1D>(d, line, s);
+            const int q = lnode<N1D>(d, line, s), n = e * NQ + q;
             const double sc_ = -fac_dt * sm.geo[e][3] * rcp64(T.wq[q]);
-            for (int c = 0; c < 4; c++) P[c] = sc_ * sm.x.b.df[e][r][c];
-            const double* uL = sm.x.b.UL[e][q];
-            l = fmin(l, lim_param(uL, zeta * uL[0], sm.x.b.Lrhoe[e][q], P, I.ZEROTOL, I.upper_mode));
+            for (int c = 0; c < 4; c++) {
+                P[c] = sc_ * sm.x.b.df[c][w];
+                uL[c] = sm.x.b.UL[c][n];
+            }
+            l = fmin(l, lim_param(uL, zeta * uL[0], sm.x.b.Lrhoe[n], P, I.ZEROTOL, I.upper_mode));
         }
         if (s > 0) {     // node below/left of the face
-            const int q = lnode<N1D>(d, line, s - 1);
+            const int q = lnode<N1D>(d, line, s - 1), n = e * NQ + q;
             const double sc_ = fac_dt * sm.geo[e][3] * rcp64(T.wq[q]);
-            for (int c = 0; c < 4; c++) P[c] = sc_ * sm.x.b.df[e][r][c];
-            const double* uL = sm.x.b.UL[e][q];
-            l = fmin(l, lim_param(uL, zeta * uL[0], sm.x.b.Lrhoe[e][q], P, I.ZEROTOL, I.upper_mode));
+            for (int c = 0; c < 4; c++) {
+                P[c] = sc_ * sm.x.b.df[c][w];
+                uL[c] = sm.x.b.UL[c][n];
+            }
+            l = fmin(l, lim_param(uL, zeta * uL[0], sm.x.b.Lrhoe[n], P, I.ZEROTOL, I.upper_mode));
         }
-        sm.x.b.l[e][r] = l;
+        sm.x.b.l[w] = l;
     }
     __syncthreads();
 
     // ---- 5c: limited RHS (accumulate_f_bar_limited!, apply_subcell_limiter!, subcell.jl:826-924),
     //      RK stage (SSPRK33.jl:30-39) and positivity of the new state
-    const size_t ib0 = (size_t)I.node_off * 4;
+    const size_t ob = ((size_t)I.node_off + (size_t)k0 * NQ) * 4;
 #pragma unroll 1
-    for (int w = tid; w < E * NQ; w += NT) {
-        const int e = w / NQ, i = w - e * NQ;
-        if (e >= nel) continue;
+    for (int n = tid; n < mN; n += NT) {
+        const int e = n / NQ, i = n - e * NQ;
         const int ia = i % N1D, ib = i / N1D;
-        const int rx = ib * (N1D + 1) + ia, ry = (N1D + ia) * (N1D + 1) + ib;
+        const int rx = e * NFB + ib * (N1D + 1) + ia, ry = e * NFB + (N1D + ia) * (N1D + 1) + ib;
         const double iwJ = sm.geo[e][3] * rcp64(T.wq[i]);
-        const double lx0 = sm.x.b.l[e][rx], lx1 = sm.x.b.l[e][rx + 1];
-        const double ly0 = sm.x.b.l[e][ry], ly1 = sm.x.b.l[e][ry + 1];
+        const double lx0 = sm.x.b.l[rx], lx1 = sm.x.b.l[rx + 1];
+        const double ly0 = sm.x.b.l[ry], ly1 = sm.x.b.l[ry + 1];
         double r[4];
         for (int c = 0; c < 4; c++) {
-            const double gx0 = fma(lx0, sm.x.b.df[e][rx][c], sm.x.b.fL[e][rx][c]);
-            const double gx1 = fma(lx1, sm.x.b.df[e][rx + 1][c], sm.x.b.fL[e][rx + 1][c]);
-            const double gy0 = fma(ly0, sm.x.b.df[e][ry][c], sm.x.b.fL[e][ry][c]);
-            const double gy1 = fma(ly1, sm.x.b.df[e][ry + 1][c], sm.x.b.fL[e][ry + 1][c]);
+            const double gx0 = fma(lx0, sm.x.b.df[c][rx], sm.x.b.fL[c][rx]);
+            const double gx1 = fma(lx1, sm.x.b.df[c][rx + 1], sm.x.b.fL[c][rx + 1]);
+            const double gy0 = fma(ly0, sm.x.b.df[c][ry], sm.x.b.fL[c][ry]);
+            const double gy1 = fma(ly1, sm.x.b.df[c][ry + 1], sm.x.b.fL[c][ry + 1]);
             r[c] = ((gx1 - gx0) + (gy1 - gy0)) * iwJ;
         }
-        const size_t o = ib0 + ((size_t)(k0 + e) * NQ + i) * 4;
+        const size_t o = ob + (size_t)n * 4;
         if (a.mode == MODE_RHS) {
-            for (int c = 0; c < 4; c++) a.rhsU[o + c] = r[c];
+            *(double4*)(a.rhsU + o) = make_double4(r[0], r[1], r[2], r[3]);
             continue;
         }
-        const double* u = sm.U[e][i];
         double un[4];
         if (a.mode == MODE_STAGE1) {
-            for (int c = 0; c < 4; c++) un[c] = fma(dt, r[c], u[c]);
+            for (int c = 0; c < 4; c++) un[c] = fma(dt, r[c], sm.U[c][n]);
         } else {
             const double wa = a.mode == MODE_STAGE2 ? 3.0 / 4.0 : 1.0 / 3.0;
             const double wb = a.mode == MODE_STAGE2 ? 1.0 / 4.0 : 2.0 / 3.0;
-            for (int c = 0; c < 4; c++) un[c] = fma(wa, a.W[o + c], wb * fma(dt, r[c], u[c]));
+            const double4 W = *(const double4*)(a.W + o);
+            const double Wc[4] = {W.x, W.y, W.z, W.w};
+            for (int c = 0; c < 4; c++) un[c] = fma(wa, Wc[c], wb * fma(dt, r[c], sm.U[c][n]));
         }
-        for (int c = 0; c < 4; c++) a.Uout[o + c] = un[c];
+        *(double4*)(a.Uout + o) = make_double4(un[0], un[1], un[2], un[3]);
         const double pr = (gd - 1.0) * (un[3] - 0.5 * (un[1] * un[1] + un[2] * un[2]) / un[0]);
         if (!(un[0] > 0.0) || !(pr > 0.0)) {
             atomicAdd(&sc->viol, 1ull);
-            atomicMin(&sc->first_bad, ((unsigned long long)a.mode << 40) |
-                                          (unsigned long long)((size_t)(k0 + e) * NQ + i));
+            atomicMin(&sc->first_bad, ((unsigned long long)a.mode << 40) | (unsigned long long)((size_t)k0 * NQ + n));
         }
     }
+}
+
+// A tile function as a call of its own, for kernels compiled for several degrees: each degree
+// then gets its own register allocation instead of the union of all of them.
+template <int N1D, int E, int NT>
+__device__ __noinline__ void stage_tile_call(const MixArgs& a, const MixInst& I, const MixTab<N1D>& T, int k0,
+                                             int nel, SmemM<N1D, E>& sm) {
+    stage_tile<N1D, E, NT>(a, I, T, k0, nel, sm);
 }
 
 // CFL time step of the low-order scheme (calculate_lambda_and_low_order_CFL!,
 // low_order_graph_viscosity.jl:222-285): per node CFL*wq*J/2 / (sum_j lambda_ij + sum_f lambda_B).
 template <int N1D, int E, int NT>
-__device__ __forceinline__ void dt_tile(const MixArgs& a, const MixInst& I, const MixTab<N1D>& T, int k0, int nel,
-                                        SmemDt<N1D, E>& sm) {
+__device__ __forceinline__ void dt_tile(const MixArgs& a, const MixInst& I, const MixTab<N1D>& T, int tile, int k0,
+                                        int nel, SmemDt<N1D, E>& sm) {
     constexpr int NQ = N1D * N1D, NFP = 4 * N1D, LW = N1D - 1;
     const int tid = threadIdx.x;
     const double gd = I.gamma;
@@ -533,11 +579,11 @@ __device__ __forceinline__ void dt_tile(const MixArgs& a, const MixInst& I, cons
     for (int w = tid; w < E * NQ; w += NT) {
         const int e = w / NQ, i = w - e * NQ;
         if (e >= nel) continue;
-        double U[4];
-        for (int c = 0; c < 4; c++) U[c] = Uin[((size_t)(k0 + e) * NQ + i) * 4 + c];
+        const double4 u = ((const double4*)Uin)[(size_t)(k0 + e) * NQ + i];
+        const double U[4] = {u.x, u.y, u.z, u.w};
         const NodeP P = node_prims(gd, U);
-        sm.ws[e][i][0] = P.wsx;
-        sm.ws[e][i][1] = P.wsy;
+        sm.ws[0][w] = P.wsx;
+        sm.ws[1][w] = P.wsy;
     }
     for (int w = tid; w < nel; w += NT) {
         const Geo G = I.geo[k0 + w];
@@ -554,12 +600,12 @@ __device__ __forceinline__ void dt_tile(const MixArgs& a, const MixInst& I, cons
         const int k = k0 + e;
         const int ia = i % N1D, ib = i / N1D;
         const float rx = (float)sm.geo[e][1], sy = (float)sm.geo[e][2];
-        const float wx = sm.ws[e][i][0], wy = sm.ws[e][i][1];
+        const float wx = sm.ws[0][w], wy = sm.ws[1][w];
         float lam = 0.f;
-        if (ia > 0) lam += fabsf(rx * T.cL[ib * LW + ia - 1]) * fmaxf(wx, sm.ws[e][i - 1][0]);
-        if (ia < N1D - 1) lam += fabsf(rx * T.cL[ib * LW + ia]) * fmaxf(wx, sm.ws[e][i + 1][0]);
-        if (ib > 0) lam += fabsf(sy * T.cL[N1D * LW + ia * LW + ib - 1]) * fmaxf(wy, sm.ws[e][i - N1D][1]);
-        if (ib < N1D - 1) lam += fabsf(sy * T.cL[N1D * LW + ia * LW + ib]) * fmaxf(wy, sm.ws[e][i + N1D][1]);
+        if (ia > 0) lam += fabsf(rx * T.cL[ib * LW + ia - 1]) * fmaxf(wx, sm.ws[0][w - 1]);
+        if (ia < N1D - 1) lam += fabsf(rx * T.cL[ib * LW + ia]) * fmaxf(wx, sm.ws[0][w + 1]);
+        if (ib > 0) lam += fabsf(sy * T.cL[N1D * LW + ia * LW + ib - 1]) * fmaxf(wy, sm.ws[1][w - N1D]);
+        if (ib < N1D - 1) lam += fabsf(sy * T.cL[N1D * LW + ia * LW + ib]) * fmaxf(wy, sm.ws[1][w + N1D]);
         const int fx = ia == 0 ? ib : ia == N1D - 1 ? N1D + ib : -1;
         const int fy = ib == 0 ? 2 * N1D + ia : ib == N1D - 1 ? 3 * N1D + ia : -1;
         for (int m = 0; m < 2; m++) {
@@ -584,7 +630,7 @@ __device__ __forceinline__ void dt_tile(const MixArgs& a, const MixInst& I, cons
     if (tid == 0) {
         double m = sm.red[0];
         for (int x = 1; x < NT / 64; x++) m = sm.red[x] < m ? sm.red[x] : m;
-        if (m < INFINITY) atomicMin(&I.sc->dt_bits, d_to_bits(m));
+        a.dtp[tile] = m;
     }
 }
 
@@ -606,7 +652,7 @@ constexpr int tab_bytes() {
     return E > 0 ? (int)sizeof(MixTab<N1D>) : 0;
 }
 
-// Copy the tables of the compiled degrees to LDS (at fixed offsets), then drain the queue.
+// LDS offsets of the tables of the compiled degrees and the size of the tile scratch.
 template <int E2, int E3, int E4, int E5>
 struct Layout {
     static constexpr int T2 = 0;
@@ -630,45 +676,35 @@ __device__ __forceinline__ void load_tab(const MixArgs& a, unsigned char* dst) {
     for (int w = threadIdx.x; w < n; w += NT) d[w] = src[w];
 }
 
-// next tile of this workgroup's queue chunk, then of the other chunks; -1 when all are drained
-__device__ __forceinline__ int next_tile(const MixArgs& a, int& xo, int* s_tile) {
-    if (threadIdx.x == 0) {
-        int t = -1;
-        const int x0 = blockIdx.x & (MIX_CHUNKS - 1);
-        while (xo < MIX_CHUNKS) {
-            const int c = (x0 + xo) & (MIX_CHUNKS - 1);
-            const int lo = (int)((int64_t)a.ntiles * c / MIX_CHUNKS);
-            const int hi = (int)((int64_t)a.ntiles * (c + 1) / MIX_CHUNKS);
-            const int v = atomicAdd(&a.queue[c], 1);
-            if (lo + v < hi) {
-                t = lo + v;
-                break;
-            }
-            xo++;
-        }
-        *s_tile = t;
+// This workgroup's tiles: t0, t0 + step, ... < hi within the chunk of its XCD.
+__device__ __forceinline__ void tile_range(int ntiles, int& t0, int& hi, int& step) {
+    const int G = gridDim.x, b = blockIdx.x;
+    if (G < MIX_CHUNKS) {
+        t0 = b;
+        hi = ntiles;
+        step = G;
+        return;
     }
-    __syncthreads();
-    const int t = __builtin_amdgcn_readfirstlane(*s_tile);
-    __syncthreads();
-    return t;
+    const int c = b % MIX_CHUNKS, r = b / MIX_CHUNKS;
+    t0 = (int)((int64_t)ntiles * c / MIX_CHUNKS) + r;
+    hi = (int)((int64_t)ntiles * (c + 1) / MIX_CHUNKS);
+    step = G / MIX_CHUNKS + (c < G % MIX_CHUNKS ? 1 : 0);
 }
 
 template <int NT, int E2, int E3, int E4, int E5>
 __global__ __launch_bounds__(NT) void mix_stage_kernel(MixArgs a) {
     using LY = Layout<E2, E3, E4, E5>;
+    constexpr bool multi = (E2 > 0) + (E3 > 0) + (E4 > 0) + (E5 > 0) > 1;
     __shared__ __align__(16) unsigned char tabs[LY::TEND > 0 ? LY::TEND : 16];
     __shared__ __align__(16) unsigned char raw[LY::SMAX];
-    __shared__ int s_tile;
     load_tab<2, E2, NT>(a, tabs + LY::T2);
     load_tab<3, E3, NT>(a, tabs + LY::T3);
     load_tab<4, E4, NT>(a, tabs + LY::T4);
     load_tab<5, E5, NT>(a, tabs + LY::T5);
     __syncthreads();
-    int xo = 0;
-    for (;;) {
-        const int t = next_tile(a, xo, &s_tile);
-        if (t < 0) break;
+    int t0, hi, step;
+    tile_range(a.ntiles, t0, hi, step);
+    for (int t = t0; t < hi; t += step) {
         const MixTile tl = a.tiles[t];
         if (a.only >= 0 && tl.inst != a.only) continue;
         const MixInst& I = a.inst[tl.inst];
@@ -676,25 +712,42 @@ __global__ __launch_bounds__(NT) void mix_stage_kernel(MixArgs a) {
         switch (tl.n1d) {
             case 2:
                 if constexpr (E2 > 0)
-                    stage_tile<2, E2, NT>(a, I, *(const MixTab<2>*)(tabs + LY::T2), tl.k0, tl.nel,
-                                          *(SmemM<2, (E2 > 0 ? E2 : 1)>*)raw);
+                {
+                    auto& sm = *(SmemM<2, (E2 > 0 ? E2 : 1)>*)raw;
+                    const auto& T = *(const MixTab<2>*)(tabs + LY::T2);
+                    if constexpr (multi) stage_tile_call<2, E2, NT>(a, I, T, tl.k0, tl.nel, sm);
+                    else stage_tile<2, E2, NT>(a, I, T, tl.k0, tl.nel, sm);
+                }
                 break;
             case 3:
                 if constexpr (E3 > 0)
-                    stage_tile<3, E3, NT>(a, I, *(const MixTab<3>*)(tabs + LY::T3), tl.k0, tl.nel,
-                                          *(SmemM<3, (E3 > 0 ? E3 : 1)>*)raw);
+                {
+                    auto& sm = *(SmemM<3, (E3 > 0 ? E3 : 1)>*)raw;
+                    const auto& T = *(const MixTab<3>*)(tabs + LY::T3);
+                    if constexpr (multi) stage_tile_call<3, E3, NT>(a, I, T, tl.k0, tl.nel, sm);
+                    else stage_tile<3, E3, NT>(a, I, T, tl.k0, tl.nel, sm);
+                }
                 break;
             case 4:
                 if constexpr (E4 > 0)
-                    stage_tile<4, E4, NT>(a, I, *(const MixTab<4>*)(tabs + LY::T4), tl.k0, tl.nel,
-                                          *(SmemM<4, (E4 > 0 ? E4 : 1)>*)raw);
+                {
+                    auto& sm = *(SmemM<4, (E4 > 0 ? E4 : 1)>*)raw;
+                    const auto& T = *(const MixTab<4>*)(tabs + LY::T4);
+                    if constexpr (multi) stage_tile_call<4, E4, NT>(a, I, T, tl.k0, tl.nel, sm);
+                    else stage_tile<4, E4, NT>(a, I, T, tl.k0, tl.nel, sm);
+                }
                 break;
             case 5:
                 if constexpr (E5 > 0)
-                    stage_tile<5, E5, NT>(a, I, *(const MixTab<5>*)(tabs + LY::T5), tl.k0, tl.nel,
-                                          *(SmemM<5, (E5 > 0 ? E5 : 1)>*)raw);
+                {
+                    auto& sm = *(SmemM<5, (E5 > 0 ? E5 : 1)>*)raw;
+                    const auto& T = *(const MixTab<5>*)(tabs + LY::T5);
+                    if constexpr (multi) stage_tile_call<5, E5, NT>(a, I, T, tl.k0, tl.nel, sm);
+                    else stage_tile<5, E5, NT>(a, I, T, tl.k0, tl.nel, sm);
+                }
                 break;
         }
+        __syncthreads();   // the next tile reuses the LDS
     }
 }
 
@@ -703,16 +756,14 @@ __global__ __launch_bounds__(NT) void mix_dt_kernel(MixArgs a) {
     using LY = Layout<E2, E3, E4, E5>;
     __shared__ __align__(16) unsigned char tabs[LY::TEND > 0 ? LY::TEND : 16];
     __shared__ __align__(16) unsigned char raw[LY::DMAX];
-    __shared__ int s_tile;
     load_tab<2, E2, NT>(a, tabs + LY::T2);
     load_tab<3, E3, NT>(a, tabs + LY::T3);
     load_tab<4, E4, NT>(a, tabs + LY::T4);
     load_tab<5, E5, NT>(a, tabs + LY::T5);
     __syncthreads();
-    int xo = 0;
-    for (;;) {
-        const int t = next_tile(a, xo, &s_tile);
-        if (t < 0) break;
+    int t0, hi, step;
+    tile_range(a.ntiles, t0, hi, step);
+    for (int t = t0; t < hi; t += step) {
         const MixTile tl = a.tiles[t];
         if (a.only >= 0 && tl.inst != a.only) continue;
         const MixInst& I = a.inst[tl.inst];
@@ -720,22 +771,22 @@ __global__ __launch_bounds__(NT) void mix_dt_kernel(MixArgs a) {
         switch (tl.n1d) {
             case 2:
                 if constexpr (E2 > 0)
-                    dt_tile<2, E2, NT>(a, I, *(const MixTab<2>*)(tabs + LY::T2), tl.k0, tl.nel,
+                    dt_tile<2, E2, NT>(a, I, *(const MixTab<2>*)(tabs + LY::T2), t, tl.k0, tl.nel,
                                        *(SmemDt<2, (E2 > 0 ? E2 : 1)>*)raw);
                 break;
             case 3:
                 if constexpr (E3 > 0)
-                    dt_tile<3, E3, NT>(a, I, *(const MixTab<3>*)(tabs + LY::T3), tl.k0, tl.nel,
+                    dt_tile<3, E3, NT>(a, I, *(const MixTab<3>*)(tabs + LY::T3), t, tl.k0, tl.nel,
                                        *(SmemDt<3, (E3 > 0 ? E3 : 1)>*)raw);
                 break;
             case 4:
                 if constexpr (E4 > 0)
-                    dt_tile<4, E4, NT>(a, I, *(const MixTab<4>*)(tabs + LY::T4), tl.k0, tl.nel,
+                    dt_tile<4, E4, NT>(a, I, *(const MixTab<4>*)(tabs + LY::T4), t, tl.k0, tl.nel,
                                        *(SmemDt<4, (E4 > 0 ? E4 : 1)>*)raw);
                 break;
             case 5:
                 if constexpr (E5 > 0)
-                    dt_tile<5, E5, NT>(a, I, *(const MixTab<5>*)(tabs + LY::T5), tl.k0, tl.nel,
+                    dt_tile<5, E5, NT>(a, I, *(const MixTab<5>*)(tabs + LY::T5), t, tl.k0, tl.nel,
                                        *(SmemDt<5, (E5 > 0 ? E5 : 1)>*)raw);
                 break;
         }
@@ -743,11 +794,9 @@ __global__ __launch_bounds__(NT) void mix_dt_kernel(MixArgs a) {
     }
 }
 
-// per instance: step start (SSPRK33.jl:30) or RHS-evaluation set-up; queue counters to zero
-__global__ void mix_init_kernel(const MixInst* inst, int n, int* queue, int nqueue, int mode, int only,
-                                double dt_lim, double dt_cap) {
+// per instance: step start (SSPRK33.jl:30) or RHS-evaluation set-up
+__global__ void mix_init_kernel(const MixInst* inst, int n, int mode, int only, double dt_lim, double dt_cap) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g < nqueue) queue[g] = 0;
     if (g >= n) return;
     StepScalars* sc = inst[g].sc;
     if (mode == 0) {
@@ -764,6 +813,31 @@ __global__ void mix_init_kernel(const MixInst* inst, int n, int* queue, int nque
         sc->done = 0;
         sc->dt_lim = dt_lim;
         sc->dt_bits = d_to_bits(dt_cap);
+    }
+}
+
+// dt = min(cap, min over the instance's tiles) (the global minimum of
+// calculate_lambda_and_low_order_CFL!, low_order_graph_viscosity.jl:230-243); one workgroup
+// per instance
+__global__ __launch_bounds__(256) void mix_dtred_kernel(const MixInst* inst, int n, const double* dtp, int only) {
+    __shared__ double red[4];
+    const int i = blockIdx.x;
+    if (i >= n || (only >= 0 && i != only)) return;
+    const MixInst& I = inst[i];
+    if (I.sc->done) return;
+    double m = INFINITY;
+    for (int t = I.tile_lo + (int)threadIdx.x; t < I.tile_hi; t += 256) m = dtp[t] < m ? dtp[t] : m;
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(m, off, 64);
+        m = o < m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int x = 1; x < 4; x++) m = red[x] < m ? red[x] : m;
+        m = red[0] < m ? red[0] : m;
+        const double cap = bits_to_d(I.sc->dt_bits);
+        I.sc->dt_bits = d_to_bits(m < cap ? m : cap);
     }
 }
 
@@ -860,11 +934,14 @@ int mix_launch_dt(int v, const MixArgs& a, int grid, void* stream) {
     return (int)hipLaunchKernel(kMix[v].dt, dim3(grid), dim3(kMix[v].NT), params, 0, (hipStream_t)stream);
 }
 
-int mix_launch_init(const MixInst* inst, int n, int* queue, int nqueue, int mode, int only, double dt_lim,
-                    double dt_cap, void* stream) {
-    const int m = n > nqueue ? n : nqueue;
-    hipLaunchKernelGGL(mix_init_kernel, dim3((m + 255) / 256), dim3(256), 0, (hipStream_t)stream, inst, n, queue,
-                       nqueue, mode, only, dt_lim, dt_cap);
+int mix_launch_init(const MixInst* inst, int n, int mode, int only, double dt_lim, double dt_cap, void* stream) {
+    hipLaunchKernelGGL(mix_init_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, inst, n, mode, only,
+                       dt_lim, dt_cap);
+    return (int)hipGetLastError();
+}
+
+int mix_launch_dt_reduce(const MixInst* inst, int n, const double* dtp, int only, void* stream) {
+    hipLaunchKernelGGL(mix_dtred_kernel, dim3(n), dim3(256), 0, (hipStream_t)stream, inst, n, dtp, only);
     return (int)hipGetLastError();
 }
 

@@ -49,7 +49,8 @@ struct p2de_batch {
     MixInst* d_inst = nullptr;
     MixTile* d_tiles = nullptr;
     int ntiles = 0;
-    int* d_queue = nullptr;
+    double* d_dtp = nullptr;                   // per-tile CFL minima
+    std::vector<MixInst> hinst;                // host copy of d_inst
     void* d_tab[MIX_MAX_N1D + 1] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     Geo* d_geo = nullptr;
     int *d_nb_node = nullptr, *d_bc = nullptr;
@@ -123,7 +124,7 @@ void free_batch(p2de_batch* b) {
         (void)hipEventDestroy(l.b);
     }
     for (auto e : b->event_pool) (void)hipEventDestroy(e);
-    void* dptrs[] = {b->d_inst, b->d_tiles, b->d_queue, b->d_geo, b->d_nb_node, b->d_bc, b->d_Ival,
+    void* dptrs[] = {b->d_inst, b->d_tiles, b->d_dtp, b->d_geo, b->d_nb_node, b->d_bc, b->d_Ival,
                      b->d_U[0], b->d_U[1], b->d_U[2], b->d_rhsU, b->d_dbg_rhsH, b->d_dbg_rhsL, b->d_sc, b->d_keys};
     for (void* p : dptrs)
         if (p) (void)hipFree(p);
@@ -253,10 +254,13 @@ int build_tiles(p2de_batch* b) {
     for (int i = 0; i < b->n; i++) {
         const BInst& I = b->inst[i];
         const int e = E[I.N1D];
+        b->hinst[i].tile_lo = (int)tiles.size();
         for (int k0 = 0; k0 < I.K; k0 += e) tiles.push_back(MixTile{i, k0, std::min(e, I.K - k0), I.N1D});
+        b->hinst[i].tile_hi = (int)tiles.size();
     }
     b->ntiles = (int)tiles.size();
     BHIP(b, hipMemcpyAsync(b->d_tiles, tiles.data(), sizeof(MixTile) * tiles.size(), hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipMemcpyAsync(b->d_inst, b->hinst.data(), sizeof(MixInst) * b->n, hipMemcpyHostToDevice, b->stream));
     BHIP(b, hipStreamSynchronize(b->stream));
     int sb = 0, db = 0;
     BLAUNCH(b, mix_blocks_per_cu(b->variants[b->vsel], &sb, &db));
@@ -303,12 +307,12 @@ void harvest(p2de_batch* b) {
     b->launches.clear();
 }
 
-MixArgs margs(p2de_batch* b, int slot, int mode, int only, const double* Uin, const double* W, double* Uout) {
+MixArgs margs(p2de_batch* b, int mode, int only, const double* Uin, const double* W, double* Uout) {
     MixArgs a{};
     a.inst = b->d_inst;
     a.tiles = b->d_tiles;
     a.ntiles = b->ntiles;
-    a.queue = b->d_queue + slot * MIX_CHUNKS;
+    a.dtp = b->d_dtp;
     for (int n = 0; n <= MIX_MAX_N1D; n++) a.tab[n] = b->d_tab[n];
     a.mode = mode;
     a.only = only;
@@ -328,12 +332,12 @@ int launch_step(p2de_batch* b) {
     void* s = b->stream;
     const int v = b->variants[b->vsel];
     double *A = b->d_U[0], *B = b->d_U[1], *C = b->d_U[2];
-    const int nq = MIX_SLOTS * MIX_CHUNKS;
-    BLAUNCH(b, timed(b, 2, [&] { return mix_launch_init(b->d_inst, b->n, b->d_queue, nq, 0, -1, 0.0, 0.0, s); }));
-    BLAUNCH(b, timed(b, 1, [&] { return mix_launch_dt(v, margs(b, 0, MODE_STAGE1, -1, A, A, nullptr), b->grid_dt, s); }));
-    BLAUNCH(b, timed(b, 0, [&] { return mix_launch_stage(v, margs(b, 1, MODE_STAGE1, -1, A, A, B), b->grid_stage, s); }));
-    BLAUNCH(b, timed(b, 0, [&] { return mix_launch_stage(v, margs(b, 2, MODE_STAGE2, -1, B, A, C), b->grid_stage, s); }));
-    BLAUNCH(b, timed(b, 0, [&] { return mix_launch_stage(v, margs(b, 3, MODE_STAGE3, -1, C, A, A), b->grid_stage, s); }));
+    BLAUNCH(b, timed(b, 2, [&] { return mix_launch_init(b->d_inst, b->n, 0, -1, 0.0, 0.0, s); }));
+    BLAUNCH(b, timed(b, 1, [&] { return mix_launch_dt(v, margs(b, MODE_STAGE1, -1, A, A, nullptr), b->grid_dt, s); }));
+    BLAUNCH(b, timed(b, 2, [&] { return mix_launch_dt_reduce(b->d_inst, b->n, b->d_dtp, -1, s); }));
+    BLAUNCH(b, timed(b, 0, [&] { return mix_launch_stage(v, margs(b, MODE_STAGE1, -1, A, A, B), b->grid_stage, s); }));
+    BLAUNCH(b, timed(b, 0, [&] { return mix_launch_stage(v, margs(b, MODE_STAGE2, -1, B, A, C), b->grid_stage, s); }));
+    BLAUNCH(b, timed(b, 0, [&] { return mix_launch_stage(v, margs(b, MODE_STAGE3, -1, C, A, A), b->grid_stage, s); }));
     BLAUNCH(b, timed(b, 2, [&] { return mix_launch_post(b->d_inst, b->n, s); }));
     return P2DE_OK;
 }
@@ -483,7 +487,7 @@ int p2de_batch_create(int n, const p2de_config* cfg, const p2de_operators* ops, 
         int st;
         const size_t nU = (size_t)b->total_nodes * 4;
         if ((st = balloc(b, &b->d_inst, (size_t)n)) || (st = balloc(b, &b->d_tiles, (size_t)b->total_elems)) ||
-            (st = balloc(b, &b->d_queue, (size_t)MIX_SLOTS * MIX_CHUNKS)) || (st = balloc(b, &b->d_geo, (size_t)b->total_elems)) ||
+            (st = balloc(b, &b->d_dtp, (size_t)b->total_elems)) || (st = balloc(b, &b->d_geo, (size_t)b->total_elems)) ||
             (st = balloc(b, &b->d_nb_node, (size_t)b->total_faces)) || (st = balloc(b, &b->d_bc, (size_t)b->total_faces)) ||
             (st = balloc(b, &b->d_Ival, (size_t)std::max<int64_t>(ival_rows, 1) * 4)) || (st = balloc(b, &b->d_U[0], nU)) ||
             (st = balloc(b, &b->d_U[1], nU)) || (st = balloc(b, &b->d_U[2], nU)) || (st = balloc(b, &b->d_rhsU, nU)) ||
@@ -492,7 +496,8 @@ int p2de_batch_create(int n, const p2de_config* cfg, const p2de_operators* ops, 
         if (hipHostMalloc((void**)&b->h_sc, sizeof(StepScalars) * n) != hipSuccess ||
             hipHostMalloc((void**)&b->h_keys, 3 * sizeof(unsigned long long)) != hipSuccess)
             return bfail(b, P2DE_E_ALLOC, "hipHostMalloc failed"), bail(P2DE_E_ALLOC);
-        std::vector<MixInst> hinst(n);
+        std::vector<MixInst>& hinst = b->hinst;
+        hinst.assign(n, MixInst{});
         for (int i = 0; i < n; i++) {
             BInst& I = b->inst[i];
             if ((st = upload_tab(b, I.N1D, tabs[i], &ops[i]))) return bail(st);
@@ -534,8 +539,7 @@ int p2de_batch_create(int n, const p2de_config* cfg, const p2de_operators* ops, 
         }
         hipError_t e;
         if ((e = hipMemcpy(b->d_inst, hinst.data(), sizeof(MixInst) * n, hipMemcpyHostToDevice)) != hipSuccess ||
-            (e = hipMemcpy(b->d_sc, b->h_sc, sizeof(StepScalars) * n, hipMemcpyHostToDevice)) != hipSuccess ||
-            (e = hipMemset(b->d_queue, 0, sizeof(int) * MIX_SLOTS * MIX_CHUNKS)) != hipSuccess)
+            (e = hipMemcpy(b->d_sc, b->h_sc, sizeof(StepScalars) * n, hipMemcpyHostToDevice)) != hipSuccess)
             return bfail(b, P2DE_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e)), bail(P2DE_E_HIP);
         b->vsel = 0;
         if ((st = build_tiles(b))) return bail(st);
@@ -638,12 +642,12 @@ int p2de_batch_rhs(p2de_batch* b, int inst, double t, double dt, int nstage, dou
     void* s = b->stream;
     const int v = b->variants[b->vsel];
     double* A = b->d_U[0];
-    BLAUNCH(b, timed(b, 2, [&] {
-                return mix_launch_init(b->d_inst, b->n, b->d_queue, MIX_SLOTS * MIX_CHUNKS, 1, inst, dt, cap, s);
-            }));
-    if (nstage == 1)
-        BLAUNCH(b, timed(b, 1, [&] { return mix_launch_dt(v, margs(b, 0, MODE_RHS, inst, A, A, nullptr), b->grid_dt, s); }));
-    BLAUNCH(b, timed(b, 0, [&] { return mix_launch_stage(v, margs(b, 4, MODE_RHS, inst, A, A, nullptr), b->grid_stage, s); }));
+    BLAUNCH(b, timed(b, 2, [&] { return mix_launch_init(b->d_inst, b->n, 1, inst, dt, cap, s); }));
+    if (nstage == 1) {
+        BLAUNCH(b, timed(b, 1, [&] { return mix_launch_dt(v, margs(b, MODE_RHS, inst, A, A, nullptr), b->grid_dt, s); }));
+        BLAUNCH(b, timed(b, 2, [&] { return mix_launch_dt_reduce(b->d_inst, b->n, b->d_dtp, inst, s); }));
+    }
+    BLAUNCH(b, timed(b, 0, [&] { return mix_launch_stage(v, margs(b, MODE_RHS, inst, A, A, nullptr), b->grid_stage, s); }));
     if (rhsU)
         BHIP(b, hipMemcpyAsync(rhsU, b->d_rhsU + I.node_off * 4, sizeof(double) * (size_t)I.K * I.Nq * 4,
                                hipMemcpyDeviceToHost, b->stream));

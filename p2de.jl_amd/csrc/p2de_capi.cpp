@@ -43,6 +43,7 @@ struct p2de_ctx {
     double* d_U[2] = {nullptr, nullptr};
     double *d_rhsUA = nullptr, *d_lA = nullptr, *d_dfA = nullptr, *d_rhsU = nullptr;
     double* d_ut = nullptr;                // projected face states of d_U[0]
+    double* d_dtp = nullptr;               // per-workgroup CFL minima of phase A (stage 1)
     bool ut_valid = false;                 // d_ut matches d_U[0]
     double *d_dbg_rhsH = nullptr, *d_dbg_rhsL = nullptr, *d_dbg_L = nullptr;
     StepScalars* d_sc = nullptr;
@@ -100,7 +101,7 @@ void free_ctx(p2de_ctx* c) {
     }
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
     void* dptrs[] = {c->d_tab, c->d_geo, c->d_nb_node, c->d_nb_face, c->d_bc, c->d_Ival, c->d_U[0], c->d_U[1],
-                     c->d_rhsUA, c->d_lA, c->d_dfA, c->d_rhsU, c->d_ut, c->d_dbg_rhsH, c->d_dbg_rhsL, c->d_dbg_L,
+                     c->d_rhsUA, c->d_lA, c->d_dfA, c->d_rhsU, c->d_ut, c->d_dtp, c->d_dbg_rhsH, c->d_dbg_rhsL, c->d_dbg_L,
                      c->d_sc, c->d_keys};
     for (void* p : dptrs)
         if (p) (void)hipFree(p);
@@ -218,6 +219,7 @@ int p2de_create(const p2de_config* cfg, const p2de_operators* ops, const p2de_ge
             (st = dalloc(c, &c->d_U[0], nU)) || (st = dalloc(c, &c->d_U[1], nU)) || (st = dalloc(c, &c->d_rhsUA, nU)) ||
             (st = dalloc(c, &c->d_lA, (size_t)K * Nfp)) || (st = dalloc(c, &c->d_dfA, (size_t)K * Nfp * Nc)) ||
             (st = dalloc(c, &c->d_rhsU, nU)) || (st = dalloc(c, &c->d_ut, (size_t)K * Nfp * Nc)) ||
+            (st = dalloc(c, &c->d_dtp, (size_t)K)) ||
             (st = dalloc(c, &c->d_sc, 1)) || (st = dalloc(c, &c->d_keys, 3)))
             return bail(st);
         if (hipHostMalloc((void**)&c->h_sc, sizeof(StepScalars)) != hipSuccess ||
@@ -313,6 +315,7 @@ PhaseAArgs argsA(p2de_ctx* c, const double* U, int stage1) {
     a.rhsUA = c->d_rhsUA;
     a.lA = c->d_lA;
     a.dfA = c->d_dfA;
+    a.dtp = c->d_dtp;
     if (c->debug) {
         a.dbg_rhsH = c->d_dbg_rhsH;
         a.dbg_rhsL = c->d_dbg_rhsL;
@@ -363,6 +366,9 @@ int enqueue_step(p2de_ctx* c) {
     LAUNCHCHK(c, timed(c, 2, [&] { return launch_step_init(c->d_sc, g.CFL, g.dt0, g.T, s); }));
     // stage 1: U1 = W + dt*rhsU (SSPRK33.jl:32-33)
     LAUNCHCHK(c, timed(c, 0, [&] { return launch_phaseA(c->Nd, c->N1D, c->variantA, argsA(c, A, 1), s); }));
+    LAUNCHCHK(c, timed(c, 2, [&] {
+                  return launch_dt_reduce(c->d_sc, c->d_dtp, phaseA_blocks(c->Nd, c->N1D, c->variantA, c->K), s);
+              }));
     LAUNCHCHK(c, timed(c, 1, [&] { return launch_phaseB(c->Nd, c->N1D, argsB(c, MODE_STAGE1, A, A, B, 0), s); }));
     // stage 2: U2 = 3/4 W + 1/4 (U1 + dt*rhsU) (SSPRK33.jl:34-36)
     LAUNCHCHK(c, timed(c, 0, [&] { return launch_phaseA(c->Nd, c->N1D, c->variantA, argsA(c, B, 0), s); }));
@@ -408,6 +414,10 @@ int p2de_rhs(p2de_ctx* ctx, double t, double dt, int nstage, double* dt_out, dou
     int st0 = ensure_face_states(ctx);
     if (st0) return st0;
     LAUNCHCHK(ctx, timed(ctx, 0, [&] { return launch_phaseA(ctx->Nd, ctx->N1D, ctx->variantA, argsA(ctx, ctx->d_U[0], nstage == 1), s); }));
+    if (nstage == 1)
+        LAUNCHCHK(ctx, timed(ctx, 2, [&] {
+                      return launch_dt_reduce(ctx->d_sc, ctx->d_dtp, phaseA_blocks(ctx->Nd, ctx->N1D, ctx->variantA, ctx->K), s);
+                  }));
     LAUNCHCHK(ctx, timed(ctx, 1, [&] {
                   return launch_phaseB(ctx->Nd, ctx->N1D, argsB(ctx, MODE_RHS, ctx->d_U[0], ctx->d_U[0], nullptr, 0), s);
               }));

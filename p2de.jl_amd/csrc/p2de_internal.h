@@ -103,6 +103,7 @@ struct PhaseAArgs {
     double* dbg_rhsH;             // optional debug outputs (nullptr: off)
     double* dbg_rhsL;
     double* dbg_L;                // L_local[:, :, :] of this stage, (Nq+N1D, Nd, K)
+    double* dtp;                  // stage 1: per-workgroup minimum of the CFL time step
 };
 
 enum { MODE_RHS = 0, MODE_STAGE1 = 1, MODE_STAGE2 = 2, MODE_STAGE3 = 3 };
@@ -136,6 +137,9 @@ int launch_face_state(int Nd, int N1D, const double* U, const ElemTables* T, int
                       void* stream);
 int launch_step_init(StepScalars* sc, double CFL, double dt0, double T, void* stream);
 int launch_rhs_init(StepScalars* sc, double dt_lim, double dt_cap, void* stream);
+// dt = min(cap, min_b dtp[b]) over the phase-A workgroups (no global atomics)
+int launch_dt_reduce(StepScalars* sc, const double* dtp, int n, void* stream);
+int phaseA_blocks(int Nd, int N1D, int variant, int K);
 int launch_positivity(const double* U, int64_t nodes, int Nd, double gamma, unsigned long long* keys,
                       unsigned long long* cnt, void* stream);
 

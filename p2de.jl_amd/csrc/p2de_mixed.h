@@ -6,8 +6,8 @@
 // assembly run in FP32; node pressures (the E - |m|^2/(2 rho) cancellation), the subcell
 // flux potentials, the limiter's bounds and coefficients and the RK stages run in FP64, and
 // the state stays FP64 in HBM.  One launch per RK stage covers every element of every
-// instance (a persistent workgroup pool draining a tile queue), plus one CFL-time-step
-// launch per step.
+// instance (a persistent workgroup pool striding over the tile list), plus one CFL-time-step
+// launch and its per-instance reduction per step.
 //
 // Device layout (HBM), all instances concatenated:
 //   state      double[sum_i K_i*Nq_i][4]    instance i starts at node node_off[i]
@@ -24,8 +24,7 @@ namespace p2de {
 
 constexpr int MIX_MIN_N1D = 2;
 constexpr int MIX_MAX_N1D = 5;
-constexpr int MIX_CHUNKS = 8;        // tile-queue chunks, one per XCD of the MI355X
-constexpr int MIX_SLOTS = 5;         // queue slots: CFL pass, three stages, RHS evaluation
+constexpr int MIX_CHUNKS = 8;        // contiguous tile chunks, one per XCD of the MI355X
 
 // Compact operator tables of one degree for LGL tensor-product quads.
 //
@@ -58,7 +57,9 @@ struct MixInst {
     const double* Ival;
     StepScalars* sc;
     double gamma, zeta, ZEROTOL, CFL, dt0, T;
-    int K, N1D, upper_mode, pad;
+    int K, N1D, upper_mode;
+    int tile_lo, tile_hi;     // the instance's tiles in the tile list
+    int pad;
 };
 
 struct MixTile {
@@ -69,7 +70,7 @@ struct MixArgs {
     const MixInst* inst;
     const MixTile* tiles;
     int ntiles;
-    int* queue;           // MIX_CHUNKS counters of this launch (zeroed by the init kernel)
+    double* dtp;          // per-tile minimum of the CFL time step (CFL pass)
     const void* tab[MIX_MAX_N1D + 1];   // MixTab<N1D> by N1D
     int mode;             // MODE_RHS, MODE_STAGE1..3
     int only;             // evaluate only this instance (-1: all)
@@ -88,8 +89,8 @@ int mix_variant_shape(int v, int* NT, int E[MIX_MAX_N1D + 1]);
 int mix_blocks_per_cu(int v, int* stage_blocks, int* dt_blocks);
 int mix_launch_stage(int v, const MixArgs& a, int grid, void* stream);
 int mix_launch_dt(int v, const MixArgs& a, int grid, void* stream);
-int mix_launch_init(const MixInst* inst, int n, int* queue, int nqueue, int mode, int only, double dt_lim,
-                    double dt_cap, void* stream);
+int mix_launch_init(const MixInst* inst, int n, int mode, int only, double dt_lim, double dt_cap, void* stream);
+int mix_launch_dt_reduce(const MixInst* inst, int n, const double* dtp, int only, void* stream);
 int mix_launch_post(const MixInst* inst, int n, void* stream);
 int mix_launch_positivity(const double* U, int64_t nodes, double gamma, unsigned long long* keys,
                           unsigned long long* cnt, void* stream);

@@ -6,8 +6,8 @@
 //     low-order graph-viscosity RHS, high-order flux-differencing RHS, subcell flux
 //     potentials and the limiter parameters of every subcell face with the element's own
 //     data; writes the element RHS plus, for each element-boundary subcell face, its
-//     parameter and f_bar_H - f_bar_L; at stage 1 it also reduces the CFL time step
-//     (global atomic min).
+//     parameter and f_bar_H - f_bar_L; at stage 1 it also writes its workgroup's minimum of
+//     the CFL time step, which dt_reduce_kernel folds into the global minimum.
 //   phase B  (node-parallel) symmetrises the element-boundary parameters with the
 //     neighbour (min), corrects the adjacent nodes' RHS, applies the RK stage, checks
 //     positivity of the new state and forms its projected face states u(v(u)) once per
@@ -447,7 +447,7 @@ __global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
     if (a.stage1 && tid == 0) {
         double m = sm.red[0];
         for (int x = 1; x < NT / 64; x++) m = sm.red[x] < m ? sm.red[x] : m;
-        atomicMin(&a.sc->dt_bits, double_to_bits(m));
+        a.dtp[blockIdx.x] = m;   // reduced by dt_reduce_kernel (one-address atomics serialise across XCDs)
     }
 
     // ---- phase 5a: subcell flux potentials along every line (accumulate_f_bar! subcell.jl:163-206)
@@ -691,6 +691,26 @@ __global__ void rhs_init_kernel(StepScalars* sc, double dt_lim, double dt_cap) {
     sc->dt_bits = double_to_bits(dt_cap);
 }
 
+// dt = min(cap, min over the phase-A workgroups' minima); a NaN minimum is ignored, as the
+// bit-pattern minimum over positive doubles ignores it
+__global__ __launch_bounds__(1024) void dt_reduce_kernel(StepScalars* sc, const double* dtp, int n) {
+    __shared__ double red[16];
+    if (sc->done) return;
+    double m = INFINITY;
+    for (int b = (int)threadIdx.x; b < n; b += 1024) m = dtp[b] < m ? dtp[b] : m;
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(m, off, 64);
+        m = o < m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int x = 0; x < 16; x++) m = red[x] < m ? red[x] : m;
+        const double cap = bits_to_double(sc->dt_bits);
+        sc->dt_bits = double_to_bits(m < cap ? m : cap);
+    }
+}
+
 // order-preserving map of a double to an unsigned key (for atomicMin)
 __device__ __forceinline__ unsigned long long order_key(double v) {
     unsigned long long b = (unsigned long long)__double_as_longlong(v);
@@ -848,6 +868,18 @@ int launch_step_init(StepScalars* sc, double CFL, double dt0, double T, void* st
 int launch_rhs_init(StepScalars* sc, double dt_lim, double dt_cap, void* stream) {
     hipLaunchKernelGGL(rhs_init_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, sc, dt_lim, dt_cap);
     return (int)hipGetLastError();
+}
+
+int launch_dt_reduce(StepScalars* sc, const double* dtp, int n, void* stream) {
+    hipLaunchKernelGGL(dt_reduce_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, sc, dtp, n);
+    return (int)hipGetLastError();
+}
+
+int phaseA_blocks(int Nd, int N1D, int variant, int K) {
+    int n = 0;
+    const VariantA* t = variants(Nd, N1D, &n);
+    if (!t || variant < 0 || variant >= n) return -1;
+    return (K + t[variant].E - 1) / t[variant].E;
 }
 
 int launch_positivity(const double* U, int64_t nodes, int Nd, double gamma, unsigned long long* keys,
