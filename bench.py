@@ -1,24 +1,32 @@
-"""Benchmark: DOF-stage updates per second of the SSP-RK3 limited-ESDG step on one MI355X.
+"""Benchmark: DOF-stage updates per second of the limited-ESDG SSP-RK3 step on one MI355X.
 
-Workload (default): the 2D isentropic vortex on periodic [-5,5]^2, 64x64 quads, N = 3
-(16 nodes per element, 65,536 DOFs), FP64, the reference's limiter (SubcellLimiter
-with PositivityBound).  A step is one SSP-RK3 step (three limited RHS evaluations and
-stage updates) of the whole mesh; the state, time and dt stay in HBM for the timed region.
+Headline workload (``value``): the batched workload -- 48 independent 2D periodic Euler solves
+(element grids Kx, Ky in {32..160}, N in {2,3,4}, seed 0; about 7.4M DOFs) advanced together
+in mixed precision (FP32 bulk, FP64 limiter, RK stages and state), one fused launch per RK
+stage for all instances, steps replayed from a hipGraph, launch shape autotuned.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config vortex_2d]
+Beside it, on the 2D N=3 quad configurations (vortex 64x64, four-state Riemann 256x256), the
+FP64 drop-in (``p2de_create``) and the mixed path are timed on the same steps, with their
+ratio and the density L1/Linf difference of the two results (the batched instances are also
+run one by one through the FP64 drop-in for that comparison).
 
-Prints one JSON line.  ``value`` = DOFs x 3 stages x K steps / wall seconds.
-``roofline`` is for the dominant kernel (the element RHS, phase A): algorithmic bytes =
-96 B per DOF-stage (read U^(s) and U^n, write U^(s+1), FP64, 2D; 72 B in 1D) x the DOFs one
-launch processes, over its average duration from HIP events on the library's stream.
-``cpu_baseline`` times the CPU oracle (a restatement of the reference loops, OpenMP over
-elements) on the host cores on a bounded sample of the same workload.
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-compare]
+
+Prints one JSON line.  ``value`` = DOFs x 3 stages x K steps / wall seconds of the batch.
+``roofline`` is for the dominant kernel (the fused stage kernel): algorithmic bytes = 96 B per
+DOF-stage (a step reads U^n three times and the two intermediate stages once each and writes
+three stages: 9 FP64 state vectors of 32 B per 3 stages) x the DOFs one launch covers, over its
+average duration from HIP events on the library's stream.  ``traffic`` is the HBM traffic per
+launch measured with rocprofv3 (FETCH_SIZE, WRITE_SIZE, calibrated on a device copy) for the
+same workload, read from profiles/ when recorded there.  ``cpu_baseline`` times the CPU oracle
+(a restatement of the reference loops, OpenMP over elements) on the host on a bounded sample.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -28,52 +36,96 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "DOF-stage updates/s, 2D Euler N=3: mixed/batched vs FP64 at matched L1 error"
-
-WORKLOADS = {
-    "vortex_2d": dict(kw=dict(), desc="2D isentropic vortex, periodic [-5,5]^2, 64x64 quads, N=3, FP64"),
-    "riemann4_2d": dict(kw=dict(), desc="2D four-state Riemann, [0,1]^2, 256x256 quads, N=3, FP64"),
-    "kelvin_helmholtz": dict(kw=dict(), desc="2D Kelvin-Helmholtz, periodic [-1,1]^2, 512x512 quads, N=4, FP64"),
-    "sod_1d": dict(kw=dict(), desc="1D Sod tube, K=100, N=3, FP64"),
-}
+BYTES_PER_DOF_STAGE = 96
+PROFILE = ROOT / "profiles" / "mix_stage_batched48_traffic.md"
 
 
-def bytes_per_dof_stage(Nd: int) -> int:
-    return 3 * (Nd + 2) * 8
+def profiled_traffic():
+    """HBM bytes per fused-stage launch of the batched workload, as recorded by the rocprofv3
+    counter passes summarised in profiles/ (None if not recorded)."""
+    if not PROFILE.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", PROFILE.read_text())
+    return float(m.group(1)) if m else None
 
 
-def cpu_baseline(pkg, name, kw, min_seconds=6.0):
-    """Time the oracle on the host: warm-up step, then steps until min_seconds elapse."""
+def density_diff(fp, Um, U64):
+    """Relative L1 (quadrature-weighted) and Linf difference of the densities."""
+    import numpy as np
+    wJ = fp.wq[None, :] * fp.Jq.reshape(fp.K, fp.Nq)
+    d = Um[..., 0] - U64[..., 0]
+    l1 = float(np.sum(wJ * np.abs(d)) / np.sum(wJ * np.abs(U64[..., 0])))
+    linf = float(np.max(np.abs(d)) / np.max(np.abs(U64[..., 0])))
+    return l1, linf
+
+
+def time_batch(BatchSolver, fps, Us, steps, warmup, tune=True):
+    bs = BatchSolver(fps)
+    for i, U in enumerate(Us):
+        bs.set_state(i, U)
+    bs.set_time(0.0)
+    if tune:
+        bs.autotune(3)
+    bs.enqueue_steps(warmup)
+    bs.synchronize()
+    for i, U in enumerate(Us):
+        bs.set_state(i, U)
+    bs.set_time(0.0)
+    bs.synchronize()
+    t0 = time.perf_counter()
+    bs.enqueue_steps(steps)
+    bs.synchronize()
+    wall = time.perf_counter() - t0
+    return bs, wall
+
+
+def time_fp64(DeviceSolver, fp, U, steps, warmup):
+    dev = DeviceSolver(fp)
+    dev.set_state(U)
+    dev.set_time(0.0)
+    dev.autotune(3)
+    dev.enqueue_steps(warmup)
+    dev.synchronize()
+    dev.set_state(U)
+    dev.set_time(0.0)
+    dev.synchronize()
+    t0 = time.perf_counter()
+    dev.enqueue_steps(steps)
+    dev.synchronize()
+    return dev, time.perf_counter() - t0
+
+
+def cpu_baseline(pkg, fp, U, min_seconds=10.0):
+    """Time the oracle on the host cores: one warm-up step, then steps until min_seconds pass."""
     from oracle import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     O.set_threads(threads)
-    solver, state, sp = pkg.configs.build(name, **kw)
-    fp = pkg.flat_problem(solver, sp)
     orc = O.Oracle(fp)
-    U = state.preallocation.Uq.copy()
-    t = 0.0
-    t, _ = orc.step(U, t)
+    U = U.copy()
+    t, _ = orc.step(U, 0.0)
     steps = 0
     t0 = time.perf_counter()
     while True:
         t, _ = orc.step(U, t)
         steps += 1
         el = time.perf_counter() - t0
-        if el >= min_seconds or steps >= 200:
+        if el >= min_seconds or steps >= 1000:
             break
     dofs = fp.K * fp.Nq
     return {"value": dofs * 3 * steps / el, "unit": "DOF-stage updates/s", "cores": O.get_threads(),
             "kind": "port",
-            "sample": f"{name} {fp.K} elements N={fp.N1D - 1}: {steps} SSP-RK3 steps after 1 warm-up step, "
-                      f"{el:.2f} s wall on {O.get_threads()} OpenMP threads"}
+            "sample": f"batched instance 0 ({fp.Kx}x{fp.Ky} elements, N={fp.N1D - 1}, {dofs} DOFs), FP64: "
+                      f"{steps} SSP-RK3 steps after 1 warm-up step, {el:.2f} s wall on {O.get_threads()} "
+                      f"OpenMP threads"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--config", default="vortex_2d", choices=sorted(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison legs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB mebibytes (calibrates rocprof "
                          "FETCH_SIZE/WRITE_SIZE against a known byte count) and exit")
@@ -98,45 +150,38 @@ def main():
 
     import __graft_entry__ as G
     pkg = G.load_package()
-    from p2de_jl_amd.solver import DeviceSolver
+    from p2de_jl_amd.solver import BatchSolver, DeviceSolver
 
-    wl = WORKLOADS[args.config]
-    solver, state, sp = pkg.configs.build(args.config, **wl["kw"])
-    fp = pkg.flat_problem(solver, sp)
-    dev = DeviceSolver(fp)
-    U0 = state.preallocation.Uq
-    dofs = fp.K * fp.Nq
-
-    # warm-up (untimed)
-    dev.set_state(U0)
-    dev.set_time(0.0)
-    dev.enqueue_steps(args.warmup)
-    dev.synchronize()
-
-    # timed region: K steps, state resident in HBM
-    dev.set_state(U0)
-    dev.set_time(0.0)
-    dev.synchronize()
-    t0 = time.perf_counter()
-    dev.enqueue_steps(args.steps)
-    dev.synchronize()
-    wall = time.perf_counter() - t0
-    t_end, dt_last = dev.get_time()
-    rep = dev.check_positivity()
-
-    # per-kernel durations (HIP events on the library's stream), separate pass
-    dev.set_state(U0)
-    dev.set_time(0.0)
-    dev.set_kernel_timing(True)
-    dev.enqueue_steps(min(args.steps, 50))
-    ms, cnt = dev.kernel_timing()
-    dev.set_kernel_timing(False)
-    avgA = ms[0] / max(cnt[0], 1) * 1e-3
-    avgB = ms[1] / max(cnt[1], 1) * 1e-3
-    bpd = bytes_per_dof_stage(fp.Nd)
-    achieved = bpd * dofs / avgA / 1e9
-
+    # ---------------------------------------------------------------- headline: the batch
+    inst = pkg.configs.batched_instances()
+    fps = [pkg.flat_problem(s, sp) for s, st, sp in inst]
+    Us = [st.preallocation.Uq for s, st, sp in inst]
+    bs, wall = time_batch(BatchSolver, fps, Us, args.steps, args.warmup)
+    dofs = bs.num_dofs(-1)
     value = dofs * 3 * args.steps / wall
+    Um = [bs.get_state(i) for i in range(len(fps))]
+    viol = 0
+    min_rho = min_p = float("inf")
+    for i in range(len(fps)):
+        rep = bs.check_positivity(i)
+        viol += int(rep.violations)
+        min_rho = min(min_rho, rep.min_rho)
+        min_p = min(min_p, rep.min_p)
+    v, NT, E = bs.variant()
+    # per-kernel durations (HIP events on the library's stream), separate pass
+    for i, U in enumerate(Us):
+        bs.set_state(i, U)
+    bs.set_time(0.0)
+    bs.set_kernel_timing(True)
+    bs.enqueue_steps(min(args.steps, 10))
+    ms, cnt = bs.kernel_timing()
+    bs.set_kernel_timing(False)
+    avg_stage = ms[0] / max(cnt[0], 1) * 1e-3
+    avg_cfl = ms[1] / max(cnt[1], 1) * 1e-3
+    alg_bytes = BYTES_PER_DOF_STAGE * dofs
+    achieved = alg_bytes / avg_stage / 1e9
+    traffic = profiled_traffic()
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -148,21 +193,64 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (analytic initial condition of the named problem)",
-        "config": {"workload": wl["desc"], "name": args.config, "elements": fp.K, "nodes_per_element": fp.Nq,
-                   "dofs": dofs, "precision": "fp64", "limiter": "SubcellLimiter(PositivityBound), reference bounds",
-                   "t_end": t_end, "last_dt": dt_last, "positivity_violations": int(rep.violations),
-                   "min_rho": rep.min_rho, "min_p": rep.min_p},
+        "dtype": "f32 bulk + f64 limiter/RK/state",
+        "data": "synthetic (analytic initial conditions of the named problems, seed 0)",
+        "config": {"workload": "batched: 48 periodic 2D Euler solves, Kx,Ky in {32..160}, N in {2,3,4}, "
+                               "seed 0, mixed precision, hipGraph RK3", "instances": len(fps), "dofs": dofs,
+                   "dofs_by_degree": {f"N{n}": sum(fp.K * fp.Nq for fp in fps if fp.N1D == n + 1) for n in (2, 3, 4)},
+                   "limiter": "SubcellLimiter(PositivityBound), reference bounds",
+                   "launch_shape": {"variant": v, "threads": NT, "elements_per_tile": {f"N{n - 1}": E[n] for n in (3, 4, 5)}},
+                   "positivity_violations": viol, "min_rho": min_rho, "min_p": min_p},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "phaseA_kernel (element RHS + limiter)",
-                     "avg_launch_us": avgA * 1e6, "algorithmic_bytes_per_launch": bpd * dofs,
-                     "phaseB_avg_launch_us": avgB * 1e6},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "mix_stage_kernel (fused RHS + limiter + RK stage)",
+                     "avg_launch_us": avg_stage * 1e6, "algorithmic_bytes_per_launch": alg_bytes,
+                     "cfl_pass_avg_us": avg_cfl * 1e6},
     }
+    del bs
+
+    if not args.no_compare:
+        cmp = {}
+        # the batch through the FP64 drop-in, instance by instance, for the density difference
+        l1s, linfs, fp64_wall = [], [], 0.0
+        for i, (fp, U) in enumerate(zip(fps, Us)):
+            dev = DeviceSolver(fp)
+            dev.set_state(U)
+            dev.set_time(0.0)
+            dev.synchronize()
+            t0 = time.perf_counter()
+            dev.enqueue_steps(args.steps)
+            dev.synchronize()
+            fp64_wall += time.perf_counter() - t0
+            l1, linf = density_diff(fp, Um[i], dev.get_state())
+            l1s.append(l1)
+            linfs.append(linf)
+            del dev
+        cmp["batched"] = {"fp64_dropin_one_by_one": dofs * 3 * args.steps / fp64_wall, "mixed_batched": value,
+                          "ratio": value / (dofs * 3 * args.steps / fp64_wall),
+                          "density_L1_max_over_instances": max(l1s), "density_Linf_max_over_instances": max(linfs)}
+        for name in ("vortex_2d", "riemann4_2d"):
+            solver, state, sp = pkg.configs.build(name)
+            fp = pkg.flat_problem(solver, sp)
+            U = state.preallocation.Uq
+            n = fp.K * fp.Nq
+            dev, w64 = time_fp64(DeviceSolver, fp, U, args.steps, args.warmup)
+            U64 = dev.get_state()
+            rep64 = dev.check_positivity()
+            del dev
+            b1, wm = time_batch(BatchSolver, [fp], [U], args.steps, args.warmup)
+            l1, linf = density_diff(fp, b1.get_state(0), U64)
+            repm = b1.check_positivity(0)
+            del b1
+            r64 = n * 3 * args.steps / w64
+            rm = n * 3 * args.steps / wm
+            cmp[name] = {"dofs": n, "fp64_dropin": r64, "mixed": rm, "ratio": rm / r64, "density_L1": l1,
+                         "density_Linf": linf, "min_p_fp64": rep64.min_p, "min_p_mixed": repm.min_p,
+                         "positivity_violations": int(rep64.violations) + int(repm.violations)}
+        out["config"]["comparison"] = cmp
     if not args.no_cpu:
         try:
-            out["cpu_baseline"] = cpu_baseline(pkg, args.config, wl["kw"])
+            out["cpu_baseline"] = cpu_baseline(pkg, fps[0], Us[0])
         except Exception as e:  # the baseline is reported, never the thing measured
             out["cpu_baseline"] = {"value": None, "error": str(e)}
     print(json.dumps(out), flush=True)

@@ -96,10 +96,8 @@ __device__ __forceinline__ float logmean_f(float a, float b) {
     const float s = a + b, d = b - a;
     const float f = d * rcpf(s);
     const float u = f * f;
-    const float F = fmaf(u, fmaf(u, fmaf(u, 1.f / 7.f, 0.2f), 1.f / 3.f), 1.f);
-    const float ser = 0.5f * s * rcpf(F);
-    const float lg = d * rcpf(__logf(b * rcpf(a)));
-    return u < 1e-2f ? ser : lg;
+    if (u < 1e-2f) return 0.5f * s * rcpf(fmaf(u, fmaf(u, fmaf(u, 1.f / 7.f, 0.2f), 1.f / 3.f), 1.f));
+    return d * rcpf(__logf(b * rcpf(a)));
 }
 
 // Chandrashekar flux fS (compressible_Navier_Stokes.jl:196-249), components of direction d
@@ -124,8 +122,14 @@ __device__ __forceinline__ double lim_param(const double* uL, double Lrho, doubl
                                             double ZEROTOL, int upper_mode) {
     double l = 1.0;
     const double r0 = uL[0] + P[0];
-    if (r0 < Lrho) l = fmax((Lrho - uL[0]) * rcp64(P[0]), 0.0);
-    if (upper_mode == 0 && r0 > 0.0) l = fmin(l, fmax(-uL[0] * rcp64(P[0]), 0.0));
+    const bool lo = r0 < Lrho, up = upper_mode == 0 && r0 > 0.0;
+    if (lo || up) {
+        const double iP = rcp64(P[0]);
+        if (lo) l = fmax((Lrho - uL[0]) * iP, 0.0);
+        if (up) l = fmin(l, fmax(-uL[0] * iP, 0.0));
+    }
+    // the quadratic only returns values in (ZEROTOL, 1]: it cannot lower l = 0
+    if (!(l > 0.0)) return l;
     const double a = P[0] * P[3] - 0.5 * (P[1] * P[1] + P[2] * P[2]);
     const double b = uL[3] * P[0] + uL[0] * P[3] - uL[1] * P[1] - uL[2] * P[2] - P[0] * Lrhoe;
     const double c = uL[3] * uL[0] - 0.5 * (uL[1] * uL[1] + uL[2] * uL[2]) - uL[0] * Lrhoe;
@@ -184,11 +188,15 @@ struct SmemM {
             float PL[4][ML];
         } a;
         struct {
-            double fL[4][MB];    // low-order subcell flux potentials
-            double df[4][MB];    // f_bar_H - f_bar_L
-            double l[MB];
+            // subcell flux potentials, accumulated in FP64 and stored once rounded: the
+            // limiter (5b) and the limited update (5c) read the same values, so the update is
+            // the convex combination the limiter bounded
+            float fL[4][MB];     // low-order potentials
+            float df[4][MB];     // f_bar_H - f_bar_L
+            float ls[2][MB];     // limiting parameter seen from the node above/right (0) and below/left (1)
             double UL[4][M];
             double Lrhoe[M];
+            double iwJ[M];       // 1 / (wq J)
         } b;
     } x;
 };
@@ -442,8 +450,8 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
         for (int c = 0; c < 4; c++) {
             fh[c] = (double)sm.BF[c][f0];
             fl[c] = (double)sm.BFL[c][f0];
-            sm.x.b.fL[c][base] = fl[c];
-            sm.x.b.df[c][base] = fh[c] - fl[c];
+            sm.x.b.fL[c][base] = (float)fl[c];
+            sm.x.b.df[c][base] = (float)(fh[c] - fl[c]);
         }
         for (int s = 1; s < N1D; s++) {
             const int q = lnode<N1D>(d, line, s - 1);
@@ -452,14 +460,13 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
             for (int c = 0; c < 4; c++) {
                 fh[c] = fma(wJ, (double)sm.RH[d][c][n], fh[c]);
                 fl[c] = fma(wJ, (double)sm.RL[d][c][n], fl[c]);
-                sm.x.b.fL[c][base + s] = fl[c];
-                sm.x.b.df[c][base + s] = fh[c] - fl[c];
+                sm.x.b.fL[c][base + s] = (float)fl[c];
+                sm.x.b.df[c][base + s] = (float)(fh[c] - fl[c]);
             }
         }
-        for (int c = 0; c < 4; c++) {
-            const double h = -(double)sm.BF[c][f1], lo = -(double)sm.BFL[c][f1];
-            sm.x.b.fL[c][base + N1D] = lo;
-            sm.x.b.df[c][base + N1D] = h - lo;
+        for (int c = 0; c < 4; c++) {   // far end: -BF exactly (FP32 values), so interfaces conserve
+            sm.x.b.fL[c][base + N1D] = -sm.BFL[c][f1];
+            sm.x.b.df[c][base + N1D] = (float)((double)sm.BFL[c][f1] - (double)sm.BF[c][f1]);
         }
     }
     __syncthreads();
@@ -474,9 +481,11 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
         const int ia = i % N1D, ib = i / N1D;
         const int rx = e * NFB + ib * (N1D + 1) + ia, ry = e * NFB + (N1D + ia) * (N1D + 1) + ib;
         const double iwJ = sm.geo[e][3] * rcp64(T.wq[i]);
+        sm.x.b.iwJ[n] = iwJ;
         double uL[4];
         for (int c = 0; c < 4; c++) {
-            const double r = (sm.x.b.fL[c][rx + 1] - sm.x.b.fL[c][rx]) + (sm.x.b.fL[c][ry + 1] - sm.x.b.fL[c][ry]);
+            const double r = ((double)sm.x.b.fL[c][rx + 1] - (double)sm.x.b.fL[c][rx]) +
+                             ((double)sm.x.b.fL[c][ry + 1] - (double)sm.x.b.fL[c][ry]);
             uL[c] = sm.U[c][n] + dt_lim * (r * iwJ);
             sm.x.b.UL[c][n] = uL[c];
         }
@@ -484,34 +493,51 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
     }
     __syncthreads();
 
-    // ---- 5b: limiter parameter of every subcell face (subcell_bound_limiter!, subcell.jl:248-349)
+    // ---- 5b: limiter parameter of every subcell face (subcell_bound_limiter!, subcell.jl:248-349),
+    //      one work item per (face, adjacent node): the faces inside elements first, then the
+    //      element-boundary faces, whose f_bar_H - f_bar_L is zero except at inflow boundaries (a
+    //      zero difference makes the parameter irrelevant, so those items finish at once)
     const double fac_dt = 4.0 * dt_lim;
+    constexpr int NI = 4 * N1D * (N1D - 1);   // sides of the inner faces per element
+    constexpr int NBS = 4 * N1D;              // element-boundary faces per element
 #pragma unroll 1
-    for (int w = tid; w < nel * NFB; w += NT) {
-        const int e = w / NFB, r = w - e * NFB;
-        const int L = r / (N1D + 1), s = r - L * (N1D + 1);
+    for (int w = tid; w < nel * (NI + NBS); w += NT) {
+        int e, L, s, side;
+        if (w < nel * NI) {
+            e = w / NI;
+            const int j = w - e * NI, fi = j >> 1;
+            side = j & 1;
+            L = fi / (N1D - 1);
+            s = 1 + (fi - L * (N1D - 1));
+        } else {
+            const int jb = w - nel * NI;
+            e = jb / NBS;
+            const int j = jb - e * NBS;
+            L = j >> 1;
+            side = j & 1;
+            s = side ? N1D : 0;
+        }
         const int d = L / N1D, line = L - d * N1D;
+        const int r = e * NFB + L * (N1D + 1) + s;
+        // side 0: the node at position s (above/right of the face), P = -4 dt df / (wq J);
+        // side 1: the node at position s - 1 (below/left), P = +4 dt df / (wq J)
+        const int n = e * NQ + lnode<N1D>(d, line, side ? s - 1 : s);
+        double dfr[4];
+        for (int c = 0; c < 4; c++) dfr[c] = (double)sm.x.b.df[c][r];
         double l = 1.0;
-        double P[4], uL[4];
-        if (s < N1D) {   // node above/right of the face
-            const int q = lnode<N1D>(d, line, s), n = e * NQ + q;
-            const double sc_ = -fac_dt * sm.geo[e][3] * rcp64(T.wq[q]);
+        if (dfr[0] != 0.0 || dfr[1] != 0.0 || dfr[2] != 0.0 || dfr[3] != 0.0) {
+            const double sc_ = (side ? fac_dt : -fac_dt) * sm.x.b.iwJ[n];
+            double P[4], uL[4];
             for (int c = 0; c < 4; c++) {
-                P[c] = sc_ * sm.x.b.df[c][w];
+                P[c] = sc_ * dfr[c];
                 uL[c] = sm.x.b.UL[c][n];
             }
-            l = fmin(l, lim_param(uL, zeta * uL[0], sm.x.b.Lrhoe[n], P, I.ZEROTOL, I.upper_mode));
+            l = lim_param(uL, zeta * uL[0], sm.x.b.Lrhoe[n], P, I.ZEROTOL, I.upper_mode);
         }
-        if (s > 0) {     // node below/left of the face
-            const int q = lnode<N1D>(d, line, s - 1), n = e * NQ + q;
-            const double sc_ = fac_dt * sm.geo[e][3] * rcp64(T.wq[q]);
-            for (int c = 0; c < 4; c++) {
-                P[c] = sc_ * sm.x.b.df[c][w];
-                uL[c] = sm.x.b.UL[c][n];
-            }
-            l = fmin(l, lim_param(uL, zeta * uL[0], sm.x.b.Lrhoe[n], P, I.ZEROTOL, I.upper_mode));
-        }
-        sm.x.b.l[w] = l;
+        // rounded toward zero: a smaller parameter stays admissible
+        sm.x.b.ls[side][r] = __double2float_rz(l);
+        if (s == 0) sm.x.b.ls[1][r] = 1.0f;
+        if (s == N1D) sm.x.b.ls[0][r] = 1.0f;
     }
     __syncthreads();
 
@@ -523,15 +549,17 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
         const int e = n / NQ, i = n - e * NQ;
         const int ia = i % N1D, ib = i / N1D;
         const int rx = e * NFB + ib * (N1D + 1) + ia, ry = e * NFB + (N1D + ia) * (N1D + 1) + ib;
-        const double iwJ = sm.geo[e][3] * rcp64(T.wq[i]);
-        const double lx0 = sm.x.b.l[rx], lx1 = sm.x.b.l[rx + 1];
-        const double ly0 = sm.x.b.l[ry], ly1 = sm.x.b.l[ry + 1];
+        const double iwJ = sm.x.b.iwJ[n];
+        const double lx0 = fminf(sm.x.b.ls[0][rx], sm.x.b.ls[1][rx]);
+        const double lx1 = fminf(sm.x.b.ls[0][rx + 1], sm.x.b.ls[1][rx + 1]);
+        const double ly0 = fminf(sm.x.b.ls[0][ry], sm.x.b.ls[1][ry]);
+        const double ly1 = fminf(sm.x.b.ls[0][ry + 1], sm.x.b.ls[1][ry + 1]);
         double r[4];
         for (int c = 0; c < 4; c++) {
-            const double gx0 = fma(lx0, sm.x.b.df[c][rx], sm.x.b.fL[c][rx]);
-            const double gx1 = fma(lx1, sm.x.b.df[c][rx + 1], sm.x.b.fL[c][rx + 1]);
-            const double gy0 = fma(ly0, sm.x.b.df[c][ry], sm.x.b.fL[c][ry]);
-            const double gy1 = fma(ly1, sm.x.b.df[c][ry + 1], sm.x.b.fL[c][ry + 1]);
+            const double gx0 = fma(lx0, (double)sm.x.b.df[c][rx], (double)sm.x.b.fL[c][rx]);
+            const double gx1 = fma(lx1, (double)sm.x.b.df[c][rx + 1], (double)sm.x.b.fL[c][rx + 1]);
+            const double gy0 = fma(ly0, (double)sm.x.b.df[c][ry], (double)sm.x.b.fL[c][ry]);
+            const double gy1 = fma(ly1, (double)sm.x.b.df[c][ry + 1], (double)sm.x.b.fL[c][ry + 1]);
             r[c] = ((gx1 - gx0) + (gy1 - gy0)) * iwJ;
         }
         const size_t o = ob + (size_t)n * 4;
