@@ -11,6 +11,9 @@ ratio and the density L1/Linf difference of the two results (the batched instanc
 run one by one through the FP64 drop-in for that comparison).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-compare]
+    python bench.py --profile WORKLOAD [--precision mixed|fp64] [--variant V] [--steps K]
+        (runs K steps of one workload and nothing else: the command the rocprofv3
+         summaries in profiles/ were taken with)
 
 Prints one JSON line.  ``value`` = DOFs x 3 stages x K steps / wall seconds of the batch.
 ``roofline`` is for the dominant kernel (the fused stage kernel): algorithmic bytes = 96 B per
@@ -126,6 +129,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison legs")
+    ap.add_argument("--profile", choices=["batched", "vortex_2d", "riemann4_2d", "kelvin_helmholtz"],
+                    help="only run --steps steps of this workload (for rocprofv3) and exit")
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--variant", type=int, default=0, help="launch shape for --profile")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB mebibytes (calibrates rocprof "
                          "FETCH_SIZE/WRITE_SIZE against a known byte count) and exit")
@@ -151,6 +158,36 @@ def main():
     import __graft_entry__ as G
     pkg = G.load_package()
     from p2de_jl_amd.solver import BatchSolver, DeviceSolver
+
+    if args.profile:
+        if args.profile == "batched":
+            inst = pkg.configs.batched_instances()
+        else:
+            inst = [pkg.configs.build(args.profile)]
+        fps = [pkg.flat_problem(s, sp) for s, st, sp in inst]
+        Us = [st.preallocation.Uq for s, st, sp in inst]
+        if args.precision == "fp64":
+            if len(fps) != 1:
+                sys.exit("bench.py: the FP64 drop-in runs one instance")
+            dev = DeviceSolver(fps[0])
+            dev.set_variant(args.variant)
+            dev.set_state(Us[0])
+            dev.set_time(0.0)
+            dev.enqueue_steps(args.steps)
+            dev.synchronize()
+            desc = dev.variant()
+        else:
+            dev = BatchSolver(fps)
+            dev.set_variant(args.variant)
+            for i, U in enumerate(Us):
+                dev.set_state(i, U)
+            dev.set_time(0.0)
+            dev.enqueue_steps(args.steps)
+            dev.synchronize()
+            desc = dev.variant()
+        print(json.dumps({"profile": args.profile, "precision": args.precision, "variant": desc,
+                          "steps": args.steps, "dofs": sum(fp.K * fp.Nq for fp in fps)}), flush=True)
+        return
 
     # ---------------------------------------------------------------- headline: the batch
     inst = pkg.configs.batched_instances()

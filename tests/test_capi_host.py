@@ -117,3 +117,46 @@ def test_null_arguments():
     assert L.p2de_rhs(None, 0.0, 0.0, 1, None, None) == S.P2DE_E_INVALID
     assert L.p2de_run(None, 1, None) == S.P2DE_E_INVALID
     assert L.p2de_num_dofs(None) == 0
+
+
+# ----------------------------------------------------------------- batched C-ABI, host checks
+def _batch_create(fps, **cfg_over):
+    """p2de_batch_create on descriptors as BatchSolver builds them, with config overrides;
+    returns (status, message).  The checks run before any device call."""
+    n = len(fps)
+    cfgs, opss, geos, bcs = (S._Config * n)(), (S._Operators * n)(), (S._Geometry * n)(), (S._BCData * n)()
+    keep = []
+    for j, fp in enumerate(fps):
+        cfg, ops, geo, bc, kp = S._descriptors(fp, None, S.PRECISION_MIXED)
+        for k, v in cfg_over.items():
+            setattr(cfg, k, v)
+        cfgs[j], opss[j], geos[j], bcs[j] = cfg, ops, geo, bc
+        keep.append(kp)
+    h = C.c_void_p()
+    st = S.lib().p2de_batch_create(n, cfgs, opss, geos, bcs, C.byref(h))
+    msg = S.lib().p2de_batch_last_error(None).decode()
+    if st == 0:
+        S.lib().p2de_batch_destroy(h)
+    return st, msg
+
+
+def test_batch_create_rejects_before_device_work():
+    _, _, _, fp2 = problem("vortex_2d", K=4)
+    _, _, _, fp1 = problem("sod_1d", K=8)
+    L = S.lib()
+    h = C.c_void_p()
+    assert L.p2de_batch_create(0, None, None, None, None, C.byref(h)) == S.P2DE_E_INVALID
+    st, msg = _batch_create([fp2], precision=S.PRECISION_FP64)
+    assert st == S.P2DE_E_UNSUPPORTED and "mixed" in msg
+    st, msg = _batch_create([fp1])
+    assert st == S.P2DE_E_UNSUPPORTED and msg
+    st, msg = _batch_create([fp2], N=9)
+    assert st == S.P2DE_E_UNSUPPORTED and msg
+    st, msg = _batch_create([fp2], upper_bound_mode=3)
+    assert st == S.P2DE_E_INVALID and msg
+
+
+def test_create_sends_mixed_precision_to_the_batch_api():
+    _, _, _, fp2 = problem("vortex_2d", K=4)
+    st, msg = _create(fp2, precision=S.PRECISION_MIXED)
+    assert st == S.P2DE_E_UNSUPPORTED and "p2de_batch_create" in msg
