@@ -147,26 +147,6 @@ __device__ __forceinline__ double lim_param(const double* uL, double Lrho, doubl
     return l;
 }
 
-template <int N1D>
-__device__ __forceinline__ int lnode(int d, int line, int s) {
-    return d == 0 ? s + N1D * line : line + N1D * s;
-}
-// volume node of face node f (faces r=-1, r=+1, s=-1, s=+1)
-template <int N1D>
-__device__ __forceinline__ int face_node(int f) {
-    const int face = f / N1D, m = f - face * N1D;
-    return face == 0 ? m * N1D : face == 1 ? m * N1D + N1D - 1 : face == 2 ? m : (N1D - 1) * N1D + m;
-}
-// a of pair index q = a(a-1)/2 + b, b < a
-__device__ __forceinline__ void pair_ab(int q, int& a, int& b) {
-    a = 1;
-    while (q >= a) {
-        q -= a;
-        a++;
-    }
-    b = q;
-}
-
 // LDS of one tile, component-major (structure of arrays) so that consecutive lanes touch
 // consecutive words: node n = e*NQ + i, face node e*NFP + f, pairs e*NPV + p and e*NPL + p,
 // subcell faces e*NFB + r.  Arrays whose lifetimes do not overlap share storage.
@@ -243,7 +223,7 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
                                            int nel, SmemM<N1D, E>& sm) {
     using Sm = SmemM<N1D, E>;
     constexpr int NQ = Sm::NQ, NFP = Sm::NFP, NPV = Sm::NPV, NPL = Sm::NPL, NFB = Sm::NFB;
-    constexpr int NP1 = MixTab<N1D>::NP1, LW = N1D - 1;
+    constexpr int LW = N1D - 1;
     const int tid = threadIdx.x;
     const double gd = I.gamma;
     const float g2m1 = (float)(2.0 * (gd - 1.0));
@@ -287,7 +267,7 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
         const int e = w / NFP, f = w - e * NFP;
         const int k = k0 + e;
         const int ld = f < 2 * N1D ? 0 : 1;
-        const int nq = e * NQ + face_node<N1D>(f);
+        const int nq = e * NQ + T.fq[f];
         const NodeP P = ld_np(sm, nq);
         const size_t fi = (size_t)k * NFP + f;
         const int bcc = I.bc[fi];
@@ -337,10 +317,9 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
 #pragma unroll 1
     for (int w = tid; w < nel * NPV; w += NT) {
         const int e = w / NPV, p = w - e * NPV;
-        const int d = p / (N1D * NP1), r = p - d * (N1D * NP1), line = r / NP1;
-        int aa, bb;
-        pair_ab(r - line * NP1, aa, bb);
-        const int i = e * NQ + lnode<N1D>(d, line, aa), j = e * NQ + lnode<N1D>(d, line, bb);
+        const uint32_t pk = T.pH[p];
+        const int d = (int)(pk >> 16);
+        const int i = e * NQ + (int)(pk & 255u), j = e * NQ + (int)((pk >> 8) & 255u);
         const float Sd = (float)sm.geo[e][1 + d] * T.cH[p];
         float F[4];
         fS_f(g2m1, ld_np(sm, i), ld_np(sm, j), d, F);
@@ -350,9 +329,9 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
 #pragma unroll 1
     for (int w = tid; w < nel * NPL; w += NT) {
         const int e = w / NPL, p = w - e * NPL;
-        const int d = p / (N1D * LW), r = p - d * (N1D * LW), line = r / LW;
-        const int s = r - line * LW;
-        const int i = e * NQ + lnode<N1D>(d, line, s + 1), j = e * NQ + lnode<N1D>(d, line, s);
+        const uint32_t pk = T.pL[p];
+        const int d = (int)(pk >> 16);
+        const int i = e * NQ + (int)(pk & 255u), j = e * NQ + (int)((pk >> 8) & 255u);
         const float S0 = (float)sm.geo[e][1 + d] * T.cL[p];
         const NodeP Pi = ld_np(sm, i), Pj = ld_np(sm, j);
         const float lam = fabsf(S0) * fmaxf(d == 0 ? Pi.wsx : Pi.wsy, d == 0 ? Pj.wsx : Pj.wsy);
@@ -372,19 +351,15 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
         const int ia = i % N1D, ib = i / N1D;
         const int pv = e * NPV, pl = e * NPL, fe = e * NFP;
         float Qx[4] = {0.f, 0.f, 0.f, 0.f}, Qy[4] = {0.f, 0.f, 0.f, 0.f};
-        for (int o = 0; o < N1D; o++) {
-            if (o == ia) continue;
-            const int hi = ia > o ? ia : o, lo = ia > o ? o : ia;
-            const float sg = ia > o ? 1.f : -1.f;
-            const int p = pv + ib * NP1 + hi * (hi - 1) / 2 + lo;
-            for (int c = 0; c < 4; c++) Qx[c] = fmaf(sg, sm.x.a.PH[c][p], Qx[c]);
-        }
-        for (int o = 0; o < N1D; o++) {
-            if (o == ib) continue;
-            const int hi = ib > o ? ib : o, lo = ib > o ? o : ib;
-            const float sg = ib > o ? 1.f : -1.f;
-            const int p = pv + N1D * NP1 + ia * NP1 + hi * (hi - 1) / 2 + lo;
-            for (int c = 0; c < 4; c++) Qy[c] = fmaf(sg, sm.x.a.PH[c][p], Qy[c]);
+#pragma unroll
+        for (int m = 0; m < N1D - 1; m++) {
+            const uint32_t gxm = T.gx[i][m], gym = T.gy[i][m];
+            const float sx = (gxm & 0x8000u) ? -1.f : 1.f, sy = (gym & 0x8000u) ? -1.f : 1.f;
+            const int px = pv + (int)(gxm & 0x7fffu), py = pv + (int)(gym & 0x7fffu);
+            for (int c = 0; c < 4; c++) {
+                Qx[c] = fmaf(sx, sm.x.a.PH[c][px], Qx[c]);
+                Qy[c] = fmaf(sy, sm.x.a.PH[c][py], Qy[c]);
+            }
         }
         const float av = T.a_v[i];
         float hx[4], hy[4];
@@ -454,7 +429,7 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
             sm.x.b.df[c][base] = (float)(fh[c] - fl[c]);
         }
         for (int s = 1; s < N1D; s++) {
-            const int q = lnode<N1D>(d, line, s - 1);
+            const int q = T.lq[L][s - 1];
             const int n = e * NQ + q;
             const double wJ = T.wq[q] * J;
             for (int c = 0; c < 4; c++) {
@@ -478,8 +453,7 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
 #pragma unroll 1
     for (int n = tid; n < mN; n += NT) {
         const int e = n / NQ, i = n - e * NQ;
-        const int ia = i % N1D, ib = i / N1D;
-        const int rx = e * NFB + ib * (N1D + 1) + ia, ry = e * NFB + (N1D + ia) * (N1D + 1) + ib;
+        const int rx = e * NFB + (T.nfc[i] & 255), ry = e * NFB + (T.nfc[i] >> 8);
         const double iwJ = sm.geo[e][3] * rcp64(T.wq[i]);
         sm.x.b.iwJ[n] = iwJ;
         double uL[4];
@@ -498,30 +472,23 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
     //      element-boundary faces, whose f_bar_H - f_bar_L is zero except at inflow boundaries (a
     //      zero difference makes the parameter irrelevant, so those items finish at once)
     const double fac_dt = 4.0 * dt_lim;
-    constexpr int NI = 4 * N1D * (N1D - 1);   // sides of the inner faces per element
-    constexpr int NBS = 4 * N1D;              // element-boundary faces per element
+    constexpr int NI = MixTab<N1D>::NI, NBS = MixTab<N1D>::NBS;
 #pragma unroll 1
     for (int w = tid; w < nel * (NI + NBS); w += NT) {
-        int e, L, s, side;
+        int e;
+        uint32_t pk;
         if (w < nel * NI) {
             e = w / NI;
-            const int j = w - e * NI, fi = j >> 1;
-            side = j & 1;
-            L = fi / (N1D - 1);
-            s = 1 + (fi - L * (N1D - 1));
+            pk = T.sI[w - e * NI];
         } else {
             const int jb = w - nel * NI;
             e = jb / NBS;
-            const int j = jb - e * NBS;
-            L = j >> 1;
-            side = j & 1;
-            s = side ? N1D : 0;
+            pk = T.sB[jb - e * NBS];
         }
-        const int d = L / N1D, line = L - d * N1D;
-        const int r = e * NFB + L * (N1D + 1) + s;
-        // side 0: the node at position s (above/right of the face), P = -4 dt df / (wq J);
+        // side 0: the node at position s (above/right of face s), P = -4 dt df / (wq J);
         // side 1: the node at position s - 1 (below/left), P = +4 dt df / (wq J)
-        const int n = e * NQ + lnode<N1D>(d, line, side ? s - 1 : s);
+        const int side = (int)(pk >> 16);
+        const int r = e * NFB + (int)(pk & 255u), n = e * NQ + (int)((pk >> 8) & 255u);
         double dfr[4];
         for (int c = 0; c < 4; c++) dfr[c] = (double)sm.x.b.df[c][r];
         double l = 1.0;
@@ -534,10 +501,9 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
             }
             l = lim_param(uL, zeta * uL[0], sm.x.b.Lrhoe[n], P, I.ZEROTOL, I.upper_mode);
         }
-        // rounded toward zero: a smaller parameter stays admissible
+        // rounded toward zero: a smaller parameter stays admissible; a boundary face has one side
         sm.x.b.ls[side][r] = __double2float_rz(l);
-        if (s == 0) sm.x.b.ls[1][r] = 1.0f;
-        if (s == N1D) sm.x.b.ls[0][r] = 1.0f;
+        if (w >= nel * NI) sm.x.b.ls[1 - side][r] = 1.0f;
     }
     __syncthreads();
 
@@ -547,8 +513,7 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
 #pragma unroll 1
     for (int n = tid; n < mN; n += NT) {
         const int e = n / NQ, i = n - e * NQ;
-        const int ia = i % N1D, ib = i / N1D;
-        const int rx = e * NFB + ib * (N1D + 1) + ia, ry = e * NFB + (N1D + ia) * (N1D + 1) + ib;
+        const int rx = e * NFB + (T.nfc[i] & 255), ry = e * NFB + (T.nfc[i] >> 8);
         const double iwJ = sm.x.b.iwJ[n];
         const double lx0 = fminf(sm.x.b.ls[0][rx], sm.x.b.ls[1][rx]);
         const double lx1 = fminf(sm.x.b.ls[0][rx + 1], sm.x.b.ls[1][rx + 1]);

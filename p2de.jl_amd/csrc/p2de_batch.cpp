@@ -208,10 +208,48 @@ int build_mixtab(const ElemTables& T, const p2de_operators* ops, MixTab<N1D>& M,
     }
     for (int f = 0; f < NFP; f++) {
         M.Bf[f] = (float)(f < 2 * N1D ? T.Br[f] : T.Bs[f]);
+        M.fq[f] = (uint8_t)T.fq2q[f];
         if (T.vf_coef[f] != 1.0) {
             err = "Vf is not a unit face-node selection";
             return P2DE_E_UNSUPPORTED;
         }
+    }
+    // index tables (layout in p2de_mixed.h)
+    auto pack = [](int a, int b, int c) { return (uint32_t)a | ((uint32_t)b << 8) | ((uint32_t)c << 16); };
+    for (int d = 0; d < 2; d++)
+        for (int line = 0; line < N1D; line++) {
+            for (int a = 1; a < N1D; a++)
+                for (int bb = 0; bb < a; bb++)
+                    M.pH[d * N1D * NP1 + line * NP1 + a * (a - 1) / 2 + bb] = pack(node(d, line, a), node(d, line, bb), d);
+            for (int s = 0; s + 1 < N1D; s++)
+                M.pL[d * N1D * (N1D - 1) + line * (N1D - 1) + s] = pack(node(d, line, s + 1), node(d, line, s), d);
+            for (int s = 0; s < N1D; s++) M.lq[d * N1D + line][s] = (uint16_t)node(d, line, s);
+        }
+    for (int i = 0; i < NQ; i++) {
+        const int ia = i % N1D, ib = i / N1D;
+        int m = 0;
+        for (int o = 0; o < N1D; o++) {
+            if (o == ia) continue;
+            const int hi = std::max(ia, o), lo = std::min(ia, o);
+            M.gx[i][m++] = (uint16_t)(ib * NP1 + hi * (hi - 1) / 2 + lo) | (ia > o ? 0 : 0x8000);
+        }
+        m = 0;
+        for (int o = 0; o < N1D; o++) {
+            if (o == ib) continue;
+            const int hi = std::max(ib, o), lo = std::min(ib, o);
+            M.gy[i][m++] = (uint16_t)(N1D * NP1 + ia * NP1 + hi * (hi - 1) / 2 + lo) | (ib > o ? 0 : 0x8000);
+        }
+        M.nfc[i] = (uint16_t)((ib * (N1D + 1) + ia) | (((N1D + ia) * (N1D + 1) + ib) << 8));
+    }
+    // limiter items: side 0 = the node at position s (above/right of face s), side 1 = position s - 1
+    int ni = 0, nb = 0;
+    for (int L = 0; L < 2 * N1D; L++) {
+        const int d = L / N1D, line = L % N1D;
+        for (int s = 1; s < N1D; s++)
+            for (int side = 0; side < 2; side++)
+                M.sI[ni++] = pack(L * (N1D + 1) + s, node(d, line, side ? s - 1 : s), side);
+        M.sB[nb++] = pack(L * (N1D + 1), node(d, line, 0), 0);
+        M.sB[nb++] = pack(L * (N1D + 1) + N1D, node(d, line, N1D - 1), 1);
     }
     return P2DE_OK;
 }
