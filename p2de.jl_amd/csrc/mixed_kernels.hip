@@ -551,14 +551,6 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
     }
 }
 
-// A tile function as a call of its own, for kernels compiled for several degrees: each degree
-// then gets its own register allocation instead of the union of all of them.
-template <int N1D, int E, int NT>
-__device__ __noinline__ void stage_tile_call(const MixArgs& a, const MixInst& I, const MixTab<N1D>& T, int k0,
-                                             int nel, SmemM<N1D, E>& sm) {
-    stage_tile<N1D, E, NT>(a, I, T, k0, nel, sm);
-}
-
 // CFL time step of the low-order scheme (calculate_lambda_and_low_order_CFL!,
 // low_order_graph_viscosity.jl:222-285): per node CFL*wq*J/2 / (sum_j lambda_ij + sum_f lambda_B).
 template <int N1D, int E, int NT>
@@ -670,24 +662,31 @@ __device__ __forceinline__ void load_tab(const MixArgs& a, unsigned char* dst) {
 }
 
 // This workgroup's tiles: t0, t0 + step, ... < hi within the chunk of its XCD.
-__device__ __forceinline__ void tile_range(int ntiles, int& t0, int& hi, int& step) {
+__device__ __forceinline__ void tile_range(const MixArgs& a, int& t0, int& hi, int& step) {
     const int G = gridDim.x, b = blockIdx.x;
     if (G < MIX_CHUNKS) {
         t0 = b;
-        hi = ntiles;
+        hi = a.ntiles;
         step = G;
         return;
     }
     const int c = b % MIX_CHUNKS, r = b / MIX_CHUNKS;
-    t0 = (int)((int64_t)ntiles * c / MIX_CHUNKS) + r;
-    hi = (int)((int64_t)ntiles * (c + 1) / MIX_CHUNKS);
+    t0 = a.chunk[c] + r;
+    hi = a.chunk[c + 1];
     step = G / MIX_CHUNKS + (c < G % MIX_CHUNKS ? 1 : 0);
 }
 
+// Kernels compiled for several degrees ask for 4 waves per SIMD: the union of the inlined
+// tile functions would otherwise take more registers than any one of them (137-169 VGPRs
+// against at most 112), and calling them out of line passes the uniform arguments in VGPRs.
+template <int E2, int E3, int E4, int E5>
+constexpr int min_waves() {
+    return (E2 > 0) + (E3 > 0) + (E4 > 0) + (E5 > 0) > 1 ? 4 : 1;
+}
+
 template <int NT, int E2, int E3, int E4, int E5>
-__global__ __launch_bounds__(NT) void mix_stage_kernel(MixArgs a) {
+__global__ __launch_bounds__(NT, (min_waves<E2, E3, E4, E5>())) void mix_stage_kernel(MixArgs a) {
     using LY = Layout<E2, E3, E4, E5>;
-    constexpr bool multi = (E2 > 0) + (E3 > 0) + (E4 > 0) + (E5 > 0) > 1;
     __shared__ __align__(16) unsigned char tabs[LY::TEND > 0 ? LY::TEND : 16];
     __shared__ __align__(16) unsigned char raw[LY::SMAX];
     load_tab<2, E2, NT>(a, tabs + LY::T2);
@@ -696,7 +695,7 @@ __global__ __launch_bounds__(NT) void mix_stage_kernel(MixArgs a) {
     load_tab<5, E5, NT>(a, tabs + LY::T5);
     __syncthreads();
     int t0, hi, step;
-    tile_range(a.ntiles, t0, hi, step);
+    tile_range(a, t0, hi, step);
     for (int t = t0; t < hi; t += step) {
         const MixTile tl = a.tiles[t];
         if (a.only >= 0 && tl.inst != a.only) continue;
@@ -708,8 +707,7 @@ __global__ __launch_bounds__(NT) void mix_stage_kernel(MixArgs a) {
                 {
                     auto& sm = *(SmemM<2, (E2 > 0 ? E2 : 1)>*)raw;
                     const auto& T = *(const MixTab<2>*)(tabs + LY::T2);
-                    if constexpr (multi) stage_tile_call<2, E2, NT>(a, I, T, tl.k0, tl.nel, sm);
-                    else stage_tile<2, E2, NT>(a, I, T, tl.k0, tl.nel, sm);
+                    stage_tile<2, E2, NT>(a, I, T, tl.k0, tl.nel, sm);
                 }
                 break;
             case 3:
@@ -717,8 +715,7 @@ __global__ __launch_bounds__(NT) void mix_stage_kernel(MixArgs a) {
                 {
                     auto& sm = *(SmemM<3, (E3 > 0 ? E3 : 1)>*)raw;
                     const auto& T = *(const MixTab<3>*)(tabs + LY::T3);
-                    if constexpr (multi) stage_tile_call<3, E3, NT>(a, I, T, tl.k0, tl.nel, sm);
-                    else stage_tile<3, E3, NT>(a, I, T, tl.k0, tl.nel, sm);
+                    stage_tile<3, E3, NT>(a, I, T, tl.k0, tl.nel, sm);
                 }
                 break;
             case 4:
@@ -726,8 +723,7 @@ __global__ __launch_bounds__(NT) void mix_stage_kernel(MixArgs a) {
                 {
                     auto& sm = *(SmemM<4, (E4 > 0 ? E4 : 1)>*)raw;
                     const auto& T = *(const MixTab<4>*)(tabs + LY::T4);
-                    if constexpr (multi) stage_tile_call<4, E4, NT>(a, I, T, tl.k0, tl.nel, sm);
-                    else stage_tile<4, E4, NT>(a, I, T, tl.k0, tl.nel, sm);
+                    stage_tile<4, E4, NT>(a, I, T, tl.k0, tl.nel, sm);
                 }
                 break;
             case 5:
@@ -735,8 +731,7 @@ __global__ __launch_bounds__(NT) void mix_stage_kernel(MixArgs a) {
                 {
                     auto& sm = *(SmemM<5, (E5 > 0 ? E5 : 1)>*)raw;
                     const auto& T = *(const MixTab<5>*)(tabs + LY::T5);
-                    if constexpr (multi) stage_tile_call<5, E5, NT>(a, I, T, tl.k0, tl.nel, sm);
-                    else stage_tile<5, E5, NT>(a, I, T, tl.k0, tl.nel, sm);
+                    stage_tile<5, E5, NT>(a, I, T, tl.k0, tl.nel, sm);
                 }
                 break;
         }
@@ -755,7 +750,7 @@ __global__ __launch_bounds__(NT) void mix_dt_kernel(MixArgs a) {
     load_tab<5, E5, NT>(a, tabs + LY::T5);
     __syncthreads();
     int t0, hi, step;
-    tile_range(a.ntiles, t0, hi, step);
+    tile_range(a, t0, hi, step);
     for (int t = t0; t < hi; t += step) {
         const MixTile tl = a.tiles[t];
         if (a.only >= 0 && tl.inst != a.only) continue;
@@ -888,9 +883,9 @@ struct MixVariant {
 const MixVariant kMix[] = {
     MV(128, 0, 0, 4, 0), MV(64, 0, 0, 4, 0), MV(128, 0, 0, 8, 0), MV(256, 0, 0, 8, 0), MV(64, 0, 0, 2, 0),
     MV(128, 0, 0, 0, 2), MV(64, 0, 0, 0, 2), MV(128, 0, 0, 0, 4), MV(256, 0, 0, 0, 4),
-    MV(128, 0, 8, 0, 0), MV(64, 0, 4, 0, 0), MV(256, 0, 16, 0, 0),
+    MV(128, 0, 8, 0, 0), MV(64, 0, 4, 0, 0), MV(256, 0, 16, 0, 0), MV(256, 0, 12, 0, 0),
     MV(128, 16, 0, 0, 0), MV(64, 8, 0, 0, 0),
-    MV(128, 0, 8, 4, 2), MV(64, 0, 4, 2, 1), MV(256, 0, 16, 8, 4), MV(128, 16, 8, 4, 2),
+    MV(256, 0, 12, 8, 5), MV(256, 0, 12, 8, 4), MV(256, 12, 12, 8, 5), MV(256, 12, 12, 8, 4),
 };
 #undef MV
 constexpr int kNumMix = (int)(sizeof(kMix) / sizeof(kMix[0]));

@@ -49,6 +49,7 @@ struct p2de_batch {
     MixInst* d_inst = nullptr;
     MixTile* d_tiles = nullptr;
     int ntiles = 0;
+    int chunk[MIX_CHUNKS + 1] = {};            // XCD chunk boundaries in the tile list
     double* d_dtp = nullptr;                   // per-tile CFL minima
     std::vector<MixInst> hinst;                // host copy of d_inst
     void* d_tab[MIX_MAX_N1D + 1] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -297,6 +298,23 @@ int build_tiles(p2de_batch* b) {
         b->hinst[i].tile_hi = (int)tiles.size();
     }
     b->ntiles = (int)tiles.size();
+    // XCD chunks of about equal estimated cost: DOFs, with the degree-4 tiles weighted by the
+    // per-DOF time the single-degree kernels measure (about 1.3x that of degrees 1-3)
+    {
+        std::vector<double> cost(tiles.size() + 1, 0.0);
+        for (size_t t = 0; t < tiles.size(); t++) {
+            const int n1d = tiles[t].n1d;
+            cost[t + 1] = cost[t] + (double)tiles[t].nel * n1d * n1d * (n1d == 5 ? 1.3 : 1.0);
+        }
+        b->chunk[0] = 0;
+        size_t t = 0;
+        for (int c = 1; c < MIX_CHUNKS; c++) {
+            const double target = cost.back() * c / MIX_CHUNKS;
+            while (t < tiles.size() && cost[t + 1] <= target) t++;
+            b->chunk[c] = (int)t;
+        }
+        b->chunk[MIX_CHUNKS] = b->ntiles;
+    }
     BHIP(b, hipMemcpyAsync(b->d_tiles, tiles.data(), sizeof(MixTile) * tiles.size(), hipMemcpyHostToDevice, b->stream));
     BHIP(b, hipMemcpyAsync(b->d_inst, b->hinst.data(), sizeof(MixInst) * b->n, hipMemcpyHostToDevice, b->stream));
     BHIP(b, hipStreamSynchronize(b->stream));
@@ -350,6 +368,7 @@ MixArgs margs(p2de_batch* b, int mode, int only, const double* Uin, const double
     a.inst = b->d_inst;
     a.tiles = b->d_tiles;
     a.ntiles = b->ntiles;
+    for (int c = 0; c <= MIX_CHUNKS; c++) a.chunk[c] = b->chunk[c];
     a.dtp = b->d_dtp;
     for (int n = 0; n <= MIX_MAX_N1D; n++) a.tab[n] = b->d_tab[n];
     a.mode = mode;

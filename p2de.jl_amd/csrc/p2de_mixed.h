@@ -83,6 +83,7 @@ struct MixArgs {
     const MixInst* inst;
     const MixTile* tiles;
     int ntiles;
+    int chunk[MIX_CHUNKS + 1];  // tile ranges of the XCD chunks (cut by estimated cost)
     double* dtp;          // per-tile minimum of the CFL time step (CFL pass)
     const void* tab[MIX_MAX_N1D + 1];   // MixTab<N1D> by N1D
     int mode;             // MODE_RHS, MODE_STAGE1..3
