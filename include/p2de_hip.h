@@ -42,7 +42,8 @@ enum { P2DE_POSITIVITY_BOUND = 0 };
 enum { P2DE_NO_SHOCK_CAPTURE = 0 };
 /* FP64: the drop-in, matching the reference's FP64 arithmetic.  MIXED (2D only): FP32 flux
  * differencing, surface and low-order fluxes and element assembly; FP64 node pressures,
- * subcell flux potentials, limiter bounds and coefficients, RK stages and state. */
+ * subcell flux potentials, limiter bounds and coefficients, RK stages and state.  A context
+ * created with MIXED holds a one-instance p2de_batch and forwards every call to it. */
 enum { P2DE_PRECISION_FP64 = 0, P2DE_PRECISION_MIXED = 1 };
 /* Upper density bound of the subcell limiter.  REFERENCE: as limiter_utils.jl:67
  * destructures the 5-value bound tuple of subcell.jl:299 into 4 names, so

@@ -465,6 +465,19 @@ std::string tune_key(const p2de_batch* b) {
 
 }  // namespace
 
+int batch_instance_stats(p2de_batch* b, int inst, double* t, double* last_dt, int* steps, int64_t* viol) {
+    if (!b) return bfail(b, P2DE_E_INVALID, "NULL batch");
+    if (inst < 0 || inst >= b->n) return bfail(b, P2DE_E_INVALID, "instance out of range");
+    int st = read_sc(b);
+    if (st) return st;
+    const StepScalars& S = b->h_sc[inst];
+    if (t) *t = S.t;
+    if (last_dt) *last_dt = h_bits_to_double(S.dt_bits);
+    if (steps) *steps = S.steps;
+    if (viol) *viol = (int64_t)S.viol;
+    return P2DE_OK;
+}
+
 extern "C" {
 
 int p2de_batch_create(int n, const p2de_config* cfg, const p2de_operators* ops, const p2de_geometry* geom,

@@ -16,6 +16,7 @@
 
 #include "p2de_hip.h"
 #include "p2de_internal.h"
+#include "p2de_mixed.h"
 #include "p2de_setup.h"
 
 using namespace p2de;
@@ -60,6 +61,7 @@ struct p2de_ctx {
     double time_ms[3] = {0, 0, 0};
     int64_t time_count[3] = {0, 0, 0};
     std::string err;
+    p2de_batch* mix = nullptr;             // precision MIXED: a one-instance batch does the work
 };
 
 namespace {
@@ -68,6 +70,15 @@ int fail(p2de_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
     g_last_error = msg;
     return code;
+}
+
+// status of a call forwarded to the context's one-instance batch, with its message
+int fwd(p2de_ctx* c, int st) {
+    if (st) {
+        c->err = p2de_batch_last_error(c->mix);
+        g_last_error = c->err;
+    }
+    return st;
 }
 
 #define HIPCHK(ctx, expr)                                                                         \
@@ -94,6 +105,7 @@ int dalloc(p2de_ctx* c, T** p, size_t n) {
 
 void free_ctx(p2de_ctx* c) {
     if (!c) return;
+    if (c->mix) p2de_batch_destroy(c->mix);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& l : c->launches) {
         (void)hipEventDestroy(l.a);
@@ -187,9 +199,17 @@ int p2de_create(const p2de_config* cfg, const p2de_operators* ops, const p2de_ge
         c->cfg = *cfg;
         int st = setup_config(cfg, &c->Nq, &c->Nfp, &c->Nh, &c->K, c->err);
         if (st) return bail(st);
-        if (cfg->precision != P2DE_PRECISION_FP64)
-            return fail(c, P2DE_E_UNSUPPORTED, "p2de_create runs FP64; mixed precision runs through p2de_batch_create"),
-                   bail(P2DE_E_UNSUPPORTED);
+        if (cfg->precision == P2DE_PRECISION_MIXED) {
+            c->Nd = cfg->Nd;
+            c->Nc = cfg->Nd + 2;
+            c->N1D = cfg->N + 1;
+            if ((st = p2de_batch_create(1, cfg, ops, geom, bc, &c->mix))) {
+                c->err = p2de_batch_last_error(nullptr);
+                return bail(st);
+            }
+            *out = c;
+            return P2DE_OK;
+        }
         c->Nd = cfg->Nd;
         c->Nc = cfg->Nd + 2;
         c->N1D = cfg->N + 1;
@@ -250,16 +270,21 @@ int p2de_create(const p2de_config* cfg, const p2de_operators* ops, const p2de_ge
 
 void p2de_destroy(p2de_ctx* ctx) { free_ctx(ctx); }
 
-int64_t p2de_num_dofs(const p2de_ctx* ctx) { return ctx ? (int64_t)ctx->K * ctx->Nq : 0; }
+int64_t p2de_num_dofs(const p2de_ctx* ctx) {
+    if (ctx && ctx->mix) return p2de_batch_num_dofs(ctx->mix, 0);
+    return ctx ? (int64_t)ctx->K * ctx->Nq : 0;
+}
 
 int p2de_device_state(p2de_ctx* ctx, void** dptr) {
     if (!ctx || !dptr) return fail(ctx, P2DE_E_INVALID, "NULL argument");
+    if (ctx->mix) return fwd(ctx, p2de_batch_device_state(ctx->mix, 0, dptr));
     *dptr = ctx->d_U[0];
     return P2DE_OK;
 }
 
 int p2de_set_state(p2de_ctx* ctx, const double* Uq) {
     if (!ctx || !Uq) return fail(ctx, P2DE_E_INVALID, "NULL argument");
+    if (ctx->mix) return fwd(ctx, p2de_batch_set_state(ctx->mix, 0, Uq));
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_U[0], Uq, sizeof(double) * (size_t)ctx->K * ctx->Nq * ctx->Nc,
                                hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -270,6 +295,7 @@ int p2de_set_state(p2de_ctx* ctx, const double* Uq) {
 
 int p2de_get_state(p2de_ctx* ctx, double* Uq) {
     if (!ctx || !Uq) return fail(ctx, P2DE_E_INVALID, "NULL argument");
+    if (ctx->mix) return fwd(ctx, p2de_batch_get_state(ctx->mix, 0, Uq));
     HIPCHK(ctx, hipMemcpyAsync(Uq, ctx->d_U[0], sizeof(double) * (size_t)ctx->K * ctx->Nq * ctx->Nc,
                                hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -278,6 +304,7 @@ int p2de_get_state(p2de_ctx* ctx, double* Uq) {
 
 int p2de_set_debug(p2de_ctx* ctx, int on) {
     if (!ctx) return fail(ctx, P2DE_E_INVALID, "NULL context");
+    if (ctx->mix) return fwd(ctx, p2de_batch_set_debug(ctx->mix, on));
     if (on && !ctx->d_dbg_rhsH) {
         const size_t nU = (size_t)ctx->K * ctx->Nq * ctx->Nc;
         const size_t nL = (size_t)(ctx->Nq + ctx->N1D) * ctx->Nd * ctx->K;
@@ -404,6 +431,7 @@ extern "C" {
 
 int p2de_rhs(p2de_ctx* ctx, double t, double dt, int nstage, double* dt_out, double* rhsU) {
     if (!ctx) return fail(ctx, P2DE_E_INVALID, "NULL context");
+    if (ctx->mix) return fwd(ctx, p2de_batch_rhs(ctx->mix, 0, t, dt, nstage, dt_out, rhsU));
     if (nstage < 1 || nstage > 3) return fail(ctx, P2DE_E_INVALID, "nstage must be 1, 2 or 3");
     if (!ctx->state_set) return fail(ctx, P2DE_E_STATE, "no state uploaded (p2de_set_state)");
     void* s = ctx->stream;
@@ -432,6 +460,7 @@ int p2de_rhs(p2de_ctx* ctx, double t, double dt, int nstage, double* dt_out, dou
 
 int p2de_get_field(p2de_ctx* ctx, int field, double* out) {
     if (!ctx || !out) return fail(ctx, P2DE_E_INVALID, "NULL argument");
+    if (ctx->mix) return fwd(ctx, p2de_batch_get_field(ctx->mix, 0, field, out));
     const size_t nU = (size_t)ctx->K * ctx->Nq * ctx->Nc;
     const double* src = nullptr;
     size_t n = nU;
@@ -453,6 +482,7 @@ int p2de_get_field(p2de_ctx* ctx, int field, double* out) {
 
 int p2de_set_time(p2de_ctx* ctx, double t) {
     if (!ctx) return fail(ctx, P2DE_E_INVALID, "NULL context");
+    if (ctx->mix) return fwd(ctx, p2de_batch_set_time(ctx->mix, 0, t));
     int st = read_scalars(ctx);
     if (st) return st;
     ctx->h_sc->t = t;
@@ -464,6 +494,7 @@ int p2de_set_time(p2de_ctx* ctx, double t) {
 
 int p2de_get_time(p2de_ctx* ctx, double* t, double* last_dt) {
     if (!ctx) return fail(ctx, P2DE_E_INVALID, "NULL context");
+    if (ctx->mix) return fwd(ctx, p2de_batch_get_time(ctx->mix, 0, t, last_dt, nullptr));
     int st = read_scalars(ctx);
     if (st) return st;
     if (t) *t = ctx->h_sc->t;
@@ -473,6 +504,17 @@ int p2de_get_time(p2de_ctx* ctx, double* t, double* last_dt) {
 
 int p2de_ssprk3_step(p2de_ctx* ctx, double* t_inout, double* dt_out) {
     if (!ctx || !t_inout) return fail(ctx, P2DE_E_INVALID, "NULL argument");
+    if (ctx->mix) {
+        int n0 = 0, n1 = 0, st;
+        double dt = 0.0;
+        if ((st = fwd(ctx, p2de_batch_set_time(ctx->mix, 0, *t_inout))) ||
+            (st = fwd(ctx, p2de_batch_get_time(ctx->mix, 0, nullptr, nullptr, &n0))) ||
+            (st = fwd(ctx, p2de_batch_enqueue_steps(ctx->mix, 1))) ||
+            (st = fwd(ctx, p2de_batch_get_time(ctx->mix, 0, t_inout, &dt, &n1))))
+            return st;
+        if (dt_out) *dt_out = n1 > n0 ? dt : 0.0;
+        return P2DE_OK;
+    }
     if (!ctx->state_set) return fail(ctx, P2DE_E_STATE, "no state uploaded (p2de_set_state)");
     int st = p2de_set_time(ctx, *t_inout);
     if (st) return st;
@@ -486,6 +528,7 @@ int p2de_ssprk3_step(p2de_ctx* ctx, double* t_inout, double* dt_out) {
 
 int p2de_enqueue_steps(p2de_ctx* ctx, int steps) {
     if (!ctx) return fail(ctx, P2DE_E_INVALID, "NULL context");
+    if (ctx->mix) return fwd(ctx, p2de_batch_enqueue_steps(ctx->mix, steps));
     if (!ctx->state_set) return fail(ctx, P2DE_E_STATE, "no state uploaded (p2de_set_state)");
     for (int i = 0; i < steps; i++) {
         int st = enqueue_step(ctx);
@@ -496,6 +539,7 @@ int p2de_enqueue_steps(p2de_ctx* ctx, int steps) {
 
 int p2de_synchronize(p2de_ctx* ctx) {
     if (!ctx) return fail(ctx, P2DE_E_INVALID, "NULL context");
+    if (ctx->mix) return fwd(ctx, p2de_batch_synchronize(ctx->mix));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->timing) harvest_timing(ctx);
     return P2DE_OK;
@@ -503,6 +547,19 @@ int p2de_synchronize(p2de_ctx* ctx) {
 
 int p2de_run(p2de_ctx* ctx, int max_steps, p2de_run_stats* stats) {
     if (!ctx) return fail(ctx, P2DE_E_INVALID, "NULL context");
+    if (ctx->mix) {
+        int n0 = 0, n1 = 0, taken = 0, st;
+        if ((st = fwd(ctx, p2de_batch_get_time(ctx->mix, 0, nullptr, nullptr, &n0))) ||
+            (st = fwd(ctx, p2de_batch_run(ctx->mix, max_steps, &taken))))
+            return st;
+        if (stats) {
+            int64_t viol = 0;
+            if ((st = fwd(ctx, batch_instance_stats(ctx->mix, 0, &stats->t, &stats->last_dt, &n1, &viol)))) return st;
+            stats->steps = n1 - n0;
+            stats->violations = viol;
+        }
+        return P2DE_OK;
+    }
     if (!ctx->state_set) return fail(ctx, P2DE_E_STATE, "no state uploaded (p2de_set_state)");
     int st = read_scalars(ctx);
     if (st) return st;
@@ -529,6 +586,7 @@ int p2de_run(p2de_ctx* ctx, int max_steps, p2de_run_stats* stats) {
 
 int p2de_check_positivity(p2de_ctx* ctx, p2de_pos_report* rep) {
     if (!ctx || !rep) return fail(ctx, P2DE_E_INVALID, "NULL argument");
+    if (ctx->mix) return fwd(ctx, p2de_batch_check_positivity(ctx->mix, 0, rep));
     if (!ctx->state_set) return fail(ctx, P2DE_E_STATE, "no state uploaded (p2de_set_state)");
     ctx->h_keys[0] = ~0ull;
     ctx->h_keys[1] = ~0ull;
@@ -558,6 +616,7 @@ int p2de_check_positivity(p2de_ctx* ctx, p2de_pos_report* rep) {
 
 int p2de_set_kernel_timing(p2de_ctx* ctx, int on) {
     if (!ctx) return fail(ctx, P2DE_E_INVALID, "NULL context");
+    if (ctx->mix) return fwd(ctx, p2de_batch_set_kernel_timing(ctx->mix, on));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     harvest_timing(ctx);
     for (int k = 0; k < 3; k++) {
@@ -570,6 +629,7 @@ int p2de_set_kernel_timing(p2de_ctx* ctx, int on) {
 
 int p2de_get_kernel_timing(p2de_ctx* ctx, double ms[3], int64_t count[3]) {
     if (!ctx || !ms || !count) return fail(ctx, P2DE_E_INVALID, "NULL argument");
+    if (ctx->mix) return fwd(ctx, p2de_batch_get_kernel_timing(ctx->mix, ms, count));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     harvest_timing(ctx);
     for (int k = 0; k < 3; k++) {
@@ -610,10 +670,14 @@ void write_cache(const char* path, const std::map<std::string, int>& m) {
 
 extern "C" {
 
-int p2de_num_variants(const p2de_ctx* ctx) { return ctx ? num_variants_A(ctx->Nd, ctx->N1D) : 0; }
+int p2de_num_variants(const p2de_ctx* ctx) {
+    if (ctx && ctx->mix) return p2de_batch_num_variants(ctx->mix);
+    return ctx ? num_variants_A(ctx->Nd, ctx->N1D) : 0;
+}
 
 int p2de_set_variant(p2de_ctx* ctx, int variant) {
     if (!ctx) return fail(ctx, P2DE_E_INVALID, "NULL context");
+    if (ctx->mix) return fwd(ctx, p2de_batch_set_variant(ctx->mix, variant));
     if (variant < 0 || variant >= num_variants_A(ctx->Nd, ctx->N1D)) return fail(ctx, P2DE_E_INVALID, "variant out of range");
     ctx->variantA = variant;
     return P2DE_OK;
@@ -621,6 +685,12 @@ int p2de_set_variant(p2de_ctx* ctx, int variant) {
 
 int p2de_get_variant(const p2de_ctx* ctx, int* variant, int* E, int* NT) {
     if (!ctx) return P2DE_E_INVALID;
+    if (ctx->mix) {
+        int Ev[6] = {0, 0, 0, 0, 0, 0};
+        const int st = p2de_batch_get_variant(ctx->mix, variant, NT, Ev);
+        if (E) *E = Ev[ctx->N1D];
+        return st;
+    }
     int e = 0, nt = 0;
     variant_shape_A(ctx->Nd, ctx->N1D, ctx->variantA, &e, &nt);
     if (variant) *variant = ctx->variantA;
@@ -631,6 +701,7 @@ int p2de_get_variant(const p2de_ctx* ctx, int* variant, int* E, int* NT) {
 
 int p2de_autotune(p2de_ctx* ctx, int steps, const char* cache_path, int* chosen) {
     if (!ctx) return fail(ctx, P2DE_E_INVALID, "NULL context");
+    if (ctx->mix) return fwd(ctx, p2de_batch_autotune(ctx->mix, steps, cache_path, chosen));
     if (!ctx->state_set) return fail(ctx, P2DE_E_STATE, "no state uploaded (p2de_set_state)");
     const std::string key = tune_key(ctx);
     auto cache = read_cache(cache_path);

@@ -18,6 +18,7 @@
 #pragma once
 #include <stdint.h>
 
+#include "p2de_hip.h"
 #include "p2de_internal.h"
 
 namespace p2de {
@@ -110,3 +111,7 @@ int mix_launch_positivity(const double* U, int64_t nodes, double gamma, unsigned
                           unsigned long long* cnt, void* stream);
 
 }  // namespace p2de
+
+// time, last dt, steps and the positivity violations counted since the last check of one
+// instance, without resetting the count (for p2de_run on a mixed-precision context)
+int batch_instance_stats(p2de_batch* b, int inst, double* t, double* last_dt, int* steps, int64_t* viol);

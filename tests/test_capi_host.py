@@ -156,7 +156,14 @@ def test_batch_create_rejects_before_device_work():
     assert st == S.P2DE_E_INVALID and msg
 
 
-def test_create_sends_mixed_precision_to_the_batch_api():
+def test_create_mixed_precision_goes_through_the_batch_checks():
+    """p2de_create with precision MIXED runs a one-instance batch: the batch's own checks
+    reject what the mixed path does not cover, with their message, before device work."""
     _, _, _, fp2 = problem("vortex_2d", K=4)
-    st, msg = _create(fp2, precision=S.PRECISION_MIXED)
-    assert st == S.P2DE_E_UNSUPPORTED and "p2de_batch_create" in msg
+    _, _, _, fp1 = problem("sod_1d", K=8)
+    st, msg = _create(fp1, precision=S.PRECISION_MIXED)
+    assert st == S.P2DE_E_UNSUPPORTED and "2D" in msg
+    st, msg = _create(fp2, precision=S.PRECISION_MIXED, N=9)
+    assert st == S.P2DE_E_UNSUPPORTED and msg
+    st, msg = _create(fp2, precision=S.PRECISION_MIXED, upper_bound_mode=3)
+    assert st == S.P2DE_E_INVALID and msg

@@ -186,3 +186,37 @@ def test_mixed_trajectory_against_fp64():
     print(f"  density L1 {l1:.2e} Linf {linf:.2e}  t {tm:.10e} vs {t64:.10e}", flush=True)
     assert abs(tm - t64) <= 1e-6 * t64
     assert l1 <= 1e-6 and linf <= 1e-5
+
+
+def test_single_context_mixed_precision_is_the_one_instance_batch():
+    """p2de_create with precision MIXED forwards to a one-instance batch: the same bits as
+    BatchSolver for rhs, fields, steps, time, run statistics and the positivity report."""
+    from p2de_jl_amd.solver import BatchSolver, DeviceSolver, FIELD_RHSH, FIELD_RHSL, PRECISION_MIXED
+    solver, state, sp, fp = problem("riemann4_2d", K=10)
+    U0 = state.preallocation.Uq
+    dev = DeviceSolver(fp, precision=PRECISION_MIXED)
+    bs = BatchSolver([fp])
+    dev.set_debug(True)
+    bs.set_debug(True)
+    dev.set_state(U0)
+    bs.set_state(0, U0)
+    dt_d, r_d = dev.rhs(0.0, 0.9 * fp.CFL * fp.dt0, 1)
+    dt_b, r_b = bs.rhs(0, 0.0, 0.9 * fp.CFL * fp.dt0, 1)
+    assert dt_d == dt_b and np.array_equal(r_d, r_b)
+    assert np.array_equal(dev.field(FIELD_RHSH), bs.field(0, FIELD_RHSH))
+    assert np.array_equal(dev.field(FIELD_RHSL), bs.field(0, FIELD_RHSL))
+    t, dt = dev.step(0.0)
+    bs.set_time(0.0)
+    bs.enqueue_steps(1)
+    bs.synchronize()
+    tb, dtb, nb = bs.get_time(0)
+    print(f"  step: t {t:.6e} dt {dt:.6e} vs batch t {tb:.6e} dt {dtb:.6e}", flush=True)
+    assert t == tb and dt == dtb and nb == 1
+    assert np.array_equal(dev.get_state(), bs.get_state(0))
+    stats = dev.run(max_steps=7)
+    bs.enqueue_steps(7)
+    bs.synchronize()
+    assert stats.steps == 7 and stats.t == bs.get_time(0)[0]
+    assert np.array_equal(dev.get_state(), bs.get_state(0))
+    rd, rb = dev.check_positivity(), bs.check_positivity(0)
+    assert rd.violations == rb.violations == 0 and rd.min_rho == rb.min_rho and rd.min_p == rb.min_p
