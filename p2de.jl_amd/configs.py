@@ -10,10 +10,15 @@ Problems (γ = 1.4, ζ = 0.1, POSTOL = 1e-14, ZEROTOL = 5e-16 unless stated):
 * ``riemann4_2d``     -- four-state Riemann problem on [0,1]^2, 256x256, N=3, T 0.25,
                          transmissive boundary.
 * ``kelvin_helmholtz``-- periodic [-1,1]^2, 512x512, N=4 (examples/2D/kelvin-helmholtz.jl:9-18).
+* ``kelvin_helmholtz_gauss`` -- the reference's KH example as written: 80x80, N=3, Gauss
+                         collocation with NodewiseScaledExtrapolation, dt0 1e-3.
 * ``smoke_vortex``    -- the reference's own smoke problem (test/test_smoke.jl:6-20,117-129).
 * ``periodic_wave``   -- one instance of the batched workload: periodic [0,1]^2, density
                          1 + 0.5 sin(2 pi kx x) sin(2 pi ky y), constant velocity, p = 1, with an
                          optional near-vacuum Gaussian dip (``batched_instances`` draws 48).
+
+Every problem takes ``basis``: "lobatto" (default), "gauss" (GaussCollocation +
+NodewiseScaledExtrapolation) or "gauss_noep" (GaussCollocation, no entropy-projection limiter).
 
 ``dt0`` for the first four is ``0.9 * dt_pos(U0) / CFL`` where ``dt_pos`` is the
 low-order positivity time step of the initial state, recorded once (so the CFL cap
@@ -29,9 +34,16 @@ import numpy as np
 
 from .dg import initialize_DG, make_periodic, primitive_to_conservative
 from .operators import BCData
-from .types import (CompressibleEulerIdealGas, Dim1, Dim2, ESLimitedLowOrderPos, GlobalConstant,
-                    LimitingParameter, LobattoCollocation, NoEntropyProjectionLimiter, Param,
-                    SubcellLimiter, TimesteppingParameter)
+from .types import (CompressibleEulerIdealGas, Dim1, Dim2, ESLimitedLowOrderPos, GaussCollocation,
+                    GlobalConstant, LimitingParameter, LobattoCollocation, NodewiseScaledExtrapolation,
+                    NoEntropyProjectionLimiter, Param, SubcellLimiter, TimesteppingParameter)
+
+# approximation basis and entropy-projection limiter by name
+BASES = {
+    "lobatto": (LobattoCollocation, NoEntropyProjectionLimiter),
+    "gauss": (GaussCollocation, NodewiseScaledExtrapolation),        # examples/2D/kelvin-helmholtz.jl:52-53
+    "gauss_noep": (GaussCollocation, NoEntropyProjectionLimiter),
+}
 
 # dt0 = 0.9*dt_pos(U0)/CFL, recorded from the initial states (tests/golden/record_dt0.py)
 RECORDED_DT0: Dict[str, float] = {
@@ -42,15 +54,16 @@ RECORDED_DT0: Dict[str, float] = {
 }
 
 
-def _param(N, K, xL, xR, T, CFL, dt0, Nd, zeta=0.1, eta=1.0):
+def _param(N, K, xL, xR, T, CFL, dt0, Nd, zeta=0.1, eta=1.0, basis="lobatto"):
+    B, EPL = BASES[basis]
     return Param(N=N, K=K, xL=xL, xR=xR,
                  global_constants=GlobalConstant(POSTOL=1e-14, ZEROTOL=5e-16),
                  timestepping_param=TimesteppingParameter(T=T, CFL=CFL, dt0=dt0, t0=0.0),
                  limiting_param=LimitingParameter(zeta=zeta, eta=eta),
                  equation=CompressibleEulerIdealGas(Dim1() if Nd == 1 else Dim2(), 1.4),
                  rhs=ESLimitedLowOrderPos(),
-                 approximation_basis=LobattoCollocation(),
-                 entropyproj_limiter=NoEntropyProjectionLimiter(),
+                 approximation_basis=B(),
+                 entropyproj_limiter=EPL(),
                  rhs_limiter=SubcellLimiter())
 
 
@@ -77,9 +90,9 @@ def sod_ic(param, x):
     return primitive_to_conservative(g, (rho, u, p))
 
 
-def sod_1d(K=100, N=3, T=0.2, CFL=0.5, dt0=None):
+def sod_1d(K=100, N=3, T=0.2, CFL=0.5, dt0=None, basis="lobatto"):
     dt0 = RECORDED_DT0["sod_1d"] if dt0 is None else dt0
-    return _param(N, K, 0.0, 1.0, T, CFL, dt0, 1), sod_ic, _transmissive_bc
+    return _param(N, K, 0.0, 1.0, T, CFL, dt0, 1, basis=basis), sod_ic, _transmissive_bc
 
 
 # ------------------------------------------------------------------------------- vortex
@@ -103,9 +116,9 @@ def vortex_ic(param, x, y):
     return primitive_to_conservative(param.equation.gamma, vortex_exact_prim(x, y, t0, param.equation.gamma))
 
 
-def vortex_2d(K=64, N=3, T=1.0, CFL=0.5, dt0=None):
+def vortex_2d(K=64, N=3, T=1.0, CFL=0.5, dt0=None, basis="lobatto"):
     dt0 = RECORDED_DT0["vortex_2d"] if dt0 is None else dt0
-    return _param(N, (K, K), (-5.0, -5.0), (5.0, 5.0), T, CFL, dt0, 2), vortex_ic, _periodic_bc
+    return _param(N, (K, K), (-5.0, -5.0), (5.0, 5.0), T, CFL, dt0, 2, basis=basis), vortex_ic, _periodic_bc
 
 
 # ------------------------------------------------------------------------------- 4-state Riemann
@@ -127,9 +140,9 @@ def riemann4_ic(param, x, y):
     return primitive_to_conservative(param.equation.gamma, tuple(out))
 
 
-def riemann4_2d(K=256, N=3, T=0.25, CFL=0.5, dt0=None):
+def riemann4_2d(K=256, N=3, T=0.25, CFL=0.5, dt0=None, basis="lobatto"):
     dt0 = RECORDED_DT0["riemann4_2d"] if dt0 is None else dt0
-    return _param(N, (K, K), (0.0, 0.0), (1.0, 1.0), T, CFL, dt0, 2), riemann4_ic, _transmissive_bc
+    return _param(N, (K, K), (0.0, 0.0), (1.0, 1.0), T, CFL, dt0, 2, basis=basis), riemann4_ic, _transmissive_bc
 
 
 # ------------------------------------------------------------------------------- Kelvin-Helmholtz
@@ -142,9 +155,15 @@ def kh_ic(param, x, y):
     return primitive_to_conservative(param.equation.gamma, (rho, u, v, p))
 
 
-def kelvin_helmholtz(K=512, N=4, T=10.0, CFL=0.5, dt0=None):
+def kelvin_helmholtz(K=512, N=4, T=10.0, CFL=0.5, dt0=None, basis="lobatto"):
     dt0 = RECORDED_DT0["kelvin_helmholtz"] if dt0 is None else dt0
-    return _param(N, (K, K), (-1.0, -1.0), (1.0, 1.0), T, CFL, dt0, 2), kh_ic, _periodic_bc
+    return _param(N, (K, K), (-1.0, -1.0), (1.0, 1.0), T, CFL, dt0, 2, basis=basis), kh_ic, _periodic_bc
+
+
+def kelvin_helmholtz_gauss(K=80, N=3, T=10.0, CFL=0.5, dt0=1e-3, basis="gauss"):
+    """The reference's example as written (examples/2D/kelvin-helmholtz.jl:43-55): 80x80, N=3,
+    GaussCollocation + NodewiseScaledExtrapolation, zeta 0.1, eta 1.0, CFL 0.5, dt0 1e-3."""
+    return _param(N, (K, K), (-1.0, -1.0), (1.0, 1.0), T, CFL, dt0, 2, basis=basis), kh_ic, _periodic_bc
 
 
 # ------------------------------------------------------------------------------- reference smoke
@@ -164,9 +183,10 @@ def smoke_ic(param, x, y):
     return primitive_to_conservative(param.equation.gamma, smoke_exact_prim(x, y, param.timestepping_param.t0))
 
 
-def smoke_vortex(N=3, K=5, T=2e-2, CFL=1.0, dt0=1e-2):
+def smoke_vortex(N=3, K=5, T=2e-2, CFL=1.0, dt0=1e-2, basis="lobatto"):
     """test/test_smoke.jl:117-129 with SubcellLimiter(PositivityBound)."""
-    return _param(N, (K, K), (0.0, 0.0), (10.0, 10.0), T, CFL, dt0, 2, zeta=0.1, eta=0.5), smoke_ic, _periodic_bc
+    return (_param(N, (K, K), (0.0, 0.0), (10.0, 10.0), T, CFL, dt0, 2, zeta=0.1, eta=0.5, basis=basis),
+            smoke_ic, _periodic_bc)
 
 
 # ------------------------------------------------------------------------------- batched workload
@@ -211,8 +231,9 @@ def periodic_wave_ic(param, x, y, kx=1, ky=1, u0=0.0, v0=0.0, dip=None, sigma=0.
     return primitive_to_conservative(g, (rho, np.full_like(x, u0), np.full_like(x, v0), np.ones_like(x)))
 
 
-def periodic_wave(Kx=32, Ky=32, N=3, kx=1, ky=1, u0=0.0, v0=0.0, dip=None, T=1.0, CFL=0.5, dt0=1e-4):
-    param = _param(N, (Kx, Ky), (0.0, 0.0), (1.0, 1.0), T, CFL, dt0, 2)
+def periodic_wave(Kx=32, Ky=32, N=3, kx=1, ky=1, u0=0.0, v0=0.0, dip=None, T=1.0, CFL=0.5, dt0=1e-4,
+                  basis="lobatto"):
+    param = _param(N, (Kx, Ky), (0.0, 0.0), (1.0, 1.0), T, CFL, dt0, 2, basis=basis)
     return param, (lambda prm, x, y: periodic_wave_ic(prm, x, y, kx, ky, u0, v0, dip)), _periodic_bc
 
 
@@ -256,6 +277,7 @@ CONFIGS: Dict[str, Callable] = {
     "vortex_2d": vortex_2d,
     "riemann4_2d": riemann4_2d,
     "kelvin_helmholtz": kelvin_helmholtz,
+    "kelvin_helmholtz_gauss": kelvin_helmholtz_gauss,
     "smoke_vortex": smoke_vortex,
     "periodic_wave": periodic_wave,
 }

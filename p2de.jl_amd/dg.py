@@ -17,8 +17,8 @@ from typing import Callable, Optional
 import numpy as np
 
 from .operators import (BCData, MeshData, Operators, RefElem, SizeData, build_operators,
-                        lobatto_ref_elem, uniform_mesh)
-from .types import Param, check_supported
+                        gauss_ref_elem, lobatto_ref_elem, uniform_mesh)
+from .types import GaussCollocation, NodewiseScaledExtrapolation, Param, check_supported
 
 
 @dataclass
@@ -97,8 +97,12 @@ def initialize_DG(param: Param, initial_condition: Callable, initial_boundary_co
     """
     check_supported(param)
     Nd = param.equation.Nd
-    rd = lobatto_ref_elem(param.N, Nd)
-    ops, sizes = build_operators(rd, param.global_constants.ZEROTOL)
+    if isinstance(param.approximation_basis, GaussCollocation):    # init.jl:66-72, 82-95
+        rd = gauss_ref_elem(param.N, Nd)
+        ops, sizes = build_operators(rd, param.global_constants.ZEROTOL, quad="gauss")
+    else:                                                          # init.jl:74-80, 97-103
+        rd = lobatto_ref_elem(param.N, Nd)
+        ops, sizes = build_operators(rd, param.global_constants.ZEROTOL)
     md = uniform_mesh(rd, param.K, param.xL, param.xR, periodic=False)
     sizes.K = param.num_elements
     bcdata = initial_boundary_conditions(param, md)
@@ -181,6 +185,8 @@ class FlatProblem:
     Ival: np.ndarray
     mapO: np.ndarray
     upper_bound_mode: int = 0
+    basis: int = 0                  # 0 LobattoCollocation, 1 GaussCollocation
+    entropyproj_limiter: int = 0    # 0 NoEntropyProjectionLimiter, 1 NodewiseScaledExtrapolation
 
 
 def flat_problem(solver: Solver, state_param: StateParam, upper_bound_mode: int = 0) -> FlatProblem:
@@ -216,4 +222,6 @@ def flat_problem(solver: Solver, state_param: StateParam, upper_bound_mode: int 
         mapI=np.ascontiguousarray(np.asarray(bc.mapI, dtype=np.int64)),
         Ival=np.ascontiguousarray(Ival).ravel(),
         mapO=np.ascontiguousarray(np.asarray(bc.mapO, dtype=np.int64)),
-        upper_bound_mode=int(upper_bound_mode))
+        upper_bound_mode=int(upper_bound_mode),
+        basis=1 if isinstance(p.approximation_basis, GaussCollocation) else 0,
+        entropyproj_limiter=1 if isinstance(p.entropyproj_limiter, NodewiseScaledExtrapolation) else 0)

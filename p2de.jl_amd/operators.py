@@ -2,7 +2,8 @@
 
 Host-side set-up, computed once in FP64 with numpy.  It restates what the reference
 obtains from StartUpDG/NodesAndModes in ``initialize_operators``
-(src/dg/init.jl:133-228) for the Lobatto (SBP) collocation basis:
+(src/dg/init.jl:133-228) for the Lobatto (SBP) collocation basis and for the Gauss
+collocation basis (``construct_gauss_reference_data``, init.jl:105-131):
 
 * 1D Legendre-Gauss-Lobatto nodes/weights and the Lagrange differentiation matrix;
 * 2D tensor-product nodes, r index fastest (init.jl:87, limiter_utils.jl:97-120);
@@ -65,9 +66,30 @@ def lgl_nodes_weights(N: int) -> Tuple[np.ndarray, np.ndarray]:
 
 
 def gauss_nodes_weights(N: int) -> Tuple[np.ndarray, np.ndarray]:
-    """Legendre-Gauss nodes and weights (N+1 points)."""
+    """Legendre-Gauss nodes (ascending) and weights, N+1 points: ``gauss_quad(0, 0, N)``.
+
+    numpy's ``leggauss`` (Golub-Welsch eigenvalues of the Jacobi matrix, one Newton step on
+    P_{N+1}); symmetrised so x_i = -x_{N-i} and w_i = w_{N-i} exactly.
+    """
     x, w = np.polynomial.legendre.leggauss(N + 1)
-    return np.asarray(x, dtype=np.float64), np.asarray(w, dtype=np.float64)
+    x = np.asarray(x, dtype=np.float64)
+    w = np.asarray(w, dtype=np.float64)
+    x = 0.5 * (x - x[::-1])
+    if (N + 1) % 2 == 1:
+        x[N // 2] = 0.0
+    w = 0.5 * (w + w[::-1])
+    return x, w
+
+
+def lagrange_interp_row(x: np.ndarray, xi: float) -> np.ndarray:
+    """Row l_a(xi), a = 0..n-1, of the Lagrange basis on nodes x (barycentric form)."""
+    n = len(x)
+    lam = np.array([1.0 / np.prod([x[j] - x[k] for k in range(n) if k != j]) for j in range(n)])
+    d = xi - x
+    if np.any(d == 0.0):
+        return (d == 0.0).astype(np.float64)
+    t = lam / d
+    return t / np.sum(t)
 
 
 def lagrange_diff_matrix(x: np.ndarray) -> np.ndarray:
@@ -159,6 +181,60 @@ def lobatto_ref_elem(N: int, Nd: int) -> RefElem:
     return RefElem(2, N, r, s, wq, wf, nrJ, nsJ, (Dr, Ds), Vf, r1D, w1D)
 
 
+def gauss_ref_elem(N: int, Nd: int) -> RefElem:
+    """``construct_gauss_reference_data`` (init.jl:105-131) of the Gauss-collocation element
+    (init.jl:66-72 in 1D, :82-95 in 2D): Lagrange basis on the N+1 Legendre-Gauss nodes per
+    direction (r fastest), Vq = Pq = I, M = diag(wq), Drst = the Gauss-node differentiation
+    matrices, Vf = interpolation along each line to the face points, which are the Gauss
+    points of the faces (``quad_rule_face=(r1D, w1D)``).
+
+    StartUpDG forms these through the LGL nodal basis (Vf = rd.Vf * inv(Vq) etc.); the
+    matrices are the same in exact arithmetic and are computed here directly.
+    """
+    r1D, w1D = gauss_nodes_weights(N)
+    D1D = lagrange_diff_matrix(r1D)
+    lm = lagrange_interp_row(r1D, -1.0)
+    lp = lagrange_interp_row(r1D, 1.0)
+    N1D = N + 1
+    if Nd == 1:
+        Vf = np.vstack([lm, lp])
+        return RefElem(1, N, r1D.copy(), None, w1D.copy(), np.ones(2), np.array([-1.0, 1.0]), None,
+                       (D1D,), Vf, r1D, w1D)
+    Nq = N1D * N1D
+    idx = np.arange(Nq)
+    i_r, j_s = idx % N1D, idx // N1D
+    r, s = r1D[i_r], r1D[j_s]
+    wq = w1D[i_r] * w1D[j_s]
+    I = np.eye(N1D)
+    Dr = np.kron(I, D1D)
+    Ds = np.kron(D1D, I)
+    Nfp = 4 * N1D
+    Vf = np.zeros((Nfp, Nq))
+    a = np.arange(N1D)
+    for m in range(N1D):
+        Vf[m, m * N1D + a] = lm                   # r = -1: the r-line at s index m
+        Vf[N1D + m, m * N1D + a] = lp             # r = +1
+        Vf[2 * N1D + m, a * N1D + m] = lm         # s = -1: the s-line at r index m
+        Vf[3 * N1D + m, a * N1D + m] = lp         # s = +1
+    e = np.ones(N1D)
+    z = np.zeros(N1D)
+    nrJ = np.concatenate([-e, e, z, z])
+    nsJ = np.concatenate([z, z, -e, e])
+    wf = np.concatenate([w1D, w1D, w1D, w1D])
+    return RefElem(2, N, r, s, wq, wf, nrJ, nsJ, (Dr, Ds), Vf, r1D, w1D)
+
+
+def gauss_low_order_extrapolation(N: int, Nd: int) -> np.ndarray:
+    """``Vf_low`` of the Gauss element: 1.0 at the line-end node of each face point
+    (init.jl:324-336 in 2D; ``construct_low_order_operators_1D`` init.jl:288-290 in 1D)."""
+    N1D = N + 1
+    fmap = face_node_to_volume_node(N1D, Nd)
+    Nq = N1D ** Nd
+    V = np.zeros((len(fmap), Nq))
+    V[np.arange(len(fmap)), fmap] = 1.0
+    return V
+
+
 # ---------------------------------------------------------------------------------
 # operators (init.jl:133-228, 276-322)
 # ---------------------------------------------------------------------------------
@@ -209,8 +285,9 @@ def low_order_Q01D(N: int, ZEROTOL: float) -> np.ndarray:
     return droptol(Q, ZEROTOL)
 
 
-def build_operators(rd: RefElem, ZEROTOL: float) -> Tuple[Operators, SizeData]:
-    """``initialize_operators`` minus the mesh (init.jl:141-220)."""
+def build_operators(rd: RefElem, ZEROTOL: float, quad: str = "lobatto") -> Tuple[Operators, SizeData]:
+    """``initialize_operators`` minus the mesh (init.jl:141-220); ``quad`` is the
+    quadrature the low-order extrapolation dispatches on (init.jl:302-340)."""
     Nd = rd.Nd
     Nq = len(rd.wq)
     Nfp = rd.Vf.shape[0]
@@ -248,7 +325,10 @@ def build_operators(rd: RefElem, ZEROTOL: float) -> Tuple[Operators, SizeData]:
         Qr0 = droptol(np.kron(M1D, Q01D), ZEROTOL)
         Qs0 = droptol(np.kron(Q01D, M1D), ZEROTOL)
         Srs0 = (droptol(0.5 * (Qr0 - Qr0.T), ZEROTOL), droptol(0.5 * (Qs0 - Qs0.T), ZEROTOL))
-    Vf_low = droptol(Vf, ZEROTOL)
+    if quad == "gauss":
+        Vf_low = gauss_low_order_extrapolation(rd.N, Nd)
+    else:
+        Vf_low = droptol(Vf, ZEROTOL)
     Vf_low[np.abs(Vf_low - 1.0) < ZEROTOL] = 1.0          # init.jl:181-185
 
     Srsh_nnz = [(i + 1, j + 1) for j in range(Nh) for i in range(j + 1, Nh)

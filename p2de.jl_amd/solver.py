@@ -175,7 +175,8 @@ def _descriptors(fp: FlatProblem, upper_bound_mode: int = None, precision: int =
     prec = getattr(fp, "precision", PRECISION_FP64) if precision is None else precision
     cfg = _Config(Nd=fp.Nd, N=fp.N1D - 1, Kx=fp.Kx, Ky=fp.Ky, gamma=fp.gamma, zeta=fp.zeta, eta=fp.eta,
                   POSTOL=fp.POSTOL, ZEROTOL=fp.ZEROTOL, CFL=fp.CFL, dt0=fp.dt0, t0=fp.t0, T=fp.T,
-                  high_order_surface_flux=1, upper_bound_mode=int(mode), precision=int(prec))
+                  high_order_surface_flux=1, upper_bound_mode=int(mode), precision=int(prec),
+                  basis=int(getattr(fp, "basis", 0)), entropyproj_limiter=int(getattr(fp, "entropyproj_limiter", 0)))
     keep = {}
     for n in ("Srsh_db", "Srs0", "Brs", "MinvVhT", "MinvVfT", "Vf", "Vf_low", "wq", "Jq", "GJh", "Ival"):
         keep[n] = np.ascontiguousarray(getattr(fp, n), dtype=np.float64)

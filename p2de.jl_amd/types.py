@@ -188,9 +188,11 @@ def check_supported(param: Param) -> None:
     """Reject option combinations outside the implemented hot path.
 
     Implemented: ``ESLimitedLowOrderPos(LaxFriedrichsOnNodalVal, LaxFriedrichsOnProjectedVal)``
-    with ``LobattoCollocation`` + ``NoEntropyProjectionLimiter`` +
-    ``SubcellLimiter(PositivityBound, NoShockCapture)`` for 1D/2D Euler -- the combination
-    test/test_smoke.jl:107-108 exercises.
+    + ``SubcellLimiter(PositivityBound, NoShockCapture)`` for 1D/2D Euler with
+    ``LobattoCollocation`` + ``NoEntropyProjectionLimiter`` (the combination
+    test/test_smoke.jl:107-108 exercises) or ``GaussCollocation`` with
+    ``NodewiseScaledExtrapolation`` (examples/2D/kelvin-helmholtz.jl:50-55) or
+    ``NoEntropyProjectionLimiter``.
     """
     rhs = param.rhs
     if not isinstance(rhs, LimitedDG):
@@ -201,10 +203,16 @@ def check_supported(param: Param) -> None:
         raise UnsupportedOption("low_order_surface_flux must be LaxFriedrichsOnNodalVal")
     if not isinstance(rhs.high_order_surface_flux, LaxFriedrichsOnProjectedVal):
         raise UnsupportedOption("high_order_surface_flux must be LaxFriedrichsOnProjectedVal")
-    if not isinstance(param.approximation_basis, LobattoCollocation):
-        raise UnsupportedOption("approximation_basis must be LobattoCollocation")
-    if not isinstance(param.entropyproj_limiter, NoEntropyProjectionLimiter):
-        raise UnsupportedOption("entropyproj_limiter must be NoEntropyProjectionLimiter")
+    basis = param.approximation_basis
+    if not isinstance(basis, (LobattoCollocation, GaussCollocation)):
+        raise UnsupportedOption(f"approximation_basis {basis!r}: LobattoCollocation or GaussCollocation")
+    epl = param.entropyproj_limiter
+    if isinstance(basis, LobattoCollocation) and not isinstance(epl, NoEntropyProjectionLimiter):
+        # the Lobatto method of compute_entropyproj_limiting_param! (filter.jl:1-3) does not run
+        raise UnsupportedOption("LobattoCollocation takes NoEntropyProjectionLimiter")
+    if not isinstance(epl, (NoEntropyProjectionLimiter, NodewiseScaledExtrapolation)):
+        raise UnsupportedOption(f"entropyproj_limiter {epl!r}: NoEntropyProjectionLimiter or "
+                                "NodewiseScaledExtrapolation")
     lim = param.rhs_limiter
     if not isinstance(lim, SubcellLimiter):
         raise UnsupportedOption(f"rhs_limiter {lim!r}: only SubcellLimiter is implemented")

@@ -38,7 +38,7 @@ def _create(fp, **cfg_over):
     """Call p2de_create like DeviceSolver does, with config overrides; return (status, msg)."""
     cfg = S._Config(Nd=fp.Nd, N=fp.N1D - 1, Kx=fp.Kx, Ky=fp.Ky, gamma=fp.gamma, zeta=fp.zeta, eta=fp.eta,
                     POSTOL=fp.POSTOL, ZEROTOL=fp.ZEROTOL, CFL=fp.CFL, dt0=fp.dt0, t0=fp.t0, T=fp.T,
-                    high_order_surface_flux=1)
+                    high_order_surface_flux=1, basis=fp.basis, entropyproj_limiter=fp.entropyproj_limiter)
     for k, v in cfg_over.items():
         setattr(cfg, k, v)
     keep = {n: np.ascontiguousarray(getattr(fp, n), dtype=np.float64)

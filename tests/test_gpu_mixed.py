@@ -203,8 +203,8 @@ def test_single_context_mixed_precision_is_the_one_instance_batch():
     dt_d, r_d = dev.rhs(0.0, 0.9 * fp.CFL * fp.dt0, 1)
     dt_b, r_b = bs.rhs(0, 0.0, 0.9 * fp.CFL * fp.dt0, 1)
     assert dt_d == dt_b and np.array_equal(r_d, r_b)
-    assert np.array_equal(dev.field(FIELD_RHSH), bs.field(0, FIELD_RHSH))
-    assert np.array_equal(dev.field(FIELD_RHSL), bs.field(0, FIELD_RHSL))
+    assert np.array_equal(dev.field(FIELD_RHSH).ravel(), bs.field(0, FIELD_RHSH).ravel())
+    assert np.array_equal(dev.field(FIELD_RHSL).ravel(), bs.field(0, FIELD_RHSL).ravel())
     t, dt = dev.step(0.0)
     bs.set_time(0.0)
     bs.enqueue_steps(1)
