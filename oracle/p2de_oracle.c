@@ -220,6 +220,8 @@ struct oracle_ctx {
     double *blending_factor; /* (K, 3) */
     double *smooth_factor;   /* (K, 3) */
     double *dtarr;
+    /* EntropyProjectionLimiterCache (State.jl:87-129) and theta_local (State.jl:20) */
+    double *epUf, *epVUf, *eprhoef, *theta_local;
     int nthreads_alloc;
     double *stage_buf;       /* rhsU copy for the RK driver */
 };
@@ -241,6 +243,7 @@ int oracle_create(const oracle_problem *p, oracle_ctx **out) {
     if (!p || !out) return 1;
     if (p->Nd != 1 && p->Nd != 2) return 2;
     if (p->Nc != p->Nd + 2) return 2;
+    if (p->entropyproj_limiter != 0 && p->entropyproj_limiter != 1) return 2;
     oracle_ctx *c = (oracle_ctx *)calloc(1, sizeof(oracle_ctx));
     if (!c) return 3;
     c->p = *p;
@@ -294,6 +297,11 @@ int oracle_create(const oracle_problem *p, oracle_ctx **out) {
     c->smooth_factor = zalloc(K * 3);
     c->dtarr = zalloc(K);
     c->stage_buf = zalloc(Nq * K * Nc);
+    c->epUf = zalloc(Nfp * K * Nc);
+    c->epVUf = zalloc(Nfp * K * Nc);
+    c->eprhoef = zalloc(Nfp * K);
+    c->theta_local = zalloc(Nfp * K * 3);
+    for (int64_t x = 0; x < Nfp * K * 3; x++) c->theta_local[x] = 1.0;
     *out = c;
     return 0;
 }
@@ -307,7 +315,7 @@ void oracle_destroy(oracle_ctx *c) {
                     &c->uPH, &c->betaP, &c->rhologP, &c->betalogP, &c->lam, &c->LFc, &c->QF1,
                     &c->MinvVhTQF1, &c->MinvVfTBF1, &c->f_bar_H, &c->f_bar_L, &c->f_bar_lim,
                     &c->lbound_s_modified, &c->uL, &c->blending_factor, &c->smooth_factor,
-                    &c->dtarr, &c->stage_buf};
+                    &c->dtarr, &c->stage_buf, &c->epUf, &c->epVUf, &c->eprhoef, &c->theta_local};
     for (size_t i = 0; i < sizeof(f) / sizeof(f[0]); i++) free(*f[i]);
     free(c);
 }
@@ -326,6 +334,7 @@ int64_t oracle_field_size(const oracle_ctx *c, int field) {
     case ORACLE_LAMBDAB: return Nfp * K;
     case ORACLE_FBAR_H: case ORACLE_FBAR_L: return Nd * (int64_t)c->NLs * K * Nc;
     case ORACLE_PSI_TILDE: return Nh * K * Nd;
+    case ORACLE_THETA_LOCAL: return Nfp * K * 3;
     default: return -1;
     }
 }
@@ -350,6 +359,7 @@ const double *oracle_field(const oracle_ctx *c, int field) {
     case ORACLE_PSI_TILDE: return c->psi_tilde;
     case ORACLE_FSTAR_H: return c->fstar_H;
     case ORACLE_FSTAR_L: return c->fstar_L;
+    case ORACLE_THETA_LOCAL: return c->theta_local;
     default: return NULL;
     }
 }
@@ -406,15 +416,108 @@ static inline double Sx0_with_n(const oracle_ctx *c, int i, int j, int k, double
 }
 
 /* ------------------------------------------------------------------------------- */
-/* entropy_projection!(::NoEntropyProjectionLimiter) rhs.jl:59-110                   */
+/* Entropy-projection limiter: init_rhs!(::ScaledExtrapolation) rhs.jl:19-23,        */
+/* compute_entropyproj_limiting_param!(::GaussCollocation) filter.jl:6-15            */
 /* ------------------------------------------------------------------------------- */
-static void entropy_projection(oracle_ctx *c, const double *Uq, int nthr) {
+#define THETA(i, k, s) (c->theta_local[(int64_t)(i) + (int64_t)c->p.Nfp * ((int64_t)(k) + (int64_t)c->p.K * (s))])
+
+/* calc_face_values!(::CompressibleEulerIdealGas) filter.jl:26-42; the two mul! calls are the
+ * generic matrix-vector product, which adds column by column from zero */
+static void calc_face_values(oracle_ctx *c, const double *Uq, int nthr) {
+    const oracle_problem *p = &c->p;
+    int K = p->K, Nq = p->Nq, Nfp = p->Nfp, Nc = p->Nc, Nd = p->Nd;
+    double g = p->gamma;
+    OMP_FOR
+    for (int k = 0; k < K; k++) {
+        for (int i = 0; i < Nfp; i++) {
+            double *uf = c->epUf + IDX_V(i, k, Nfp, Nc);
+            for (int cc = 0; cc < Nc; cc++) uf[cc] = 0.0;
+            for (int j = 0; j < Nq; j++) {
+                double a = p->Vf[i + (int64_t)Nfp * j];
+                for (int cc = 0; cc < Nc; cc++) uf[cc] += a * Uq[IDX_V(j, k, Nq, Nc) + cc];
+            }
+        }
+        for (int i = 0; i < Nq; i++) v_ufun(Nd, g, Uq + IDX_V(i, k, Nq, Nc), c->vq + IDX_V(i, k, Nq, Nc));
+        for (int i = 0; i < Nfp; i++) c->eprhoef[i + (int64_t)Nfp * k] = rhoe_ufun(Nd, c->epUf + IDX_V(i, k, Nfp, Nc));
+        for (int i = 0; i < Nfp; i++) {
+            double *vf = c->epVUf + IDX_V(i, k, Nfp, Nc);
+            for (int cc = 0; cc < Nc; cc++) vf[cc] = 0.0;
+            for (int j = 0; j < Nq; j++) {
+                double a = p->Vf[i + (int64_t)Nfp * j];
+                for (int cc = 0; cc < Nc; cc++) vf[cc] += a * c->vq[IDX_V(j, k, Nq, Nc) + cc];
+            }
+        }
+    }
+}
+
+/* projected entropy variables of face node i with parameter l (rhs.jl:87-92, filter.jl:118-121):
+ * v = sum_j (l*Vf[i,j] + (1-l)*Vf_low[i,j]) vq_j, from zero, j ascending */
+static void project_face_v(const oracle_ctx *c, int i, int k, double l, double *vt) {
+    const oracle_problem *p = &c->p;
+    int Nq = p->Nq, Nfp = p->Nfp, Nc = p->Nc;
+    for (int cc = 0; cc < Nc; cc++) vt[cc] = 0.0;
+    for (int j = 0; j < Nq; j++) {
+        double coef = l * p->Vf[i + (int64_t)Nfp * j] + (1.0 - l) * p->Vf_low[i + (int64_t)Nfp * j];
+        for (int cc = 0; cc < Nc; cc++) vt[cc] += coef * c->vq[IDX_V(j, k, Nq, Nc) + cc];
+    }
+}
+
+/* update_and_check_bound_limited_entropyproj_var_on_face_node! filter.jl:101-107 with
+ * update_limited_entropyproj_vars_on_face_node! :109-130 and check_bound_on_face_node :87-98 */
+static int theta_admissible(const oracle_ctx *c, int i, int k, double theta) {
+    const oracle_problem *p = &c->p;
+    int Nfp = p->Nfp, Nc = p->Nc, Nd = p->Nd;
+    double eps = p->POSTOL, zeta = p->zeta, eta = p->eta;
+    double vt[NC_MAX], ut[NC_MAX];
+    project_face_v(c, i, k, theta, vt);
+    if (!(vt[Nc - 1] < -eps)) return 0;                   /* well-definedness :123 */
+    u_vfun(Nd, p->gamma, vt, ut);
+    double v3tilde = vt[Nc - 1], rhotilde = ut[0], rhoetilde = rhoe_ufun(Nd, ut);
+    const double *Uf = c->epUf + IDX_V(i, k, Nfp, Nc), *VUf = c->epVUf + IDX_V(i, k, Nfp, Nc);
+    double rhoef = c->eprhoef[i + (int64_t)Nfp * k];
+    return v3tilde < jmin(zeta * VUf[Nc - 1], -eps) && rhotilde > jmax((1.0 - eta) * Uf[0], eps) &&
+           rhotilde < (1.0 + eta) * Uf[0] && rhoetilde > jmax((1.0 - eta) * rhoef, eps) &&
+           rhoetilde < (1.0 + eta) * rhoef;
+}
+
+/* solve_theta!(::NodewiseScaledExtrapolation) filter.jl:49-59 with bisection
+ * nonlinear_solvers.jl:3-20 (x_valid = 0, x_invalid = 1, iterations 0..20) */
+static void compute_entropyproj_limiting_param(oracle_ctx *c, const double *Uq, int nstage, int nthr) {
+    const oracle_problem *p = &c->p;
+    int K = p->K, Nfp = p->Nfp;
+    int s = nstage - 1;
+    /* clear_entropyproj_limiting_parameter_cache! :18-20 */
+    for (int k = 0; k < K; k++)
+        for (int i = 0; i < Nfp; i++) THETA(i, k, s) = 1.0;
+    calc_face_values(c, Uq, nthr);
+    OMP_FOR
+    for (int k = 0; k < K; k++)
+        for (int i = 0; i < Nfp; i++) {
+            double x_valid = 0.0, x_invalid = 1.0, th;
+            if (theta_admissible(c, i, k, x_invalid)) {
+                th = x_invalid;
+            } else {
+                for (int iter = 0; iter <= 20; iter++) {
+                    double x_new = 0.5 * (x_valid + x_invalid);
+                    if (theta_admissible(c, i, k, x_new)) x_valid = x_new;
+                    else x_invalid = x_new;
+                }
+                th = x_valid;
+            }
+            THETA(i, k, s) = th;
+        }
+}
+
+/* ------------------------------------------------------------------------------- */
+/* entropy_projection! rhs.jl:59-133 (NoEntropyProjectionLimiter: l = 1;             */
+/* NodewiseScaledExtrapolation: l = theta_local[i, k, nstage])                        */
+/* ------------------------------------------------------------------------------- */
+static void entropy_projection(oracle_ctx *c, const double *Uq, int nstage, int nthr) {
     const oracle_problem *p = &c->p;
     int K = p->K, Nq = p->Nq, Nfp = p->Nfp, Nh = p->Nh, Nc = p->Nc, Nd = p->Nd;
     double g = p->gamma;
     OMP_FOR
     for (int k = 0; k < K; k++) {
-        double l = 1.0;
         /* calculate_entropy_var! :72-77 */
         for (int i = 0; i < Nq; i++) v_ufun(Nd, g, Uq + IDX_V(i, k, Nq, Nc), c->vq + IDX_V(i, k, Nq, Nc));
         /* entropy_projection_volume_node! :79-82 */
@@ -425,12 +528,9 @@ static void entropy_projection(oracle_ctx *c, const double *Uq, int nthr) {
             }
         /* entropy_projection_face_node! :84-94 */
         for (int i = 0; i < Nfp; i++) {
+            double l = p->entropyproj_limiter == 1 ? THETA(i, k, nstage - 1) : 1.0;
             double *vt = c->v_tilde + IDX_V(i + Nq, k, Nh, Nc);
-            for (int cc = 0; cc < Nc; cc++) vt[cc] = 0.0;
-            for (int j = 0; j < Nq; j++) {
-                double coef = l * p->Vf[i + (int64_t)Nfp * j] + (1.0 - l) * p->Vf_low[i + (int64_t)Nfp * j];
-                for (int cc = 0; cc < Nc; cc++) vt[cc] += coef * c->vq[IDX_V(j, k, Nq, Nc) + cc];
-            }
+            project_face_v(c, i, k, l, vt);
             u_vfun(Nd, g, vt, c->u_tilde + IDX_V(i + Nq, k, Nh, Nc));
         }
     }
@@ -610,7 +710,7 @@ static inline void Uvol(const oracle_ctx *c, int idx, int k, double *out) {
     }
 }
 
-static void rhs_fluxdiff(oracle_ctx *c, const double *Uq, int nthr) {
+static void rhs_fluxdiff(oracle_ctx *c, const double *Uq, int nstage, int nthr) {
     const oracle_problem *p = &c->p;
     int K = p->K, Nq = p->Nq, Nfp = p->Nfp, Nh = p->Nh, Nc = p->Nc, Nd = p->Nd, N1D = p->N1D;
     double g = p->gamma;
@@ -716,19 +816,44 @@ static void rhs_fluxdiff(oracle_ctx *c, const double *Uq, int nthr) {
                 fs[ld * Nc + cc] = fs[ld * Nc + cc] - lf / B[ld];
             }
         }
-    /* assemble_rhs! :331-361 (project_flux_difference_to_quad_unlimited! :274-281: generic mul!) */
+    /* assemble_rhs! :331-361: generic mul! with MinvVhT, MinvVfT
+     * (project_flux_difference_to_quad_unlimited! :274-281), or, on an element whose
+     * entropy-projection parameters are limited (is_Vf_limited :326-328), with
+     * MinvVhT_new = (1 ./ wq) .* [I Vf_new'] and Vf_new = l Vf + (1-l) Vf_low row by row
+     * (project_flux_difference_to_quad!(::ScaledExtrapolation) :288-308, update_limited_extrapolation! :310-319) */
     OMP_FOR
     for (int k = 0; k < K; k++) {
+        int limited = 0;
+        if (p->entropyproj_limiter == 1) {
+            double m = THETA(0, k, nstage - 1);
+            for (int f = 1; f < Nfp; f++) m = jmin(m, THETA(f, k, nstage - 1));
+            limited = m < 1.0;
+        }
         for (int i = 0; i < Nq; i++) {
             double A[ND_MAX * NC_MAX], Bv[ND_MAX * NC_MAX];
             for (int x = 0; x < Nd * Nc; x++) { A[x] = 0.0; Bv[x] = 0.0; }
+            double rw = 1.0 / p->wq[i];
             for (int j = 0; j < Nh; j++) {
-                double a = p->MinvVhT[i + (int64_t)Nq * j];
+                double a;
+                if (!limited) {
+                    a = p->MinvVhT[i + (int64_t)Nq * j];
+                } else if (j < Nq) {
+                    a = rw * (j == i ? 1.0 : 0.0);
+                } else {
+                    double l = THETA(j - Nq, k, nstage - 1);
+                    a = rw * (l * p->Vf[(j - Nq) + (int64_t)Nfp * i] + (1.0 - l) * p->Vf_low[(j - Nq) + (int64_t)Nfp * i]);
+                }
                 const double *q = c->QF1 + IDX_XY(0, j, k, Nh, Nd, Nc);
                 for (int x = 0; x < Nd * Nc; x++) A[x] += a * q[x];
             }
             for (int j = 0; j < Nfp; j++) {
-                double a = p->MinvVfT[i + (int64_t)Nq * j];
+                double a;
+                if (!limited) {
+                    a = p->MinvVfT[i + (int64_t)Nq * j];
+                } else {
+                    double l = THETA(j, k, nstage - 1);
+                    a = rw * (l * p->Vf[j + (int64_t)Nfp * i] + (1.0 - l) * p->Vf_low[j + (int64_t)Nfp * i]);
+                }
                 const double *q = c->BF_H + IDX_XY(0, j, k, Nfp, Nd, Nc);
                 for (int x = 0; x < Nd * Nc; x++) Bv[x] += a * q[x];
             }
@@ -1071,9 +1196,11 @@ static void apply_rhs_limiter(oracle_ctx *c, const double *Uq, double dt, int ns
 int oracle_rhs(oracle_ctx *c, const double *Uq, double t, double dt, int nstage, double *dt_out) {
     if (!c || !Uq || nstage < 1 || nstage > 3) return 1;
     int nthr = oracle_get_threads();
-    entropy_projection(c, Uq, nthr);
+    /* init_rhs! rhs.jl:15-23 */
+    if (c->p.entropyproj_limiter == 1) compute_entropyproj_limiting_param(c, Uq, nstage, nthr);
+    entropy_projection(c, Uq, nstage, nthr);
     double dt_ret = rhs_low_graph_visc(c, Uq, t, dt, nstage, nthr);
-    rhs_fluxdiff(c, Uq, nthr);
+    rhs_fluxdiff(c, Uq, nstage, nthr);
     apply_rhs_limiter(c, Uq, dt, nstage, nthr);
     if (dt_out) *dt_out = dt_ret;
     return 0;

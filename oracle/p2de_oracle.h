@@ -9,10 +9,12 @@
  *   src/dg/rhs/{rhs,rhs_utils,flux_differencing,low_order_graph_viscosity}.jl
  *   src/dg/limiter/{limiter,limiter_utils,subcell,shock_capture}.jl
  *   src/math/compressible_Navier_Stokes.jl, src/timestepping/SSPRK33.jl
+ * and src/dg/filter.jl, src/math/nonlinear_solvers.jl
  * for the option set ESLimitedLowOrderPos(LaxFriedrichsOnNodalVal,
- * LaxFriedrichsOnProjectedVal) + NoEntropyProjectionLimiter +
- * SubcellLimiter(PositivityBound, NoShockCapture), with the operators passed in
- * (the reference reads them from Solver.discrete_data).
+ * LaxFriedrichsOnProjectedVal) + SubcellLimiter(PositivityBound, NoShockCapture) with
+ * NoEntropyProjectionLimiter (Lobatto or Gauss operators) or NodewiseScaledExtrapolation
+ * (Gauss operators), with the operators passed in (the reference reads them from
+ * Solver.discrete_data).
  *
  * All arrays are column-major, exactly as the Julia arrays they mirror; index arrays
  * hold the reference's 1-based values.
@@ -35,6 +37,10 @@ typedef struct {
      *    the 5 values built at subcell.jl:299, so Urho = Lphi = 0 for PositivityBound);
      * 1: the evident intent, Urho = Urhoe = Inf. */
     int upper_bound_mode;
+    /* Param.entropyproj_limiter: 0 NoEntropyProjectionLimiter, 1 NodewiseScaledExtrapolation
+     * (filter.jl:6-130, rhs.jl:113-133, flux_differencing.jl:288-328; Gauss basis only).  The
+     * basis itself enters through the operators: Vf, Vf_low, MinvVhT, MinvVfT, Srsh_db. */
+    int entropyproj_limiter;
     /* Operators (Solver.jl:192-212) */
     const double *Srsh_db;   /* Nd blocks of Nh x Nh */
     const double *Srs0;      /* Nd blocks of Nq x Nq */
@@ -84,6 +90,7 @@ enum {
     ORACLE_PSI_TILDE,  /* (Nh, K) x Nd */
     ORACLE_FSTAR_H,    /* (Nfp, K) x Nd x Nc */
     ORACLE_FSTAR_L,
+    ORACLE_THETA_LOCAL, /* (Nfp, K, 3) entropy-projection limiting parameters */
     ORACLE_NFIELDS
 };
 

@@ -128,10 +128,87 @@ def test_unsupported_options_are_rejected():
     T = PKG.types
     base = PKG.configs.vortex_2d(K=4)[0]
     from dataclasses import replace
-    for bad in (replace(base, approximation_basis=T.GaussCollocation()),
-                replace(base, rhs_limiter=T.ZhangShuLimiter()),
+    for bad in (replace(base, rhs_limiter=T.ZhangShuLimiter()),
                 replace(base, rhs_limiter=T.SubcellLimiter(shockcapture=T.HennemannShockCapture())),
                 replace(base, entropyproj_limiter=T.NodewiseScaledExtrapolation()),
                 replace(base, N=5)):
         with pytest.raises(T.UnsupportedOption):
             T.check_supported(bad)
+
+
+# --------------------------------------------------------------------------- Gauss basis
+GAUSS_CLOSED = {
+    1: ([-1 / math.sqrt(3), 1 / math.sqrt(3)], [1.0, 1.0]),
+    2: ([-math.sqrt(3 / 5), 0.0, math.sqrt(3 / 5)], [5 / 9, 8 / 9, 5 / 9]),
+    3: ([-math.sqrt(3 / 7 + 2 / 7 * math.sqrt(6 / 5)), -math.sqrt(3 / 7 - 2 / 7 * math.sqrt(6 / 5)),
+         math.sqrt(3 / 7 - 2 / 7 * math.sqrt(6 / 5)), math.sqrt(3 / 7 + 2 / 7 * math.sqrt(6 / 5))],
+        [(18 - math.sqrt(30)) / 36, (18 + math.sqrt(30)) / 36, (18 + math.sqrt(30)) / 36, (18 - math.sqrt(30)) / 36]),
+    4: ([-1 / 3 * math.sqrt(5 + 2 * math.sqrt(10 / 7)), -1 / 3 * math.sqrt(5 - 2 * math.sqrt(10 / 7)), 0.0,
+         1 / 3 * math.sqrt(5 - 2 * math.sqrt(10 / 7)), 1 / 3 * math.sqrt(5 + 2 * math.sqrt(10 / 7))],
+        [(322 - 13 * math.sqrt(70)) / 900, (322 + 13 * math.sqrt(70)) / 900, 128 / 225,
+         (322 + 13 * math.sqrt(70)) / 900, (322 - 13 * math.sqrt(70)) / 900]),
+}
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4])
+def test_gauss_nodes_weights_closed_form(N):
+    x, w = ops_mod.gauss_nodes_weights(N)
+    assert np.allclose(x, GAUSS_CLOSED[N][0], rtol=0, atol=4e-16)
+    assert np.allclose(w, GAUSS_CLOSED[N][1], rtol=0, atol=4e-16)
+    assert np.array_equal(x, -x[::-1]) and np.array_equal(w, w[::-1])
+    # exact for polynomials of degree 2N+1
+    for p in range(2 * N + 2):
+        assert abs(np.dot(w, x ** p) - (0.0 if p % 2 else 2.0 / (p + 1))) < 1e-14
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4])
+def test_interpolation_rows_exact_on_polynomials(N):
+    x, _ = ops_mod.gauss_nodes_weights(N)
+    for xi in (-1.0, 1.0, 0.3):
+        row = ops_mod.lagrange_interp_row(x, xi)
+        for p in range(N + 1):
+            assert abs(row @ x ** p - xi ** p) < 1e-14
+
+
+@pytest.mark.parametrize("Nd,N", [(1, 1), (1, 3), (1, 4), (2, 1), (2, 2), (2, 3), (2, 4)])
+def test_gauss_hybridized_operators(Nd, N):
+    """The Gauss element of init.jl:66-131 through build_operators: pair structure, the
+    hybridized SBP identities, the SBP property Q + Q^T = E^T B E of the Gauss collocation
+    operator, Vf exact on polynomials, Vf_low picking the line ends (init.jl:324-336)."""
+    rd = ops_mod.gauss_ref_elem(N, Nd)
+    ops, sz = ops_mod.build_operators(rd, 5e-16, quad="gauss")
+    N1D = N + 1
+    Nq, Nfp, Nh = sz.Nq, sz.Nfp, sz.Nh
+    nph = N1D * (N1D - 1) // 2 + 2 * N1D if Nd == 1 else N1D * N1D * (N1D - 1) + Nfp * N1D
+    npl = N1D - 1 if Nd == 1 else 2 * N1D * (N1D - 1)
+    assert len(ops.Srsh_nnz) == nph and len(ops.Srs0_nnz) == npl
+    Ef = ops.Vf
+    for d in range(Nd):
+        S = ops.Srsh_db[d]
+        assert np.array_equal(S, -S.T)
+        B = np.diag(ops.Brs[d])
+        assert np.allclose(S @ np.ones(Nh), -np.concatenate([np.zeros(Nq), B]), atol=1e-13)
+        Q = np.diag(rd.wq) @ rd.Drst[d]
+        assert np.allclose(Q + Q.T, Ef.T @ ops.Brs[d] @ Ef, atol=1e-13)
+        # the volume block of Srsh_db is Q - Q^T and the face block is E^T B
+        assert np.allclose(S[:Nq, :Nq], Q - Q.T, atol=1e-14)
+        assert np.allclose(S[:Nq, Nq:], Ef.T @ ops.Brs[d], atol=1e-14)
+    # Vf interpolates degree-N polynomials to the face points exactly
+    rf = np.array([-1.0, 1.0]) if Nd == 1 else None
+    for p in range(N + 1):
+        if Nd == 1:
+            assert np.allclose(Ef @ rd.r ** p, rf ** p, atol=1e-14)
+        else:
+            for qd in range(N + 1):
+                g = rd.r ** p * rd.s ** qd
+                e = np.ones(N1D)
+                fr = np.concatenate([-e, e, rd.r1D, rd.r1D])
+                fs = np.concatenate([rd.r1D, rd.r1D, -e, e])
+                assert np.allclose(Ef @ g, fr ** p * fs ** qd, atol=1e-13)
+    assert np.allclose(ops.MinvVhT, np.diag(1 / ops.wq) @ np.hstack([np.eye(Nq), Ef.T]), rtol=1e-15, atol=0)
+    assert np.allclose(ops.MinvVfT, np.diag(1 / ops.wq) @ Ef.T, rtol=1e-15, atol=0)
+    assert np.array_equal(np.sort(np.unique(ops.Vf_low)), [0.0, 1.0]) and np.all(ops.Vf_low.sum(1) == 1.0)
+    fmap = ops_mod.face_node_to_volume_node(N1D, Nd)
+    assert np.array_equal(ops.fq2q - 1, fmap)
+    for f, q in enumerate(fmap):
+        assert (f + 1) in ops.q2fq[q]
