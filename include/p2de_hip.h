@@ -1,6 +1,6 @@
 /*
  * p2de_hip.h -- C-ABI of the MI355X drop-in for P2DE.jl's limited ESDG right-hand side
- * and SSP-RK3 stage update (1D/2D compressible Euler, Lobatto collocation).
+ * and SSP-RK3 stage update (1D/2D compressible Euler, Lobatto or Gauss collocation).
  *
  * Every entry point is extern "C", takes plain pointers and sizes, never throws, and
  * returns a status (P2DE_OK = 0).  Host arrays are owned by the caller and only read
@@ -61,8 +61,9 @@ typedef struct {
     int low_order_surface_flux;   /* P2DE_LF_ON_NODAL_VAL */
     int high_order_surface_flux;  /* P2DE_LF_ON_PROJECTED_VAL */
     int high_order_volume_flux;   /* P2DE_CHANDRASHEKAR_FLUX */
-    int basis;                    /* P2DE_BASIS_LOBATTO */
-    int entropyproj_limiter;      /* P2DE_NO_ENTROPY_PROJ_LIMITER */
+    int basis;                    /* P2DE_BASIS_LOBATTO or P2DE_BASIS_GAUSS */
+    int entropyproj_limiter;      /* P2DE_NO_ENTROPY_PROJ_LIMITER; with Gauss also
+                                     P2DE_NODEWISE_SCALED_EXTRAPOLATION */
     int rhs_limiter;              /* P2DE_SUBCELL_LIMITER */
     int bound;                    /* P2DE_POSITIVITY_BOUND */
     int shockcapture;             /* P2DE_NO_SHOCK_CAPTURE */
@@ -128,7 +129,8 @@ enum {
     P2DE_FIELD_RHSU = 0,     /* (Nq, K) x Nc */
     P2DE_FIELD_RHSH = 1,     /* (Nq, K) x Nc */
     P2DE_FIELD_RHSL = 2,     /* (Nq, K) x Nc */
-    P2DE_FIELD_L_LOCAL = 3   /* (Nq+N1D, Nd, K) for the stage of the last call */
+    P2DE_FIELD_L_LOCAL = 3,  /* (Nq+N1D, Nd, K) for the stage of the last call */
+    P2DE_FIELD_THETA = 4     /* (Nfp, K) theta_local[:, :, nstage] (NodewiseScaledExtrapolation) */
 };
 
 /* Replaces initialize_DG's products as rhs!/SSP33! consume them (init.jl:44-60):
@@ -177,8 +179,9 @@ int p2de_enqueue_steps(p2de_ctx *ctx, int steps);
 /* Wait for all queued device work of the context. */
 int p2de_synchronize(p2de_ctx *ctx);
 /* Per-kernel timing with HIP events recorded on the context's stream around every
- * launch while enabled.  kinds: 0 = element RHS (phase A), 1 = symmetrise+update
- * (phase B), 2 = step bookkeeping.  ms[k] = total milliseconds, count[k] = launches. */
+ * launch while enabled.  kinds: 0 = element RHS (phase A; on the Gauss basis also the
+ * entropy projection of the face states), 1 = symmetrise+update (phase B), 2 = step
+ * bookkeeping.  ms[k] = total milliseconds, count[k] = launches. */
 int p2de_set_kernel_timing(p2de_ctx *ctx, int on);
 int p2de_get_kernel_timing(p2de_ctx *ctx, double ms[3], int64_t count[3]);
 

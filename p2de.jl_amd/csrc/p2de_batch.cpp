@@ -520,7 +520,7 @@ int p2de_batch_create(int n, const p2de_config* cfg, const p2de_operators* ops, 
                                  ops[i].q2fq, geom[i].Jq, geom[i].GJh, bc[i].mapP};
             for (const void* p : req)
                 if (!p) return bfail(b, P2DE_E_INVALID, "instance " + std::to_string(i) + ": NULL array"), bail(P2DE_E_INVALID);
-            if ((st = setup_tables(tabs[i], 2, I.N1D, &ops[i], e)))
+            if ((st = setup_tables(tabs[i], 2, I.N1D, P2DE_BASIS_LOBATTO, &ops[i], e)))
                 return bfail(b, st, "instance " + std::to_string(i) + ": " + e), bail(st);
             if ((st = setup_mesh(2, I.N1D, I.K, tabs[i], &geom[i], &bc[i], meshes[i], e)))
                 return bfail(b, st, "instance " + std::to_string(i) + ": " + e), bail(st);

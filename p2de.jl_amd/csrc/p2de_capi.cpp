@@ -43,7 +43,10 @@ struct p2de_ctx {
     double* d_Ival = nullptr;
     double* d_U[2] = {nullptr, nullptr};
     double *d_rhsUA = nullptr, *d_lA = nullptr, *d_dfA = nullptr, *d_rhsU = nullptr;
-    double* d_ut = nullptr;                // projected face states of d_U[0]
+    double* d_ut = nullptr;                // projected face states of d_U[0] (Lobatto) or of the stage input (Gauss)
+    double* d_theta = nullptr;             // entropy-projection parameters per face node (Gauss)
+    int basis = 0;                         // P2DE_BASIS_*
+    bool ep_limit = false;                 // NodewiseScaledExtrapolation
     double* d_dtp = nullptr;               // per-workgroup CFL minima of phase A (stage 1)
     bool ut_valid = false;                 // d_ut matches d_U[0]
     double *d_dbg_rhsH = nullptr, *d_dbg_rhsL = nullptr, *d_dbg_L = nullptr;
@@ -113,7 +116,8 @@ void free_ctx(p2de_ctx* c) {
     }
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
     void* dptrs[] = {c->d_tab, c->d_geo, c->d_nb_node, c->d_nb_face, c->d_bc, c->d_Ival, c->d_U[0], c->d_U[1],
-                     c->d_rhsUA, c->d_lA, c->d_dfA, c->d_rhsU, c->d_ut, c->d_dtp, c->d_dbg_rhsH, c->d_dbg_rhsL, c->d_dbg_L,
+                     c->d_rhsUA, c->d_lA, c->d_dfA, c->d_rhsU, c->d_ut, c->d_theta, c->d_dtp, c->d_dbg_rhsH,
+                     c->d_dbg_rhsL, c->d_dbg_L,
                      c->d_sc, c->d_keys};
     for (void* p : dptrs)
         if (p) (void)hipFree(p);
@@ -220,7 +224,9 @@ int p2de_create(const p2de_config* cfg, const p2de_operators* ops, const p2de_ge
                              bc->mapP};
         for (const void* p : req)
             if (!p) return fail(c, P2DE_E_INVALID, "NULL operator, geometry or mapP array"), bail(P2DE_E_INVALID);
-        if ((st = setup_tables(c->tab, c->Nd, c->N1D, ops, c->err))) return bail(st);
+        c->basis = cfg->basis;
+        c->ep_limit = cfg->entropyproj_limiter == P2DE_NODEWISE_SCALED_EXTRAPOLATION;
+        if ((st = setup_tables(c->tab, c->Nd, c->N1D, cfg->basis, ops, c->err))) return bail(st);
         MeshArrays mesh;
         if ((st = setup_mesh(c->Nd, c->N1D, c->K, c->tab, geom, bc, mesh, c->err))) return bail(st);
         const std::vector<Geo>& geo = mesh.geo;
@@ -239,6 +245,7 @@ int p2de_create(const p2de_config* cfg, const p2de_operators* ops, const p2de_ge
             (st = dalloc(c, &c->d_U[0], nU)) || (st = dalloc(c, &c->d_U[1], nU)) || (st = dalloc(c, &c->d_rhsUA, nU)) ||
             (st = dalloc(c, &c->d_lA, (size_t)K * Nfp)) || (st = dalloc(c, &c->d_dfA, (size_t)K * Nfp * Nc)) ||
             (st = dalloc(c, &c->d_rhsU, nU)) || (st = dalloc(c, &c->d_ut, (size_t)K * Nfp * Nc)) ||
+            (st = dalloc(c, &c->d_theta, (size_t)K * Nfp)) ||
             (st = dalloc(c, &c->d_dtp, (size_t)K)) ||
             (st = dalloc(c, &c->d_sc, 1)) || (st = dalloc(c, &c->d_keys, 3)))
             return bail(st);
@@ -325,6 +332,7 @@ PhaseAArgs argsA(p2de_ctx* c, const double* U, int stage1) {
     PhaseAArgs a{};
     a.U = U;
     a.ut = c->d_ut;
+    a.theta = c->ep_limit ? c->d_theta : nullptr;
     a.T = c->d_tab;
     a.geo = c->d_geo;
     a.nb_node = c->d_nb_node;
@@ -368,18 +376,51 @@ PhaseBArgs argsB(p2de_ctx* c, int mode, const double* W, const double* Uin, doub
     b.Uout = Uout;
     b.rhsU = c->d_rhsU;
     b.dbg_L = c->debug ? c->d_dbg_L : nullptr;
-    b.ut = c->d_ut;
+    b.ut = c->basis == P2DE_BASIS_GAUSS ? nullptr : c->d_ut;
     b.finalize = finalize;
     return b;
 }
 
-// Projected face states of d_U[0] when they are not already current.
+// Lobatto: projected face states of d_U[0] when they are not already current (phase B forms
+// them for the states it writes).  Gauss: nothing, each stage projects its own input.
 int ensure_face_states(p2de_ctx* c) {
-    if (c->ut_valid) return P2DE_OK;
+    if (c->ut_valid || c->basis == P2DE_BASIS_GAUSS) return P2DE_OK;
     LAUNCHCHK(c, timed(c, 2, [&] {
                   return launch_face_state(c->Nd, c->N1D, c->d_U[0], c->d_tab, c->K, c->cfg.gamma, c->d_ut, c->stream);
               }));
     c->ut_valid = true;
+    return P2DE_OK;
+}
+
+// Gauss: entropy projection of the stage input (init_rhs! and entropy_projection!, rhs.jl:19-23,
+// 41-44): theta and u_tilde of every face node, before phase A reads them.
+int project_faces(p2de_ctx* c, const double* U) {
+    if (c->basis != P2DE_BASIS_GAUSS) return P2DE_OK;
+    EPArgs e{};
+    e.U = U;
+    e.T = c->d_tab;
+    e.sc = c->d_sc;
+    e.K = c->K;
+    e.limit = c->ep_limit ? 1 : 0;
+    e.gamma = c->cfg.gamma;
+    e.zeta = c->cfg.zeta;
+    e.eta = c->cfg.eta;
+    e.POSTOL = c->cfg.POSTOL;
+    e.ut = c->d_ut;
+    e.theta = c->d_theta;
+    LAUNCHCHK(c, timed(c, 0, [&] { return launch_entropy_projection(c->Nd, c->N1D, e, c->stream); }));
+    return P2DE_OK;
+}
+
+int stage_rhs(p2de_ctx* c, const double* U, int stage1) {
+    int st = project_faces(c, U);
+    if (st) return st;
+    LAUNCHCHK(c, timed(c, 0, [&] { return launch_phaseA(c->Nd, c->N1D, c->basis, c->variantA, argsA(c, U, stage1), c->stream); }));
+    if (stage1)
+        LAUNCHCHK(c, timed(c, 2, [&] {
+                      return launch_dt_reduce(c->d_sc, c->d_dtp, phaseA_blocks(c->Nd, c->N1D, c->basis, c->variantA, c->K),
+                                              c->stream);
+                  }));
     return P2DE_OK;
 }
 
@@ -392,16 +433,13 @@ int enqueue_step(p2de_ctx* c) {
     if (st) return st;
     LAUNCHCHK(c, timed(c, 2, [&] { return launch_step_init(c->d_sc, g.CFL, g.dt0, g.T, s); }));
     // stage 1: U1 = W + dt*rhsU (SSPRK33.jl:32-33)
-    LAUNCHCHK(c, timed(c, 0, [&] { return launch_phaseA(c->Nd, c->N1D, c->variantA, argsA(c, A, 1), s); }));
-    LAUNCHCHK(c, timed(c, 2, [&] {
-                  return launch_dt_reduce(c->d_sc, c->d_dtp, phaseA_blocks(c->Nd, c->N1D, c->variantA, c->K), s);
-              }));
+    if ((st = stage_rhs(c, A, 1))) return st;
     LAUNCHCHK(c, timed(c, 1, [&] { return launch_phaseB(c->Nd, c->N1D, argsB(c, MODE_STAGE1, A, A, B, 0), s); }));
     // stage 2: U2 = 3/4 W + 1/4 (U1 + dt*rhsU) (SSPRK33.jl:34-36)
-    LAUNCHCHK(c, timed(c, 0, [&] { return launch_phaseA(c->Nd, c->N1D, c->variantA, argsA(c, B, 0), s); }));
+    if ((st = stage_rhs(c, B, 0))) return st;
     LAUNCHCHK(c, timed(c, 1, [&] { return launch_phaseB(c->Nd, c->N1D, argsB(c, MODE_STAGE2, A, B, B, 0), s); }));
     // stage 3: U^{n+1} = 1/3 W + 2/3 (U2 + dt*rhsU) (SSPRK33.jl:37-39), t += dt
-    LAUNCHCHK(c, timed(c, 0, [&] { return launch_phaseA(c->Nd, c->N1D, c->variantA, argsA(c, B, 0), s); }));
+    if ((st = stage_rhs(c, B, 0))) return st;
     LAUNCHCHK(c, timed(c, 1, [&] { return launch_phaseB(c->Nd, c->N1D, argsB(c, MODE_STAGE3, A, B, A, 1), s); }));
     return P2DE_OK;
 }
@@ -441,11 +479,7 @@ int p2de_rhs(p2de_ctx* ctx, double t, double dt, int nstage, double* dt_out, dou
     LAUNCHCHK(ctx, timed(ctx, 2, [&] { return launch_rhs_init(ctx->d_sc, dt, cap, s); }));
     int st0 = ensure_face_states(ctx);
     if (st0) return st0;
-    LAUNCHCHK(ctx, timed(ctx, 0, [&] { return launch_phaseA(ctx->Nd, ctx->N1D, ctx->variantA, argsA(ctx, ctx->d_U[0], nstage == 1), s); }));
-    if (nstage == 1)
-        LAUNCHCHK(ctx, timed(ctx, 2, [&] {
-                      return launch_dt_reduce(ctx->d_sc, ctx->d_dtp, phaseA_blocks(ctx->Nd, ctx->N1D, ctx->variantA, ctx->K), s);
-                  }));
+    if ((st0 = stage_rhs(ctx, ctx->d_U[0], nstage == 1))) return st0;
     LAUNCHCHK(ctx, timed(ctx, 1, [&] {
                   return launch_phaseB(ctx->Nd, ctx->N1D, argsB(ctx, MODE_RHS, ctx->d_U[0], ctx->d_U[0], nullptr, 0), s);
               }));
@@ -471,6 +505,12 @@ int p2de_get_field(p2de_ctx* ctx, int field, double* out) {
         case P2DE_FIELD_L_LOCAL:
             src = ctx->d_dbg_L;
             n = (size_t)(ctx->Nq + ctx->N1D) * ctx->Nd * ctx->K;
+            break;
+        case P2DE_FIELD_THETA:
+            if (!ctx->ep_limit)
+                return fail(ctx, P2DE_E_INVALID, "theta exists with NodewiseScaledExtrapolation only");
+            src = ctx->d_theta;
+            n = (size_t)ctx->Nfp * ctx->K;
             break;
         default: return fail(ctx, P2DE_E_INVALID, "unknown field");
     }
@@ -645,7 +685,7 @@ namespace {
 
 std::string tune_key(const p2de_ctx* c) {
     std::ostringstream o;
-    o << "Nd" << c->Nd << "_N" << (c->N1D - 1) << "_K" << c->K;
+    o << "Nd" << c->Nd << "_N" << (c->N1D - 1) << "_K" << c->K << (c->basis == P2DE_BASIS_GAUSS ? "_gauss" : "");
     return o.str();
 }
 
@@ -672,13 +712,13 @@ extern "C" {
 
 int p2de_num_variants(const p2de_ctx* ctx) {
     if (ctx && ctx->mix) return p2de_batch_num_variants(ctx->mix);
-    return ctx ? num_variants_A(ctx->Nd, ctx->N1D) : 0;
+    return ctx ? num_variants_A(ctx->Nd, ctx->N1D, ctx->basis) : 0;
 }
 
 int p2de_set_variant(p2de_ctx* ctx, int variant) {
     if (!ctx) return fail(ctx, P2DE_E_INVALID, "NULL context");
     if (ctx->mix) return fwd(ctx, p2de_batch_set_variant(ctx->mix, variant));
-    if (variant < 0 || variant >= num_variants_A(ctx->Nd, ctx->N1D)) return fail(ctx, P2DE_E_INVALID, "variant out of range");
+    if (variant < 0 || variant >= num_variants_A(ctx->Nd, ctx->N1D, ctx->basis)) return fail(ctx, P2DE_E_INVALID, "variant out of range");
     ctx->variantA = variant;
     return P2DE_OK;
 }
@@ -692,7 +732,7 @@ int p2de_get_variant(const p2de_ctx* ctx, int* variant, int* E, int* NT) {
         return st;
     }
     int e = 0, nt = 0;
-    variant_shape_A(ctx->Nd, ctx->N1D, ctx->variantA, &e, &nt);
+    variant_shape_A(ctx->Nd, ctx->N1D, ctx->basis, ctx->variantA, &e, &nt);
     if (variant) *variant = ctx->variantA;
     if (E) *E = e;
     if (NT) *NT = nt;
@@ -705,7 +745,7 @@ int p2de_autotune(p2de_ctx* ctx, int steps, const char* cache_path, int* chosen)
     if (!ctx->state_set) return fail(ctx, P2DE_E_STATE, "no state uploaded (p2de_set_state)");
     const std::string key = tune_key(ctx);
     auto cache = read_cache(cache_path);
-    const int nv = num_variants_A(ctx->Nd, ctx->N1D);
+    const int nv = num_variants_A(ctx->Nd, ctx->N1D, ctx->basis);
     auto it = cache.find(key);
     if (it != cache.end() && it->second >= 0 && it->second < nv) {
         ctx->variantA = it->second;

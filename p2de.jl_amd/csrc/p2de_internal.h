@@ -8,6 +8,7 @@
 //                                          subcell face (one per face node)
 //   dfA        double[K][Nfp][Nc]          f_bar_H - f_bar_L at those faces
 //   ut         double[K][Nfp][Nc]          projected face states u(v(U)) of the current state
+//   theta      double[K][Nfp]              entropy-projection limiting parameters (Gauss basis)
 //   geo        Geo[K]                      per-element metric (constant inside an element)
 //   nb_node    int32[K][Nfp]               kP*Nq + fq2q[iP] of the mapP target
 //   nb_face    int32[K][Nfp]               kP*Nfp + iP of the mapP target
@@ -22,7 +23,8 @@ constexpr int MAX_N1D = 5;
 constexpr int MAX_NQ = 25;
 constexpr int MAX_NFP = 20;
 constexpr int MAX_NH = 45;
-constexpr int MAX_NPH = 120;
+constexpr int MAX_NPH = 200;      // Gauss 2D N=4: 100 line pairs + 100 face-volume pairs
+constexpr int MAX_HF = 4;         // faces whose interpolation row reaches a node (Gauss 2D)
 constexpr int MAX_NPL = 40;
 
 // Per-element metric terms.  The reference stores them per node (GeomData, Solver.jl:185-190);
@@ -35,6 +37,7 @@ struct Geo {
 // passes (Operators, Solver.jl:192-212).  Indices are 0-based.
 struct ElemTables {
     int Nd, Nc, N1D, Nq, Nfp, Nh, nph, npl;
+    int basis;                    // 0 Lobatto (Vf selects fq2q), 1 Gauss (Vf interpolates along the line)
     // high-order pairs, Srsh_nnz order (init.jl:188-195): hybrid nodes i > j, the metric
     // direction the pair acts in, and Srh_db[i,j], Ssh_db[i,j]
     uint8_t hp_i[MAX_NPH], hp_j[MAX_NPH], hp_dir[MAX_NPH];
@@ -57,16 +60,27 @@ struct ElemTables {
     int16_t cl_pair[2 * MAX_NPL];
     // node data
     double wq[MAX_NQ];
+    double rwq[MAX_NQ];           // 1 / wq[i] (the scaling of MinvVhT_new, flux_differencing.jl:304)
     double a_v[MAX_NQ];           // MinvVhT[i, i]
-    int8_t nf[MAX_NQ];            // faces f with fq2q[f] == i (ascending)
-    int8_t nface[MAX_NQ][2];
-    double a_h[MAX_NQ][2];        // MinvVhT[i, Nq+f]
-    double a_f[MAX_NQ][2];        // MinvVfT[i, f]
+    int8_t nf[MAX_NQ];            // faces f with fq2q[f] == i (ascending): the node's element-boundary
+    int8_t nface[MAX_NQ][2];      // subcell faces and low-order surface terms
+    // faces f whose row of Vf or Vf_low reaches node i (ascending), for the assembly
+    // MinvVhT*QF1 + MinvVfT*BF_H: Lobatto the nface list, Gauss the four line ends of its lines
+    int8_t nh[MAX_NQ];
+    int8_t hface[MAX_NQ][MAX_HF];
+    double a_h[MAX_NQ][MAX_HF];   // MinvVhT[i, Nq+f]
+    double a_f[MAX_NQ][MAX_HF];   // MinvVfT[i, f]
+    double vf_h[MAX_NQ][MAX_HF];  // Vf[f, i]       (limited assembly, flux_differencing.jl:310-319)
+    double vfl_h[MAX_NQ][MAX_HF]; // Vf_low[f, i]
     int8_t nq2fq[MAX_NQ];         // q2fq[i] (CFL face order)
     int8_t q2fq[MAX_NQ][2];
     // face data
     uint8_t fq2q[MAX_NFP];
-    double vf_coef[MAX_NFP];      // (1*Vf + 0*Vf_low)[f, fq2q[f]], the face row's only nonzero
+    double vf_coef[MAX_NFP];      // Lobatto: (1*Vf + 0*Vf_low)[f, fq2q[f]], the face row's only nonzero
+    // the line of face node f (Gauss): its nodes in ascending order, Vf and Vf_low on them
+    uint8_t fl_node[MAX_NFP][MAX_N1D];
+    double fl_vf[MAX_NFP][MAX_N1D];
+    double fl_vfl[MAX_NFP][MAX_N1D];
     double Br[MAX_NFP], Bs[MAX_NFP];
     int8_t ld[MAX_NFP];           // direction the LF term is applied to (rhs_utils.jl:78-102)
     int8_t bsign[MAX_NFP];        // -1: left/bottom element-boundary subcell face, +1 right/top
@@ -86,6 +100,7 @@ struct StepScalars {
 struct PhaseAArgs {
     const double* U;
     const double* ut;             // projected face states u_tilde, double[K][Nfp][Nc]
+    const double* theta;          // Gauss with the entropy-projection limiter: double[K][Nfp] (else nullptr)
     const ElemTables* T;
     const Geo* geo;
     const int* nb_node;
@@ -124,14 +139,28 @@ struct PhaseBArgs {
     double* Uout;                 // stage output (may alias Uin or W elementwise)
     double* rhsU;                 // MODE_RHS output
     double* dbg_L;
-    double* ut;                   // projected face states of Uout (stage modes)
+    double* ut;                   // projected face states of Uout (stage modes; nullptr: not formed here)
     int finalize;                 // stage 3: advance t
 };
 
+// Entropy projection of the face states (Gauss basis): theta per face node and u_tilde.
+struct EPArgs {
+    const double* U;
+    const ElemTables* T;
+    StepScalars* sc;
+    int K;
+    int limit;                    // NodewiseScaledExtrapolation (else theta = 1)
+    double gamma, zeta, eta, POSTOL;
+    double* ut;                   // double[K][Nfp][Nc]
+    double* theta;                // double[K][Nfp]
+};
+
 // launchers (rhs_kernels.hip); return hipError_t as int
-int launch_phaseA(int Nd, int N1D, int variant, const PhaseAArgs& a, void* stream);
-int num_variants_A(int Nd, int N1D);
-int variant_shape_A(int Nd, int N1D, int variant, int* E, int* NT);
+// basis: 0 Lobatto, 1 Gauss (phase-A variants are per basis)
+int launch_phaseA(int Nd, int N1D, int basis, int variant, const PhaseAArgs& a, void* stream);
+int num_variants_A(int Nd, int N1D, int basis);
+int variant_shape_A(int Nd, int N1D, int basis, int variant, int* E, int* NT);
+int launch_entropy_projection(int Nd, int N1D, const EPArgs& a, void* stream);
 int launch_phaseB(int Nd, int N1D, const PhaseBArgs& a, void* stream);
 int launch_face_state(int Nd, int N1D, const double* U, const ElemTables* T, int K, double gamma, double* ut,
                       void* stream);
@@ -139,7 +168,7 @@ int launch_step_init(StepScalars* sc, double CFL, double dt0, double T, void* st
 int launch_rhs_init(StepScalars* sc, double dt_lim, double dt_cap, void* stream);
 // dt = min(cap, min_b dtp[b]) over the phase-A workgroups (no global atomics)
 int launch_dt_reduce(StepScalars* sc, const double* dtp, int n, void* stream);
-int phaseA_blocks(int Nd, int N1D, int variant, int K);
+int phaseA_blocks(int Nd, int N1D, int basis, int variant, int K);
 int launch_positivity(const double* U, int64_t nodes, int Nd, double gamma, unsigned long long* keys,
                       unsigned long long* cnt, void* stream);
 

@@ -31,10 +31,17 @@ int setup_config(const p2de_config* cfg, int* Nq, int* Nfp, int* Nh, int* K, std
     // option set (Solver.jl:1-121); anything else is not implemented here
     if (cfg->rhs != P2DE_RHS_LIMITED_DG || cfg->low_order_surface_flux != P2DE_LF_ON_NODAL_VAL ||
         cfg->high_order_surface_flux != P2DE_LF_ON_PROJECTED_VAL ||
-        cfg->high_order_volume_flux != P2DE_CHANDRASHEKAR_FLUX || cfg->basis != P2DE_BASIS_LOBATTO ||
-        cfg->entropyproj_limiter != P2DE_NO_ENTROPY_PROJ_LIMITER || cfg->rhs_limiter != P2DE_SUBCELL_LIMITER ||
+        cfg->high_order_volume_flux != P2DE_CHANDRASHEKAR_FLUX || cfg->rhs_limiter != P2DE_SUBCELL_LIMITER ||
         cfg->bound != P2DE_POSITIVITY_BOUND || cfg->shockcapture != P2DE_NO_SHOCK_CAPTURE)
         return fail_s(err, P2DE_E_UNSUPPORTED, "option set not implemented");
+    if (cfg->basis != P2DE_BASIS_LOBATTO && cfg->basis != P2DE_BASIS_GAUSS)
+        return fail_s(err, P2DE_E_UNSUPPORTED, "approximation basis not implemented");
+    if (cfg->entropyproj_limiter != P2DE_NO_ENTROPY_PROJ_LIMITER &&
+        cfg->entropyproj_limiter != P2DE_NODEWISE_SCALED_EXTRAPOLATION)
+        return fail_s(err, P2DE_E_UNSUPPORTED, "entropy-projection limiter not implemented");
+    // the Lobatto method of compute_entropyproj_limiting_param! (filter.jl:1-3) does not run
+    if (cfg->basis == P2DE_BASIS_LOBATTO && cfg->entropyproj_limiter != P2DE_NO_ENTROPY_PROJ_LIMITER)
+        return fail_s(err, P2DE_E_UNSUPPORTED, "the Lobatto basis takes NoEntropyProjectionLimiter");
     if (cfg->precision != P2DE_PRECISION_FP64 && cfg->precision != P2DE_PRECISION_MIXED)
         return fail_s(err, P2DE_E_INVALID, "precision");
     if (cfg->upper_bound_mode != P2DE_UPPER_BOUND_REFERENCE && cfg->upper_bound_mode != P2DE_UPPER_BOUND_INTENDED)
@@ -45,6 +52,8 @@ int setup_config(const p2de_config* cfg, int* Nq, int* Nfp, int* Nh, int* K, std
     if (!(cfg->gamma > 1.0)) return fail_s(err, P2DE_E_INVALID, "gamma must exceed 1");
     if (cfg->precision == P2DE_PRECISION_MIXED && cfg->Nd != 2)
         return fail_s(err, P2DE_E_UNSUPPORTED, "the mixed-precision path is 2D only");
+    if (cfg->precision == P2DE_PRECISION_MIXED && cfg->basis != P2DE_BASIS_LOBATTO)
+        return fail_s(err, P2DE_E_UNSUPPORTED, "the mixed-precision path runs the Lobatto basis");
     const int N1D = cfg->N + 1;
     *Nq = (int)ipow(N1D, cfg->Nd);
     *Nfp = cfg->Nd == 1 ? 2 : 4 * N1D;
@@ -56,20 +65,24 @@ int setup_config(const p2de_config* cfg, int* Nq, int* Nfp, int* Nh, int* K, std
     return P2DE_OK;
 }
 
-int setup_tables(ElemTables& T, int Nd, int N1D, const p2de_operators* ops, std::string& err) {
+int setup_tables(ElemTables& T, int Nd, int N1D, int basis, const p2de_operators* ops, std::string& err) {
     std::memset(&T, 0, sizeof(T));
     const int Nq = Nd == 1 ? N1D : N1D * N1D, Nfp = Nd == 1 ? 2 : 4 * N1D, Nh = Nq + Nfp;
+    const bool gauss = basis == P2DE_BASIS_GAUSS;
+    T.basis = gauss ? 1 : 0;
     T.Nd = Nd;
     T.Nc = Nd + 2;
     T.N1D = N1D;
     T.Nq = Nq;
     T.Nfp = Nfp;
     T.Nh = Nh;
-    const int64_t nph_expect = Nd == 1 ? N1D * (N1D - 1) / 2 + 2 : (int64_t)N1D * N1D * (N1D - 1) + Nfp;
+    // face-volume pairs: one per face node (Lobatto), a whole line per face node (Gauss)
+    const int64_t nfv = gauss ? (int64_t)Nfp * N1D : Nfp;
+    const int64_t nph_expect = Nd == 1 ? N1D * (N1D - 1) / 2 + nfv : (int64_t)N1D * N1D * (N1D - 1) + nfv;
     const int64_t npl_expect = Nd == 1 ? N1D - 1 : 2 * N1D * (N1D - 1);
     if (ops->nnz_h != nph_expect)
-        return fail_s(err, P2DE_E_UNSUPPORTED, "Srsh_nnz has " + std::to_string(ops->nnz_h) + " pairs, the Lobatto operator has " +
-                                               std::to_string(nph_expect));
+        return fail_s(err, P2DE_E_UNSUPPORTED, "Srsh_nnz has " + std::to_string(ops->nnz_h) + " pairs, the " +
+                                               (gauss ? "Gauss" : "Lobatto") + " operator has " + std::to_string(nph_expect));
     if (ops->nnz_0 != npl_expect)
         return fail_s(err, P2DE_E_UNSUPPORTED, "Srs0_nnz has " + std::to_string(ops->nnz_0) + " pairs, expected " +
                                                std::to_string(npl_expect));
@@ -102,15 +115,36 @@ int setup_tables(ElemTables& T, int Nd, int N1D, const p2de_operators* ops, std:
         } else if (T.Br[f] == 0.0) {
             return fail_s(err, P2DE_E_UNSUPPORTED, "Br has a zero face entry");
         }
-        // entropy projection row (rhs.jl:90-92 with l = 1): a single nonzero at fq2q[f]
+        // the line of face node f: the nodes the rows of Vf and Vf_low may reach
+        bool online[MAX_NQ] = {};
+        for (int s = 0; s < N1D; s++) {
+            int node;
+            if (Nd == 1) {
+                node = s;
+            } else {
+                const int face = f / N1D, m = f % N1D;
+                node = face < 2 ? m * N1D + s : s * N1D + m;
+            }
+            online[node] = true;
+            T.fl_node[f][s] = (uint8_t)node;
+            T.fl_vf[f][s] = cm(ops->Vf, Nfp, f, node);
+            T.fl_vfl[f][s] = cm(ops->Vf_low, Nfp, f, node);
+        }
         for (int j = 0; j < Nq; j++) {
-            const double vf = cm(ops->Vf, Nfp, f, j);
-            const double coef = 1.0 * vf + (1.0 - 1.0) * cm(ops->Vf_low, Nfp, f, j);
-            if (j == q) {
-                if (coef == 0.0) return fail_s(err, P2DE_E_UNSUPPORTED, "Vf is zero at fq2q");
-                T.vf_coef[f] = coef;
-            } else if (vf != 0.0) {
-                return fail_s(err, P2DE_E_UNSUPPORTED, "Vf is not a face-node selection (Gauss basis is not implemented)");
+            const double vf = cm(ops->Vf, Nfp, f, j), vfl = cm(ops->Vf_low, Nfp, f, j);
+            if (!online[j] && (vf != 0.0 || vfl != 0.0))
+                return fail_s(err, P2DE_E_UNSUPPORTED, "a row of Vf or Vf_low reaches off the face node's line");
+            if (vfl != (j == q ? 1.0 : 0.0))
+                return fail_s(err, P2DE_E_UNSUPPORTED, "Vf_low is not the selection of the fq2q node");
+            if (!gauss) {
+                // entropy projection row (rhs.jl:90-92 with l = 1): a single nonzero at fq2q[f]
+                const double coef = 1.0 * vf + (1.0 - 1.0) * vfl;
+                if (j == q) {
+                    if (coef == 0.0) return fail_s(err, P2DE_E_UNSUPPORTED, "Vf is zero at fq2q");
+                    T.vf_coef[f] = coef;
+                } else if (vf != 0.0) {
+                    return fail_s(err, P2DE_E_UNSUPPORTED, "Vf is not a face-node selection: pass basis = Gauss");
+                }
             }
         }
     }
@@ -118,24 +152,36 @@ int setup_tables(ElemTables& T, int Nd, int N1D, const p2de_operators* ops, std:
     for (int i = 0; i < Nq; i++) {
         T.wq[i] = ops->wq[i];
         if (!(T.wq[i] > 0.0)) return fail_s(err, P2DE_E_INVALID, "wq must be positive");
+        T.rwq[i] = 1.0 / T.wq[i];
         for (int j = 0; j < Nq; j++)
             if (j != i && cm(ops->MinvVhT, Nq, i, j) != 0.0)
                 return fail_s(err, P2DE_E_UNSUPPORTED, "MinvVhT volume block is not diagonal");
         T.a_v[i] = cm(ops->MinvVhT, Nq, i, i);
-        int n = 0;
+        int n = 0, nh = 0;
         for (int f = 0; f < Nfp; f++) {
             const double ah = cm(ops->MinvVhT, Nq, i, Nq + f), af = cm(ops->MinvVfT, Nq, i, f);
+            const double vf = cm(ops->Vf, Nfp, f, i), vfl = cm(ops->Vf_low, Nfp, f, i);
             if (T.fq2q[f] == i) {
                 if (n >= 2) return fail_s(err, P2DE_E_UNSUPPORTED, "more than two faces at a node");
-                T.nface[i][n] = (int8_t)f;
-                T.a_h[i][n] = ah;
-                T.a_f[i][n] = af;
-                n++;
+                T.nface[i][n++] = (int8_t)f;
+            }
+            // assembly list: faces whose Vf or Vf_low row reaches the node (the only nonzeros of
+            // MinvVhT_new's face block); the caller's MinvVhT/MinvVfT must vanish elsewhere
+            const bool reach = gauss ? (vf != 0.0 || vfl != 0.0) : T.fq2q[f] == i;
+            if (reach) {
+                if (nh >= MAX_HF) return fail_s(err, P2DE_E_UNSUPPORTED, "more than four faces reach a node");
+                T.hface[i][nh] = (int8_t)f;
+                T.a_h[i][nh] = ah;
+                T.a_f[i][nh] = af;
+                T.vf_h[i][nh] = vf;
+                T.vfl_h[i][nh] = vfl;
+                nh++;
             } else if (ah != 0.0 || af != 0.0) {
-                return fail_s(err, P2DE_E_UNSUPPORTED, "MinvVhT/MinvVfT couple a node to a face not at that node");
+                return fail_s(err, P2DE_E_UNSUPPORTED, "MinvVhT/MinvVfT couple a node to a face whose row does not reach it");
             }
         }
         T.nf[i] = (int8_t)n;
+        T.nh[i] = (int8_t)nh;
         int m = 0;
         for (int s = 0; s < 2; s++) {
             const int64_t f1 = ops->q2fq[i + (int64_t)Nq * s];

@@ -21,9 +21,10 @@ struct MeshArrays {
 // implements and the basic sizes; fills Nq/Nfp/Nh/K.
 int setup_config(const p2de_config* cfg, int* Nq, int* Nfp, int* Nh, int* K, std::string& err);
 
-// Operator tables of the reference element from Operators{DIM} (Solver.jl:192-212); returns
-// P2DE_E_UNSUPPORTED for operator structures outside the LGL tensor-product layout.
-int setup_tables(ElemTables& T, int Nd, int N1D, const p2de_operators* ops, std::string& err);
+// Operator tables of the reference element from Operators{DIM} (Solver.jl:192-212) for the
+// basis the configuration names (P2DE_BASIS_*); returns P2DE_E_UNSUPPORTED for operator
+// structures outside the tensor-product Lobatto / Gauss layouts.
+int setup_tables(ElemTables& T, int Nd, int N1D, int basis, const p2de_operators* ops, std::string& err);
 
 // Per-element metric, face connectivity and boundary codes from GeomData and BCData
 // (Solver.jl:185-190, StateParam.jl:1-7).

@@ -13,6 +13,8 @@
 //     positivity of the new state and forms its projected face states u(v(u)) once per
 //     boundary node for the next phase A (face_state_kernel does the same for a state
 //     set from the host).
+// On the Gauss basis a third launch per stage comes first: entropy_projection_kernel forms the
+// projected face states (and the NodewiseScaledExtrapolation parameters) from the whole line.
 // The reference runs the same work as ~27 threaded loops per stage over HBM-sized
 // intermediates (src/dg/rhs/*.jl, src/dg/limiter/subcell.jl); the formulas and their
 // operation order are kept (built with -ffp-contract=off) so the FP64 result matches the
@@ -24,13 +26,16 @@
 
 namespace p2de {
 
-template <int DIM, int N1D>
+// BASIS 0: Lobatto (one face-volume pair per face node); 1: Gauss (a face node pairs with
+// every node of its line)
+template <int DIM, int N1D, int BASIS = 0>
 struct Sz {
     static constexpr int NQ = DIM == 1 ? N1D : N1D * N1D;
     static constexpr int NFP = DIM == 1 ? 2 : 4 * N1D;
     static constexpr int NH = NQ + NFP;
     static constexpr int NC = DIM + 2;
-    static constexpr int NPH = DIM == 1 ? N1D * (N1D - 1) / 2 + 2 : N1D * N1D * (N1D - 1) + NFP;
+    static constexpr int NFV = BASIS == 0 ? NFP : NFP * N1D;   // face-volume pairs
+    static constexpr int NPH = DIM == 1 ? N1D * (N1D - 1) / 2 + NFV : N1D * N1D * (N1D - 1) + NFV;
     static constexpr int NPL = DIM == 1 ? N1D - 1 : 2 * N1D * (N1D - 1);
     static constexpr int NLINE = DIM == 1 ? 1 : 2 * N1D;
     static constexpr int NLS = NQ + N1D;  // first dimension of L_local (State.jl:21)
@@ -120,14 +125,15 @@ __device__ __forceinline__ double limiting_param(const double* uL, const double*
 // LDS of one workgroup.  Arrays whose lifetimes do not overlap share storage:
 // node/face primitives and fluxes (phases 1-3) with the directional RHS and u_L (phases 4-5),
 // pair results (phases 3-4) with the subcell flux potentials and parameters (phase 5).
-template <int DIM, int N1D, int E>
+template <int DIM, int N1D, int E, int BASIS>
 struct SmemA {
-    using S = Sz<DIM, N1D>;
+    using S = Sz<DIM, N1D, BASIS>;
     static constexpr int NFB = S::NLINE * (N1D + 1);  // subcell faces per element
     double U[E][S::NQ][S::NC];
     double BFH[E][S::NFP][S::NC];
     double BFL[E][S::NFP][S::NC];
     double lamB[E][S::NFP];
+    double th[BASIS ? E : 1][S::NFP];                  // entropy-projection parameters (Gauss)
     union {
         struct {
             Prim<DIM> np[E][S::NQ];
@@ -180,12 +186,12 @@ __device__ __forceinline__ void project_face(double g, double coef, const double
     Eq<DIM>::u_of_v(g, Vt, ut);
 }
 
-template <int DIM, int N1D, int E, int NT>
+template <int DIM, int N1D, int E, int NT, int BASIS>
 __global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
-    using S = Sz<DIM, N1D>;
+    using S = Sz<DIM, N1D, BASIS>;
     constexpr int NQ = S::NQ, NFP = S::NFP, NC = S::NC, NPH = S::NPH, NPL = S::NPL, NLINE = S::NLINE;
     constexpr int NFB = NLINE * (N1D + 1);
-    __shared__ SmemA<DIM, N1D, E> sm;
+    __shared__ SmemA<DIM, N1D, E, BASIS> sm;
     if (a.sc->done) return;
     const int tid = threadIdx.x;
     const int k0 = blockIdx.x * E;
@@ -226,6 +232,7 @@ __global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
         double ut[NC];
         for (int c = 0; c < NC; c++) ut[c] = a.ut[(size_t)fi * NC + c];
         make_prim<DIM>(g, ut, sm.a.g1.fp[e][f]);
+        if (BASIS) sm.th[BASIS ? e : 0][f] = a.theta ? a.theta[fi] : 1.0;
 
         // mapP target (calculate_primitive_variables! :54-69; get_uP_and_enforce_BC! :101-108)
         const int nbn = a.nb_node[fi], nbf = a.nb_face[fi];
@@ -356,12 +363,18 @@ __global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
             else
                 for (int c = 0; c < NC; c++) Q[d][c] += -sm.x.p.PH[e][p][c];
         }
-        // assemble_rhs! (:331-361): generic mul! adds column by column from zero
-        const double av = T.a_v[i];
+        // assemble_rhs! (:331-361): generic mul! adds column by column from zero.  On a Gauss
+        // element whose entropy-projection parameters are limited (is_Vf_limited :326-328) the
+        // matrices are MinvVhT_new = (1 ./ wq) .* [I Vf_new'] with Vf_new = l Vf + (1-l) Vf_low
+        // (project_flux_difference_to_quad!(::ScaledExtrapolation) :288-319)
+        bool limited = false;
+        if (BASIS)
+            for (int f = 0; f < NFP; f++) limited = limited || sm.th[BASIS ? e : 0][f] < 1.0;
+        const double av = limited ? T.rwq[i] * 1.0 : T.a_v[i];
         for (int d = 0; d < DIM; d++)
             for (int c = 0; c < NC; c++) A[d][c] = 0.0 + av * Q[d][c];
-        for (int m = 0; m < T.nf[i]; m++) {
-            const int f = T.nface[i][m], h = NQ + f;
+        for (int m = 0; m < T.nh[i]; m++) {
+            const int f = T.hface[i][m], h = NQ + f;
             double Qf[DIM][NC];
             for (int d = 0; d < DIM; d++)
                 for (int c = 0; c < NC; c++) Qf[d][c] = 0.0;
@@ -372,11 +385,15 @@ __global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
                 else
                     for (int c = 0; c < NC; c++) Qf[d][c] += -sm.x.p.PH[e][p][c];
             }
-            const double ah = T.a_h[i][m];
+            double ah = T.a_h[i][m], af = T.a_f[i][m];
+            if (limited) {
+                const double l = sm.th[BASIS ? e : 0][f];
+                ah = T.rwq[i] * (l * T.vf_h[i][m] + (1.0 - l) * T.vfl_h[i][m]);
+                af = ah;
+            }
             for (int d = 0; d < DIM; d++)
                 for (int c = 0; c < NC; c++) A[d][c] += ah * Qf[d][c];
             const int ld = T.ld[f];
-            const double af = T.a_f[i][m];
             for (int c = 0; c < NC; c++) Bv[ld][c] += af * sm.BFH[e][f][c];
         }
         const double J = G.J;
@@ -552,6 +569,98 @@ __global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
     }
 }
 
+// Entropy projection of the face states on the Gauss basis (init_rhs! and entropy_projection!,
+// rhs.jl:19-23, 113-133).  Per face node, the NodewiseScaledExtrapolation parameter theta by
+// bisection (solve_theta! filter.jl:49-59, bisection nonlinear_solvers.jl:3-20) against the
+// bounds of check_bound_on_face_node (filter.jl:87-98) taken from calc_face_values!
+// (filter.jl:26-42), then u_tilde = u(sum_j (theta Vf + (1-theta) Vf_low)[f, j] v_j).  Rows of
+// Vf and Vf_low vanish off the face's line, so every sum runs over the line's nodes in ascending
+// order: the generic product's order without its zero terms.  Without the limiter theta = 1.
+template <int DIM, int N1D, int E, int NT>
+__global__ __launch_bounds__(NT) void entropy_projection_kernel(EPArgs a) {
+    using S = Sz<DIM, N1D, 1>;
+    constexpr int NQ = S::NQ, NFP = S::NFP, NC = S::NC;
+    __shared__ double sU[E * NQ * NC];
+    __shared__ double sV[E * NQ * NC];
+    if (a.sc->done) return;
+    const int tid = threadIdx.x;
+    const int k0 = blockIdx.x * E;
+    const int nel = min(E, a.K - k0);
+    const ElemTables& T = *a.T;
+    const double g = a.gamma;
+    {
+        const double* src = a.U + (size_t)k0 * NQ * NC;
+        for (int w = tid; w < nel * NQ * NC; w += NT) sU[w] = src[w];
+    }
+    __syncthreads();
+    // calculate_entropy_var! (rhs.jl:72-77)
+    for (int w = tid; w < nel * NQ; w += NT) Eq<DIM>::v_of_u(g, &sU[w * NC], &sV[w * NC]);
+    __syncthreads();
+    for (int w = tid; w < nel * NFP; w += NT) {
+        const int e = w / NFP, f = w - e * NFP;
+        const size_t fi = (size_t)(k0 + e) * NFP + f;
+        const double* Ue = &sU[e * NQ * NC];
+        const double* Ve = &sV[e * NQ * NC];
+        int nd[N1D];
+        double vf[N1D], vl[N1D];
+        for (int s = 0; s < N1D; s++) {
+            nd[s] = T.fl_node[f][s] * NC;
+            vf[s] = T.fl_vf[f][s];
+            vl[s] = T.fl_vfl[f][s];
+        }
+        double vt[NC], ut[NC];
+        auto project = [&](double l) {   // rhs.jl:89-92, filter.jl:118-121
+            for (int c = 0; c < NC; c++) vt[c] = 0.0;
+            for (int s = 0; s < N1D; s++) {
+                const double coef = l * vf[s] + (1.0 - l) * vl[s];
+                for (int c = 0; c < NC; c++) vt[c] += coef * Ve[nd[s] + c];
+            }
+        };
+        double theta = 1.0;
+        bool have_ut = false;
+        if (a.limit) {
+            double Uf[NC], VUf[NC];
+            for (int c = 0; c < NC; c++) {
+                Uf[c] = 0.0;
+                VUf[c] = 0.0;
+            }
+            for (int s = 0; s < N1D; s++)
+                for (int c = 0; c < NC; c++) Uf[c] += vf[s] * Ue[nd[s] + c];
+            for (int s = 0; s < N1D; s++)
+                for (int c = 0; c < NC; c++) VUf[c] += vf[s] * Ve[nd[s] + c];
+            const double rhoef = Eq<DIM>::rhoe(Uf);
+            const double eps = a.POSTOL;
+            const double v_hi = jmin(a.zeta * VUf[NC - 1], -eps);
+            const double r_lo = jmax((1.0 - a.eta) * Uf[0], eps), r_hi = (1.0 + a.eta) * Uf[0];
+            const double e_lo = jmax((1.0 - a.eta) * rhoef, eps), e_hi = (1.0 + a.eta) * rhoef;
+            auto admissible = [&](double l) -> bool {   // filter.jl:101-130, 87-98
+                project(l);
+                if (!(vt[NC - 1] < -eps)) return false;
+                Eq<DIM>::u_of_v(g, vt, ut);
+                const double re = Eq<DIM>::rhoe(ut);
+                return vt[NC - 1] < v_hi && ut[0] > r_lo && ut[0] < r_hi && re > e_lo && re < e_hi;
+            };
+            if (admissible(1.0)) {
+                have_ut = true;                       // u_tilde of theta = 1 is already formed
+            } else {
+                double x_valid = 0.0, x_invalid = 1.0;
+                for (int it = 0; it <= 20; it++) {
+                    const double x_new = 0.5 * (x_valid + x_invalid);
+                    if (admissible(x_new)) x_valid = x_new;
+                    else x_invalid = x_new;
+                }
+                theta = x_valid;
+            }
+        }
+        if (!have_ut) {
+            project(theta);
+            Eq<DIM>::u_of_v(g, vt, ut);
+        }
+        for (int c = 0; c < NC; c++) a.ut[fi * NC + c] = ut[c];
+        a.theta[fi] = theta;
+    }
+}
+
 // Projected face states u_tilde of every face node from the state (entropy projection of
 // the face nodes, rhs.jl:84-94), for states that did not come out of phase B.
 template <int DIM, int N1D>
@@ -656,8 +765,9 @@ __global__ __launch_bounds__(256) void phaseB_kernel(PhaseBArgs a) {
         atomicAdd(&a.sc->viol, 1ull);
         atomicMin(&a.sc->first_bad, ((unsigned long long)a.mode << 40) | (unsigned long long)gidx);
     }
-    // projected face states of the new state, once per node (corner nodes feed two faces)
-    if (T.nf[i] > 0) {
+    // projected face states of the new state, once per node (corner nodes feed two faces);
+    // the Gauss basis forms them in entropy_projection_kernel from the whole line instead
+    if (a.ut && T.nf[i] > 0) {
         double V[NC], o[NC];
         Eq<DIM>::v_of_u(a.gamma, un, V);
         double coef_prev = 0.0;
@@ -744,10 +854,10 @@ __global__ void positivity_kernel(const double* U, int64_t nodes, double gamma, 
 }
 
 // --------------------------------------------------------------------------- launchers
-template <int DIM, int N1D, int E, int NT>
+template <int DIM, int N1D, int E, int NT, int BASIS>
 static int launchA(const PhaseAArgs& a, hipStream_t s) {
     const int blocks = (a.K + E - 1) / E;
-    hipLaunchKernelGGL((phaseA_kernel<DIM, N1D, E, NT>), dim3(blocks), dim3(NT), 0, s, a);
+    hipLaunchKernelGGL((phaseA_kernel<DIM, N1D, E, NT, BASIS>), dim3(blocks), dim3(NT), 0, s, a);
     return (int)hipGetLastError();
 }
 template <int DIM, int N1D>
@@ -765,7 +875,7 @@ struct VariantA {
     int E, NT;
     int (*launch)(const PhaseAArgs&, hipStream_t);
 };
-#define VA(D, N, E, T) {E, T, &launchA<D, N, E, T>}
+#define VA(D, N, E, T) {E, T, &launchA<D, N, E, T, 0>}
 static const VariantA kV1_2[] = {VA(1, 2, 32, 64), VA(1, 2, 32, 128)};
 static const VariantA kV1_3[] = {VA(1, 3, 21, 64), VA(1, 3, 21, 128)};
 static const VariantA kV1_4[] = {VA(1, 4, 16, 64), VA(1, 4, 16, 128)};
@@ -776,40 +886,83 @@ static const VariantA kV2_4[] = {VA(2, 4, 4, 64), VA(2, 4, 4, 128), VA(2, 4, 4, 
                                  VA(2, 4, 8, 256), VA(2, 4, 2, 64), VA(2, 4, 2, 128)};
 static const VariantA kV2_5[] = {VA(2, 5, 2, 64), VA(2, 5, 2, 128), VA(2, 5, 4, 128), VA(2, 5, 4, 256), VA(2, 5, 1, 64)};
 #undef VA
+// Gauss: more face-volume pairs per element, so fewer elements per workgroup
+#define VG(D, N, E, T) {E, T, &launchA<D, N, E, T, 1>}
+static const VariantA kG1_2[] = {VG(1, 2, 32, 64), VG(1, 2, 32, 128)};
+static const VariantA kG1_3[] = {VG(1, 3, 16, 64), VG(1, 3, 16, 128)};
+static const VariantA kG1_4[] = {VG(1, 4, 12, 64), VG(1, 4, 12, 128)};
+static const VariantA kG1_5[] = {VG(1, 5, 8, 64), VG(1, 5, 8, 128)};
+static const VariantA kG2_2[] = {VG(2, 2, 16, 64), VG(2, 2, 16, 128), VG(2, 2, 32, 256)};
+static const VariantA kG2_3[] = {VG(2, 3, 4, 64), VG(2, 3, 7, 128), VG(2, 3, 7, 64), VG(2, 3, 14, 256)};
+static const VariantA kG2_4[] = {VG(2, 4, 4, 128), VG(2, 4, 4, 64), VG(2, 4, 2, 64), VG(2, 4, 8, 256)};
+static const VariantA kG2_5[] = {VG(2, 5, 2, 128), VG(2, 5, 2, 64), VG(2, 5, 1, 64), VG(2, 5, 4, 256)};
+#undef VG
 
-static const VariantA* variants(int Nd, int N1D, int* n) {
-#define CASE(D, N, arr)                                    \
-    if (Nd == D && N1D == N) {                             \
+static const VariantA* variants(int Nd, int N1D, int basis, int* n) {
+#define CASE(D, N, B, arr)                                 \
+    if (Nd == D && N1D == N && basis == B) {               \
         *n = (int)(sizeof(arr) / sizeof(arr[0]));          \
         return arr;                                        \
     }
-    CASE(1, 2, kV1_2) CASE(1, 3, kV1_3) CASE(1, 4, kV1_4) CASE(1, 5, kV1_5)
-    CASE(2, 2, kV2_2) CASE(2, 3, kV2_3) CASE(2, 4, kV2_4) CASE(2, 5, kV2_5)
+    CASE(1, 2, 0, kV1_2) CASE(1, 3, 0, kV1_3) CASE(1, 4, 0, kV1_4) CASE(1, 5, 0, kV1_5)
+    CASE(2, 2, 0, kV2_2) CASE(2, 3, 0, kV2_3) CASE(2, 4, 0, kV2_4) CASE(2, 5, 0, kV2_5)
+    CASE(1, 2, 1, kG1_2) CASE(1, 3, 1, kG1_3) CASE(1, 4, 1, kG1_4) CASE(1, 5, 1, kG1_5)
+    CASE(2, 2, 1, kG2_2) CASE(2, 3, 1, kG2_3) CASE(2, 4, 1, kG2_4) CASE(2, 5, 1, kG2_5)
 #undef CASE
     *n = 0;
     return nullptr;
 }
 
-int num_variants_A(int Nd, int N1D) {
+int num_variants_A(int Nd, int N1D, int basis) {
     int n = 0;
-    variants(Nd, N1D, &n);
+    variants(Nd, N1D, basis, &n);
     return n;
 }
 
-int variant_shape_A(int Nd, int N1D, int v, int* E, int* NT) {
+int variant_shape_A(int Nd, int N1D, int basis, int v, int* E, int* NT) {
     int n = 0;
-    const VariantA* t = variants(Nd, N1D, &n);
+    const VariantA* t = variants(Nd, N1D, basis, &n);
     if (!t || v < 0 || v >= n) return -1;
     *E = t[v].E;
     *NT = t[v].NT;
     return 0;
 }
 
-int launch_phaseA(int Nd, int N1D, int variant, const PhaseAArgs& a, void* stream) {
+int launch_phaseA(int Nd, int N1D, int basis, int variant, const PhaseAArgs& a, void* stream) {
     int n = 0;
-    const VariantA* t = variants(Nd, N1D, &n);
+    const VariantA* t = variants(Nd, N1D, basis, &n);
     if (!t || variant < 0 || variant >= n) return (int)hipErrorInvalidValue;
     return t[variant].launch(a, (hipStream_t)stream);
+}
+
+// entropy projection: about 128 face nodes per workgroup
+template <int DIM, int N1D>
+static int launchEP(const EPArgs& a, hipStream_t s) {
+    constexpr int NFP = Sz<DIM, N1D, 1>::NFP;
+    constexpr int E = 128 / NFP;
+    const int blocks = (a.K + E - 1) / E;
+    hipLaunchKernelGGL((entropy_projection_kernel<DIM, N1D, E, 128>), dim3(blocks), dim3(128), 0, s, a);
+    return (int)hipGetLastError();
+}
+
+int launch_entropy_projection(int Nd, int N1D, const EPArgs& a, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (Nd == 1) {
+        switch (N1D) {
+            case 2: return launchEP<1, 2>(a, s);
+            case 3: return launchEP<1, 3>(a, s);
+            case 4: return launchEP<1, 4>(a, s);
+            case 5: return launchEP<1, 5>(a, s);
+        }
+    } else {
+        switch (N1D) {
+            case 2: return launchEP<2, 2>(a, s);
+            case 3: return launchEP<2, 3>(a, s);
+            case 4: return launchEP<2, 4>(a, s);
+            case 5: return launchEP<2, 5>(a, s);
+        }
+    }
+    return (int)hipErrorInvalidValue;
 }
 
 int launch_phaseB(int Nd, int N1D, const PhaseBArgs& a, void* stream) {
@@ -875,9 +1028,9 @@ int launch_dt_reduce(StepScalars* sc, const double* dtp, int n, void* stream) {
     return (int)hipGetLastError();
 }
 
-int phaseA_blocks(int Nd, int N1D, int variant, int K) {
+int phaseA_blocks(int Nd, int N1D, int basis, int variant, int K) {
     int n = 0;
-    const VariantA* t = variants(Nd, N1D, &n);
+    const VariantA* t = variants(Nd, N1D, basis, &n);
     if (!t || variant < 0 || variant >= n) return -1;
     return (K + t[variant].E - 1) / t[variant].E;
 }

@@ -20,9 +20,10 @@ from .dg import FlatProblem, State, StateParam, Solver, TimeParam, flat_problem
 LIB_PATH = Path(__file__).resolve().parent / "csrc" / "libp2de_hip.so"
 
 P2DE_OK, P2DE_E_INVALID, P2DE_E_UNSUPPORTED, P2DE_E_HIP, P2DE_E_ALLOC, P2DE_E_STATE, P2DE_E_POSITIVITY = range(7)
-FIELD_RHSU, FIELD_RHSH, FIELD_RHSL, FIELD_L_LOCAL = range(4)
+FIELD_RHSU, FIELD_RHSH, FIELD_RHSL, FIELD_L_LOCAL, FIELD_THETA = range(5)
 UPPER_BOUND_REFERENCE, UPPER_BOUND_INTENDED = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+BASIS_LOBATTO, BASIS_GAUSS = 0, 1
 
 
 class P2DEError(RuntimeError):
@@ -253,6 +254,8 @@ class DeviceSolver:
         n = self.K * self.Nq * self.Nc
         if which == FIELD_L_LOCAL:
             n = (self.Nq + self.fp.N1D) * self.fp.Nd * self.K
+        elif which == FIELD_THETA:
+            n = self.fp.Nfp * self.K
         out = np.empty(n)
         self._check(lib().p2de_get_field(self.h, int(which), _d(out)))
         return out

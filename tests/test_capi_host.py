@@ -167,3 +167,18 @@ def test_create_mixed_precision_goes_through_the_batch_checks():
     assert st == S.P2DE_E_UNSUPPORTED and msg
     st, msg = _create(fp2, precision=S.PRECISION_MIXED, upper_bound_mode=3)
     assert st == S.P2DE_E_INVALID and msg
+
+
+def test_gauss_options_and_operators_are_checked():
+    """Gauss operators need basis = Gauss (their Vf rows interpolate); the Lobatto basis takes
+    no entropy-projection limiter; the mixed-precision path runs Lobatto only."""
+    _, _, _, fpg = problem("vortex_2d", K=4, basis="gauss")
+    _, _, _, fpl = problem("vortex_2d", K=4)
+    st, msg = _create(fpg, basis=S.BASIS_LOBATTO, entropyproj_limiter=0)
+    assert st == S.P2DE_E_UNSUPPORTED and "Lobatto operator" in msg
+    st, msg = _create(fpl, entropyproj_limiter=1)
+    assert st == S.P2DE_E_UNSUPPORTED and "Lobatto" in msg
+    st, msg = _create(fpg, precision=S.PRECISION_MIXED)
+    assert st == S.P2DE_E_UNSUPPORTED and "Lobatto" in msg
+    st, msg = _create(fpg, basis=7)
+    assert st == S.P2DE_E_UNSUPPORTED and msg
