@@ -8,10 +8,12 @@ stage for all instances, steps replayed from a hipGraph, launch shape autotuned.
 Beside it, on the 2D N=3 quad configurations (vortex 64x64, four-state Riemann 256x256), the
 FP64 drop-in (``p2de_create``) and the mixed path are timed on the same steps, with their
 ratio and the density L1/Linf difference of the two results (the batched instances are also
-run one by one through the FP64 drop-in for that comparison).
+run one by one through the FP64 drop-in for that comparison).  The Gauss basis with the
+entropy-projection limiter is timed through the FP64 drop-in on the four-state Riemann problem
+(256x256, N=3) and on the reference's Kelvin-Helmholtz example (80x80, N=3).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-compare]
-    python bench.py --profile WORKLOAD [--precision mixed|fp64] [--variant V] [--steps K]
+    python bench.py --profile WORKLOAD [--precision mixed|fp64] [--basis B] [--variant V] [--steps K]
         (runs K steps of one workload and nothing else: the command the rocprofv3
          summaries in profiles/ were taken with)
 
@@ -129,8 +131,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison legs")
-    ap.add_argument("--profile", choices=["batched", "vortex_2d", "riemann4_2d", "kelvin_helmholtz"],
+    ap.add_argument("--profile", choices=["batched", "vortex_2d", "riemann4_2d", "kelvin_helmholtz",
+                                          "kelvin_helmholtz_gauss"],
                     help="only run --steps steps of this workload (for rocprofv3) and exit")
+    ap.add_argument("--basis", choices=["lobatto", "gauss", "gauss_noep"], default=None,
+                    help="approximation basis of a --profile workload (default: the workload's own)")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--variant", type=int, default=0, help="launch shape for --profile")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -139,7 +144,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU decomposition is described in DESIGN.md, not run.", file=sys.stderr)
+              "the multi-GPU decomposition is documented, not run.", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_calibration:
@@ -163,7 +168,8 @@ def main():
         if args.profile == "batched":
             inst = pkg.configs.batched_instances()
         else:
-            inst = [pkg.configs.build(args.profile)]
+            kw = {"basis": args.basis} if args.basis else {}
+            inst = [pkg.configs.build(args.profile, **kw)]
         fps = [pkg.flat_problem(s, sp) for s, st, sp in inst]
         Us = [st.preallocation.Uq for s, st, sp in inst]
         if args.precision == "fp64":
@@ -185,7 +191,7 @@ def main():
             dev.enqueue_steps(args.steps)
             dev.synchronize()
             desc = dev.variant()
-        print(json.dumps({"profile": args.profile, "precision": args.precision, "variant": desc,
+        print(json.dumps({"profile": args.profile, "basis": args.basis, "precision": args.precision, "variant": desc,
                           "steps": args.steps, "dofs": sum(fp.K * fp.Nq for fp in fps)}), flush=True)
         return
 
@@ -284,6 +290,31 @@ def main():
             cmp[name] = {"dofs": n, "fp64_dropin": r64, "mixed": rm, "ratio": rm / r64, "density_L1": l1,
                          "density_Linf": linf, "min_p_fp64": rep64.min_p, "min_p_mixed": repm.min_p,
                          "positivity_violations": int(rep64.violations) + int(repm.violations)}
+        # the Gauss basis (GaussCollocation + NodewiseScaledExtrapolation), FP64 drop-in: the
+        # four-state Riemann problem at 256x256 N=3 and the reference's Kelvin-Helmholtz example
+        gauss = {}
+        for name, kw in (("riemann4_2d", {"basis": "gauss"}), ("kelvin_helmholtz_gauss", {})):
+            solver, state, sp = pkg.configs.build(name, **kw)
+            fp = pkg.flat_problem(solver, sp)
+            U = state.preallocation.Uq
+            n = fp.K * fp.Nq
+            dev, w64 = time_fp64(DeviceSolver, fp, U, args.steps, args.warmup)
+            rep = dev.check_positivity()
+            dev.set_state(U)
+            dev.set_time(0.0)
+            dev.set_kernel_timing(True)
+            dev.enqueue_steps(min(args.steps, 10))
+            dev.synchronize()
+            ms, cnt = dev.kernel_timing()
+            dev.set_kernel_timing(False)
+            steps_t = max(cnt[1], 1) / 3
+            del dev
+            gauss[name] = {"dofs": n, "fp64_dropin": n * 3 * args.steps / w64,
+                           "ms_per_step": w64 / args.steps * 1e3,
+                           "projection_and_element_kernels_ms_per_step": ms[0] / steps_t,
+                           "update_kernel_ms_per_step": ms[1] / steps_t,
+                           "positivity_violations": int(rep.violations), "min_p": rep.min_p}
+        cmp["gauss_fp64"] = gauss
         out["config"]["comparison"] = cmp
     if not args.no_cpu:
         try:
