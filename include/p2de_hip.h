@@ -45,7 +45,7 @@ enum { P2DE_NO_SHOCK_CAPTURE = 0 };
  * subcell flux potentials, limiter bounds and coefficients, RK stages and state.  A context
  * created with MIXED holds a one-instance p2de_batch and forwards every call to it. */
 enum { P2DE_PRECISION_FP64 = 0, P2DE_PRECISION_MIXED = 1 };
-/* Upper density bound of the subcell limiter.  REFERENCE: as limiter_utils.jl:67
+/* Upper density bound of the subcell limiter.  REFERENCE: as limiter_utils.jl:11
  * destructures the 5-value bound tuple of subcell.jl:299 into 4 names, so
  * Urho = Lphi (0 for PositivityBound).  INTENDED: Urho = Urhoe = Inf. */
 enum { P2DE_UPPER_BOUND_REFERENCE = 0, P2DE_UPPER_BOUND_INTENDED = 1 };

@@ -601,7 +601,7 @@ static double rhs_low_graph_visc(oracle_ctx *c, const double *Uq, double t, doub
             double lam_ij = nrm * ws;
             c->lambda[i + (int64_t)Nq * (j + (int64_t)Nq * k)] = lam_ij;
             c->lambda[j + (int64_t)Nq * (i + (int64_t)Nq * k)] = lam_ij;
-            /* graph_viscosity :237-261 */
+            /* graph_viscosity rhs_utils.jl:105-128 */
             double visc[NC_MAX];
             for (int cc = 0; cc < Nc; cc++) visc[cc] = lam_ij * (u_j[cc] - u_i[cc]);
             int vdir = 0;
@@ -876,7 +876,7 @@ static void rhs_fluxdiff(oracle_ctx *c, const double *Uq, int nstage, int nthr) 
 /* ------------------------------------------------------------------------------- */
 /* Subcell limiter: limiter.jl:20-56, subcell.jl, limiter_utils.jl                  */
 /* ------------------------------------------------------------------------------- */
-/* rhoe_quadratic_coefficients limiter_utils.jl:134-146 and rhoe_quadratic_solve :108-132 */
+/* rhoe_quadratic_coefficients limiter_utils.jl:78-90 and rhoe_quadratic_solve :52-76 */
 static double rhoe_quadratic_solve(const oracle_ctx *c, const double *U, const double *P, double Lrhoe) {
     double ZEROTOL = c->p.ZEROTOL;
     if (Lrhoe == INFINITY) return 1.0;
@@ -904,7 +904,7 @@ static double rhoe_quadratic_solve(const oracle_ctx *c, const double *U, const d
     }
     return l_eps_ij;
 }
-/* limiting_param_bound_rho_rhoe limiter_utils.jl:82-96 */
+/* limiting_param_bound_rho_rhoe limiter_utils.jl:26-40 */
 static double limiting_param_bound_rho_rhoe(const oracle_ctx *c, const double *U, const double *P,
                                             double Lrho, double Lrhoe, double Urho, double Urhoe) {
     double l = 1.0;
@@ -913,7 +913,7 @@ static double limiting_param_bound_rho_rhoe(const oracle_ctx *c, const double *U
     l = jmin(jmin(l, rhoe_quadratic_solve(c, U, P, Lrhoe)), rhoe_quadratic_solve(c, U, P, Urhoe));
     return l;
 }
-/* limiting_param(::SubcellLimiter, ::PositivityBound) limiter_utils.jl:66-70 with the bound
+/* limiting_param(::SubcellLimiter, ::PositivityBound) limiter_utils.jl:10-14 with the bound
  * tuple of subcell.jl:231,299 = (Lrho, Lrhoe, Lphi, Urho, Urhoe) */
 static double limiting_param(const oracle_ctx *c, const double *uL, const double *P, double Lphi) {
     double Lrho = c->p.zeta * uL[0];            /* rho_bound(::PositivityBound) subcell.jl:370-387 */
@@ -1111,7 +1111,7 @@ static void apply_rhs_limiter(oracle_ctx *c, const double *Uq, double dt, int ns
             LLOC(NLs - 1, 0, km1, s) = l;
         }
     } else {
-        /* subcell.jl:418-456 via subcell_index_P_x/_y (limiter_utils.jl:179-237) */
+        /* subcell.jl:418-456 via subcell_index_P_x/_y (limiter_utils.jl:123-181) */
         int N1Dp1 = N1D + 1;
         for (int k = 0; k < K; k++)
             for (int d = 0; d < 2; d++)

@@ -33,7 +33,7 @@ typedef struct {
     int Nd, Nc, N1D, Nq, Nfp, Nh, K;
     /* Param scalars (Solver.jl:123-170) */
     double gamma, zeta, eta, POSTOL, ZEROTOL, CFL, dt0, t0, T;
-    /* 0: bound tuple destructured as the reference does (limiter_utils.jl:67 takes 4 of
+    /* 0: bound tuple destructured as the reference does (limiter_utils.jl:11 takes 4 of
      *    the 5 values built at subcell.jl:299, so Urho = Lphi = 0 for PositivityBound);
      * 1: the evident intent, Urho = Urhoe = Inf. */
     int upper_bound_mode;

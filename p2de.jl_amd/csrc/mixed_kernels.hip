@@ -117,7 +117,7 @@ __device__ __forceinline__ void fS_f(float g2m1, const NodeP& L, const NodeP& R,
 
 // limiting_param(::SubcellLimiter, ::PositivityBound) (limiter_utils.jl:10-14, 26-40, 52-90) in
 // FP64 with precomputed bounds Lrho = zeta*rho(uL), Lrhoe = zeta*rhoe(uL).  upper_mode 0
-// keeps the reference's bound tuple destructuring (Urho = 0, limiter_utils.jl:67).
+// keeps the reference's bound tuple destructuring (Urho = 0, limiter_utils.jl:11).
 __device__ __forceinline__ double lim_param(const double* uL, double Lrho, double Lrhoe, const double* P,
                                             double ZEROTOL, int upper_mode) {
     double l = 1.0;

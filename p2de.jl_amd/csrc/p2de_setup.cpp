@@ -288,7 +288,7 @@ int setup_mesh(int Nd, int N1D, int K, const ElemTables& T, const p2de_geometry*
             if (g.sxJ != 0.0 || g.ryJ != 0.0)
                 return fail_s(err, P2DE_E_UNSUPPORTED, "elements must be axis aligned (sxJ = ryJ = 0)");
         }
-        // the graph-viscosity direction test |Sx0| > 1e-10 (rhs_utils.jl:256) must pick the
+        // the graph-viscosity direction test |Sx0| > 1e-10 (rhs_utils.jl:122) must pick the
         // direction the pair acts in
         for (int p = 0; p < T.npl && Nd == 2; p++) {
             const double s0x = g.rxJ * T.lp_cr[p] + g.sxJ * T.lp_cs[p];

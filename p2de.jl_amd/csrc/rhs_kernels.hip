@@ -76,7 +76,7 @@ __device__ __forceinline__ void make_prim(double g, const double* u, Prim<DIM>& 
     P.lbeta = log(P.beta);
 }
 
-// rhoe_quadratic_solve / _coefficients (limiter_utils.jl:108-146)
+// rhoe_quadratic_solve / _coefficients (limiter_utils.jl:52-90)
 template <int DIM>
 __device__ __forceinline__ double rhoe_quadratic_solve(const double* U, const double* P, double Lrhoe, double ZEROTOL) {
     if (Lrhoe == INFINITY) return 1.0;
@@ -105,7 +105,7 @@ __device__ __forceinline__ double rhoe_quadratic_solve(const double* U, const do
     return l;
 }
 
-// limiting_param(::SubcellLimiter, ::PositivityBound) (limiter_utils.jl:66-70, 82-96) with the
+// limiting_param(::SubcellLimiter, ::PositivityBound) (limiter_utils.jl:10-14, 26-40) with the
 // bound tuple of subcell.jl:299 = (zeta*rho_L, zeta*rhoe(u_L), Lphi = 0, Inf, Inf).
 // upper_mode 0 reproduces the reference's 4-of-5 destructuring (Urho = Lphi = 0).
 template <int DIM>
@@ -310,7 +310,7 @@ __global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
         fS_dir<DIM>(g, Pi, Pj, dir, F);
         for (int c = 0; c < NC; c++) sm.x.p.PH[e][p][c] = Sd * F[c];
     }
-    // ---- phase 3b: low-order volume pairs (accumulate_low_order_rhs_volume! :139-166, graph_viscosity rhs_utils.jl:237-261)
+    // ---- phase 3b: low-order volume pairs (accumulate_low_order_rhs_volume! :139-166, graph_viscosity rhs_utils.jl:105-128)
     for (int w = tid; w < E * NPL; w += NT) {
         const int e = w / NPL, p = w - e * NPL;
         const int k = k0 + e;

@@ -26,7 +26,7 @@ def limiter_mismatch(orc, L_gpu, fp, nstage):
     above roundoff.  Where the jump is roundoff (the element-boundary faces, where the
     high- and low-order surface fluxes agree to the last bits) the reference's own
     parameter is decided by the sign of that noise under its Urho = 0 bound
-    (limiter_utils.jl:67), and either value changes rhsU only at roundoff."""
+    (limiter_utils.jl:11), and either value changes rhsU only at roundoff."""
     NLs = fp.Nq + fp.N1D
     L_cpu = orc.field("L_local").reshape(3, fp.K, fp.Nd, NLs)[nstage - 1]
     L_gpu = L_gpu.reshape(fp.K, fp.Nd, NLs)

@@ -155,7 +155,7 @@ def test_limiting_param_matches_brute_force(Nd):
 
 
 def test_limiting_param_reference_bounds_quirk():
-    """Reference bounds: limiter_utils.jl:67 reads Urho from the tuple's third slot
+    """Reference bounds: limiter_utils.jl:11 reads Urho from the tuple's third slot
     (Lphi = 0), so any step with P_rho >= +0 and rho_L + P_rho > 0 gets l = 0."""
     uL = np.array([1.0, 0.1, 0.0, 2.5])
     P = np.array([1e-3, 0.0, 0.0, 0.0])
