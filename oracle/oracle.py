@@ -20,7 +20,9 @@ LIB_PATH = HERE / "libp2de_oracle.so"
 FIELDS = {
     "rhsU": 0, "rhsH": 1, "rhsL": 2, "rhsxyH": 3, "rhsxyL": 4, "L_local": 5, "BF_H": 6, "BF_L": 7,
     "u_tilde": 8, "QF1": 9, "lambda": 10, "lambdaB": 11, "f_bar_H": 12, "f_bar_L": 13, "vq": 14,
-    "psi_tilde": 15, "fstar_H": 16, "fstar_L": 17, "theta_local": 18,
+    "psi_tilde": 15, "fstar_H": 16, "fstar_L": 17, "theta_local": 18, "smooth_indicator": 19,
+    "blending_factor": 20, "smooth_factor": 21, "lbound_s_modified": 22, "L_zhangshu": 23,
+    "s_modified_min": 24,
 }
 
 _dp = C.POINTER(C.c_double)
@@ -33,10 +35,11 @@ class _Problem(C.Structure):
         ("Nh", C.c_int), ("K", C.c_int),
         ("gamma", C.c_double), ("zeta", C.c_double), ("eta", C.c_double), ("POSTOL", C.c_double),
         ("ZEROTOL", C.c_double), ("CFL", C.c_double), ("dt0", C.c_double), ("t0", C.c_double), ("T", C.c_double),
-        ("upper_bound_mode", C.c_int), ("entropyproj_limiter", C.c_int),
+        ("upper_bound_mode", C.c_int), ("entropyproj_limiter", C.c_int), ("rhs_limiter", C.c_int),
+        ("bound", C.c_int), ("shockcapture", C.c_int), ("shock_a", C.c_double), ("shock_c", C.c_double),
         ("Srsh_db", _dp), ("Srs0", _dp), ("nnz_h", C.c_int), ("Srsh_nnz", _ip), ("nnz_0", C.c_int),
         ("Srs0_nnz", _ip), ("Brs", _dp), ("MinvVhT", _dp), ("MinvVfT", _dp), ("Vf", _dp), ("Vf_low", _dp),
-        ("wq", _dp), ("fq2q", _ip), ("q2fq", _ip), ("Jq", _dp), ("GJh", _dp), ("mapP", _ip),
+        ("wq", _dp), ("fq2q", _ip), ("q2fq", _ip), ("VDM_inv", _dp), ("Jq", _dp), ("GJh", _dp), ("mapP", _ip),
         ("nI", C.c_int), ("mapI", _ip), ("Ival", _dp), ("nO", C.c_int), ("mapO", _ip),
     ]
 
@@ -101,6 +104,8 @@ class Oracle:
         self._keep = {}
         for name in ("Srsh_db", "Srs0", "Brs", "MinvVhT", "MinvVfT", "Vf", "Vf_low", "wq", "Jq", "GJh", "Ival"):
             self._keep[name] = np.ascontiguousarray(getattr(fp, name), dtype=np.float64)
+        vinv = getattr(fp, "VDM_inv", None)
+        self._keep["VDM_inv"] = None if vinv is None else np.ascontiguousarray(vinv, dtype=np.float64)
         for name in ("Srsh_nnz", "Srs0_nnz", "fq2q", "q2fq", "mapP", "mapI", "mapO"):
             self._keep[name] = np.ascontiguousarray(getattr(fp, name), dtype=np.int64)
         k = self._keep
@@ -110,6 +115,9 @@ class Oracle:
             gamma=fp.gamma, zeta=fp.zeta, eta=fp.eta, POSTOL=fp.POSTOL, ZEROTOL=fp.ZEROTOL,
             CFL=fp.CFL, dt0=fp.dt0, t0=fp.t0, T=fp.T, upper_bound_mode=int(mode),
             entropyproj_limiter=int(getattr(fp, "entropyproj_limiter", 0)),
+            rhs_limiter=int(getattr(fp, "rhs_limiter", 0)), bound=int(getattr(fp, "bound", 0)),
+            shockcapture=int(getattr(fp, "shockcapture", 0)), shock_a=float(getattr(fp, "shock_a", 0.5)),
+            shock_c=float(getattr(fp, "shock_c", 1.8)), VDM_inv=_d(self._keep["VDM_inv"]),
             Srsh_db=_d(k["Srsh_db"]), Srs0=_d(k["Srs0"]), nnz_h=len(k["Srsh_nnz"]) // 2,
             Srsh_nnz=_i(k["Srsh_nnz"]), nnz_0=len(k["Srs0_nnz"]) // 2, Srs0_nnz=_i(k["Srs0_nnz"]),
             Brs=_d(k["Brs"]), MinvVhT=_d(k["MinvVhT"]), MinvVfT=_d(k["MinvVfT"]), Vf=_d(k["Vf"]),

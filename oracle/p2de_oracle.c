@@ -222,6 +222,9 @@ struct oracle_ctx {
     double *dtarr;
     /* EntropyProjectionLimiterCache (State.jl:87-129) and theta_local (State.jl:20) */
     double *epUf, *epVUf, *eprhoef, *theta_local;
+    /* smoothness indicator (State.jl:23-25), entropy bounds (State.jl:148-153), Zhang-Shu L */
+    double *indicator, *indicator_modal, *smooth_indicator, *s_modified, *L_zs;
+    double s_modified_min;
     int nthreads_alloc;
     double *stage_buf;       /* rhsU copy for the RK driver */
 };
@@ -244,6 +247,10 @@ int oracle_create(const oracle_problem *p, oracle_ctx **out) {
     if (p->Nd != 1 && p->Nd != 2) return 2;
     if (p->Nc != p->Nd + 2) return 2;
     if (p->entropyproj_limiter != 0 && p->entropyproj_limiter != 1) return 2;
+    if (p->rhs_limiter < 0 || p->rhs_limiter > 1 || p->bound < 0 || p->bound > 2 || p->shockcapture < 0 ||
+        p->shockcapture > 1)
+        return 2;
+    if (!p->VDM_inv && (p->shockcapture != 0 || p->bound == 2)) return 2;
     oracle_ctx *c = (oracle_ctx *)calloc(1, sizeof(oracle_ctx));
     if (!c) return 3;
     c->p = *p;
@@ -302,6 +309,12 @@ int oracle_create(const oracle_problem *p, oracle_ctx **out) {
     c->eprhoef = zalloc(Nfp * K);
     c->theta_local = zalloc(Nfp * K * 3);
     for (int64_t x = 0; x < Nfp * K * 3; x++) c->theta_local[x] = 1.0;
+    c->indicator = zalloc(Nq * K);
+    c->indicator_modal = zalloc(Nq * K);
+    c->smooth_indicator = zalloc(K);
+    c->s_modified = zalloc(Nq * K);
+    c->L_zs = zalloc(K * 3);
+    c->s_modified_min = 0.0;   /* SubcellLimiterCache: zeros(Float64, 1) (State.jl:180) */
     *out = c;
     return 0;
 }
@@ -315,7 +328,8 @@ void oracle_destroy(oracle_ctx *c) {
                     &c->uPH, &c->betaP, &c->rhologP, &c->betalogP, &c->lam, &c->LFc, &c->QF1,
                     &c->MinvVhTQF1, &c->MinvVfTBF1, &c->f_bar_H, &c->f_bar_L, &c->f_bar_lim,
                     &c->lbound_s_modified, &c->uL, &c->blending_factor, &c->smooth_factor,
-                    &c->dtarr, &c->stage_buf, &c->epUf, &c->epVUf, &c->eprhoef, &c->theta_local};
+                    &c->dtarr, &c->stage_buf, &c->epUf, &c->epVUf, &c->eprhoef, &c->theta_local,
+                    &c->indicator, &c->indicator_modal, &c->smooth_indicator, &c->s_modified, &c->L_zs};
     for (size_t i = 0; i < sizeof(f) / sizeof(f[0]); i++) free(*f[i]);
     free(c);
 }
@@ -335,6 +349,10 @@ int64_t oracle_field_size(const oracle_ctx *c, int field) {
     case ORACLE_FBAR_H: case ORACLE_FBAR_L: return Nd * (int64_t)c->NLs * K * Nc;
     case ORACLE_PSI_TILDE: return Nh * K * Nd;
     case ORACLE_THETA_LOCAL: return Nfp * K * 3;
+    case ORACLE_SMOOTH_INDICATOR: return K;
+    case ORACLE_BLENDING_FACTOR: case ORACLE_SMOOTH_FACTOR: case ORACLE_L_ZHANGSHU: return K * 3;
+    case ORACLE_LBOUND_S: return Nq * K;
+    case ORACLE_S_MODIFIED_MIN: return 1;
     default: return -1;
     }
 }
@@ -360,6 +378,12 @@ const double *oracle_field(const oracle_ctx *c, int field) {
     case ORACLE_FSTAR_H: return c->fstar_H;
     case ORACLE_FSTAR_L: return c->fstar_L;
     case ORACLE_THETA_LOCAL: return c->theta_local;
+    case ORACLE_SMOOTH_INDICATOR: return c->smooth_indicator;
+    case ORACLE_BLENDING_FACTOR: return c->blending_factor;
+    case ORACLE_SMOOTH_FACTOR: return c->smooth_factor;
+    case ORACLE_LBOUND_S: return c->lbound_s_modified;
+    case ORACLE_L_ZHANGSHU: return c->L_zs;
+    case ORACLE_S_MODIFIED_MIN: return &c->s_modified_min;
     default: return NULL;
     }
 }
@@ -913,12 +937,41 @@ static double limiting_param_bound_rho_rhoe(const oracle_ctx *c, const double *U
     l = jmin(jmin(l, rhoe_quadratic_solve(c, U, P, Lrhoe)), rhoe_quadratic_solve(c, U, P, Urhoe));
     return l;
 }
-/* limiting_param(::SubcellLimiter, ::PositivityBound) limiter_utils.jl:10-14 with the bound
- * tuple of subcell.jl:231,299 = (Lrho, Lrhoe, Lphi, Urho, Urhoe) */
+/* s_modified_ufun compressible_Navier_Stokes.jl:80-85 */
+static inline double s_modified_ufun(int Nd, double g, const double *U) {
+    double rho = U[0];
+    double rhoe = rhoe_ufun(Nd, U);
+    return rhoe * pow(rho, -g);
+}
+/* limiting_param_bound_phi limiter_utils.jl:42-50 with bisection nonlinear_solvers.jl:3-20
+ * (x_valid = 0, x_invalid = lpos) */
+static int phi_admissible(const oracle_ctx *c, const double *U, const double *P, double Lphi, double l) {
+    double Ul[NC_MAX];
+    for (int cc = 0; cc < c->p.Nc; cc++) Ul[cc] = U[cc] + l * P[cc];
+    return s_modified_ufun(c->p.Nd, c->p.gamma, Ul) >= Lphi - c->p.POSTOL;
+}
+static double limiting_param_bound_phi(const oracle_ctx *c, const double *U, const double *P, double Lphi, double lpos) {
+    if (phi_admissible(c, U, P, Lphi, lpos)) return lpos;
+    double x_valid = 0.0, x_invalid = lpos;
+    for (int iter = 0; iter <= 20; iter++) {
+        double x_new = 0.5 * (x_valid + x_invalid);
+        if (phi_admissible(c, U, P, Lphi, x_new)) x_valid = x_new;
+        else x_invalid = x_new;
+    }
+    return x_valid;
+}
+/* limiting_param(::SubcellLimiter, bound) limiter_utils.jl:10-24 with the bound tuple of
+ * subcell.jl:231,299 = (Lrho, Lrhoe, Lphi, Urho, Urhoe) and rho_bound(::Positivity...)
+ * subcell.jl:370-387 (Lrho = zeta rho_L, Urho = Inf) */
 static double limiting_param(const oracle_ctx *c, const double *uL, const double *P, double Lphi) {
-    double Lrho = c->p.zeta * uL[0];            /* rho_bound(::PositivityBound) subcell.jl:370-387 */
+    double Lrho = c->p.zeta * uL[0];
     double Urho_i = INFINITY, Urhoe = INFINITY;
     double Lrhoe = c->p.zeta * rhoe_ufun(c->p.Nd, uL);
+    if (c->p.bound != 0) {
+        /* min-entropy bounds destructure all five values (limiter_utils.jl:20-22) */
+        double lpos = limiting_param_bound_rho_rhoe(c, uL, P, Lrho, Lrhoe, Urho_i, Urhoe);
+        return limiting_param_bound_phi(c, uL, P, Lphi, lpos);
+    }
     if (c->p.upper_bound_mode == 0) /* `Lrho, Lrhoe, Urho, Urhoe = bound_val` takes the first four */
         return limiting_param_bound_rho_rhoe(c, uL, P, Lrho, Lrhoe, Lphi, Urho_i);
     return limiting_param_bound_rho_rhoe(c, uL, P, Lrho, Lrhoe, Urho_i, Urhoe);
@@ -963,19 +1016,191 @@ double oracle_pt_limiting_param(int Nd, double zeta, double ZEROTOL, int upper_b
 #define LLOC(i, d, k, s) (c->L_local[(int64_t)(i) + (int64_t)c->NLs * ((d) + (int64_t)Nd * ((int64_t)(k) + (int64_t)K * (s)))])
 #define FB(arr, d, i, k) ((arr) + (int64_t)(d) * c->NLs * K * Nc + (int64_t)Nc * ((int64_t)(i) + (int64_t)c->NLs * (k)))
 
-static void apply_rhs_limiter(oracle_ctx *c, const double *Uq, double dt, int nstage, int nthr) {
+/* initialize_indicator!(::CompressibleIdealGas) shock_capture.jl:82-94 and
+ * initialize_smoothness_indicator!(::Dim1 / ::Dim2) :14-80: rho*p at the nodes, its modal
+ * coefficients (generic mul! with VDM_inv), the energy share of the top two mode shells */
+static void smoothness_indicator(oracle_ctx *c, const double *Uq, int nthr) {
+    const oracle_problem *p = &c->p;
+    int K = p->K, Nq = p->Nq, Nc = p->Nc, Nd = p->Nd, N = p->N1D - 1;
+    double g = p->gamma;
+    OMP_FOR
+    for (int k = 0; k < K; k++) {
+        double *ind = c->indicator + (int64_t)Nq * k, *mod = c->indicator_modal + (int64_t)Nq * k;
+        for (int i = 0; i < Nq; i++) {
+            const double *Ui = Uq + IDX_V(i, k, Nq, Nc);
+            ind[i] = Ui[0] * pfun(Nd, g, Ui);
+        }
+        for (int i = 0; i < Nq; i++) mod[i] = 0.0;
+        for (int j = 0; j < Nq; j++)
+            for (int i = 0; i < Nq; i++) mod[i] += p->VDM_inv[i + (int64_t)Nq * j] * ind[j];
+        int count = 0;
+        double modeN = 0.0, modeNm1 = 0.0, total = 0.0;
+        if (Nd == 1) {
+            for (int i = 0; i <= N; i++) {
+                double energy = mod[count] * mod[count];
+                if (i == N) modeN += energy;
+                if (i == N - 1) modeNm1 += energy;
+                total += energy;
+                count++;
+            }
+        } else {
+            for (int j = 0; j <= N; j++)
+                for (int i = 0; i <= N; i++) {
+                    double energy = mod[count] * mod[count];
+                    if (i == N || j == N) modeN += energy;
+                    if (i == N - 1 || j == N - 1) modeNm1 += energy;
+                    total += energy;
+                    count++;
+                }
+        }
+        c->smooth_indicator[k] = jmax(modeN / total, modeNm1 / total);
+    }
+}
+
+/* update_blending_factor! shock_capture.jl:111-132 (Hennemann et al., eq. (46)) */
+static void update_blending_factor(oracle_ctx *c, int s) {
+    const oracle_problem *p = &c->p;
+    int K = p->K, N = p->N1D - 1;
+    if (p->shockcapture == 0) {
+        for (int k = 0; k < K; k++) c->blending_factor[k + (int64_t)K * s] = 1.0;
+        return;
+    }
+    double TN = p->shock_a * pow(10.0, -p->shock_c * pow((double)(N + 1), 0.25));
+    double alphamax = 0.5, alphaE0 = 0.0001;
+    double s_factor = log((1.0 - alphaE0) / alphaE0);
+    for (int k = 0; k < K; k++) {
+        double alpha = 1.0 / (1.0 + exp(-s_factor / TN * (c->smooth_indicator[k] - TN)));
+        c->blending_factor[k + (int64_t)K * s] = jmax(jmin(1.0 - alpha, 1.0), alphamax);
+    }
+}
+
+/* update_smoothness_factor! subcell.jl:927-956 */
+static void update_smoothness_factor(oracle_ctx *c, int s) {
+    const oracle_problem *p = &c->p;
+    int K = p->K, N = p->N1D - 1;
+    for (int k = 0; k < K; k++) {
+        double f;
+        if (p->bound == 0) {
+            f = 0.0;
+        } else if (p->bound == 1) {
+            f = 1.0;
+        } else {
+            double kappa = 1.0;
+            double s0 = log(pow((double)N, -4.0)) / log(10.0);   /* log(10, N^-4) */
+            double sk = log(c->smooth_indicator[k]) / log(10.0);
+            if (sk < s0 - kappa) f = 0.0;
+            else if (sk > s0 + kappa) f = 1.0;
+            else f = 0.5 - 0.5 * sin(3.141592653589793 * (sk - s0) / (2.0 * kappa));   /* Float64(pi) */
+        }
+        c->smooth_factor[k + (int64_t)K * s] = f;
+    }
+}
+
+/* the stencil node across face q2fq[i][m] (quad_index_to_quad_index_P limiter_utils.jl:184-212):
+ * element kP and node iP (0-based) */
+static void node_across(const oracle_ctx *c, int i, int k, int m, int *iP, int *kP) {
+    const oracle_problem *p = &c->p;
+    int Nq = p->Nq, Nfp = p->Nfp;
+    int iface = (int)p->q2fq[i + (int64_t)Nq * m] - 1;
+    int64_t mp = p->mapP[iface + (int64_t)Nfp * k] - 1;
+    *iP = (int)p->fq2q[mp % Nfp] - 1;
+    *kP = (int)(mp / Nfp);
+}
+
+/* initialize_entropy_bounds! subcell.jl:4-75: s_modified at the nodes, its global minimum at the
+ * first stage of t0, and the lower bound epsk*min(stencil) + (1-epsk)*min(s_modified(U0)) with
+ * the low-order stencil of low_order_stencil limiter_utils.jl:214-231 */
+static void initialize_entropy_bounds(oracle_ctx *c, const double *Uq, double t, int nstage, int nthr) {
+    const oracle_problem *p = &c->p;
+    int K = p->K, Nq = p->Nq, Nc = p->Nc, Nd = p->Nd, N1D = p->N1D;
+    int s = nstage - 1;
+    if (p->bound == 0) {
+        memset(c->lbound_s_modified, 0, sizeof(double) * (size_t)Nq * K);
+        return;
+    }
+    double g = p->gamma;
+    OMP_FOR
+    for (int k = 0; k < K; k++)
+        for (int i = 0; i < Nq; i++) c->s_modified[i + (int64_t)Nq * k] = s_modified_ufun(Nd, g, Uq + IDX_V(i, k, Nq, Nc));
+    if (t == p->t0 && nstage == 1) {
+        double m = c->s_modified[0];
+        for (int64_t x = 1; x < (int64_t)Nq * K; x++) m = jmin(m, c->s_modified[x]);
+        c->s_modified_min = m;
+    }
+    OMP_FOR
+    for (int k = 0; k < K; k++) {
+        double epsk = c->smooth_factor[k + (int64_t)K * s];
+        for (int q = 0; q < Nq; q++) {
+            int si = Nd == 1 ? q : q % N1D, sj = Nd == 1 ? 0 : q / N1D;
+            /* stencil (sl, sr[, sb, st]); across the element boundary by face q2fq[q][direction],
+             * where a node on one face only takes its one face */
+            int nodes[4], elems[4], n = 0;
+            int nfq = p->q2fq[q + (int64_t)Nq * 1] > 0 ? 2 : 1;
+            for (int d = 0; d < Nd; d++) {
+                int pos = d == 0 ? si : sj;
+                for (int side = -1; side <= 1; side += 2) {
+                    int np = pos + side;
+                    if (np >= 0 && np < N1D) {
+                        nodes[n] = d == 0 ? (Nd == 1 ? np : np + N1D * sj) : si + N1D * np;
+                        elems[n] = k;
+                    } else {
+                        node_across(c, q, k, nfq == 1 ? 0 : d, &nodes[n], &elems[n]);
+                    }
+                    n++;
+                }
+            }
+            double lb = c->s_modified[q + (int64_t)Nq * k];
+            for (int x = 0; x < n; x++) lb = jmin(lb, c->s_modified[nodes[x] + (int64_t)Nq * elems[x]]);
+            c->lbound_s_modified[q + (int64_t)Nq * k] = epsk * lb + (1.0 - epsk) * c->s_modified_min;
+        }
+    }
+}
+
+/* apply_zhang_shu_limiter! zhangshu.jl:4-24 with zhang_shu_bound_limiter! :35-45 and
+ * limiting_param(::ZhangShuLimiter) limiter_utils.jl:4-8 */
+static void apply_zhang_shu_limiter(oracle_ctx *c, const double *Uq, double dt, int nstage, int nthr) {
+    const oracle_problem *p = &c->p;
+    int K = p->K, Nq = p->Nq, Nc = p->Nc, Nd = p->Nd;
+    int s = nstage - 1;
+    double zeta = p->zeta;
+    OMP_FOR
+    for (int k = 0; k < K; k++) {
+        double l = 1.0;
+        for (int i = 0; i < Nq; i++) {
+            double uL[NC_MAX], P[NC_MAX];
+            for (int cc = 0; cc < Nc; cc++) {
+                uL[cc] = Uq[IDX_V(i, k, Nq, Nc) + cc] + dt * c->rhsL[IDX_V(i, k, Nq, Nc) + cc];
+                P[cc] = dt * (c->rhsH[IDX_V(i, k, Nq, Nc) + cc] - c->rhsL[IDX_V(i, k, Nq, Nc) + cc]);
+            }
+            double li = jmin(1.0, limiting_param_bound_rho_rhoe(c, uL, P, zeta * uL[0], zeta * rhoe_ufun(Nd, uL),
+                                                                 INFINITY, INFINITY));
+            l = jmin(l, li);
+        }
+        c->L_zs[k + (int64_t)K * s] = l;
+        l = jmin(c->L_zs[k + (int64_t)K * s], c->blending_factor[k + (int64_t)K * s]);
+        for (int i = 0; i < Nq; i++)
+            for (int cc = 0; cc < Nc; cc++)
+                c->rhsU[IDX_V(i, k, Nq, Nc) + cc] =
+                    (1.0 - l) * c->rhsL[IDX_V(i, k, Nq, Nc) + cc] + l * c->rhsH[IDX_V(i, k, Nq, Nc) + cc];
+    }
+}
+
+static void apply_rhs_limiter(oracle_ctx *c, const double *Uq, double t, double dt, int nstage, int nthr) {
     const oracle_problem *p = &c->p;
     int K = p->K, Nq = p->Nq, Nfp = p->Nfp, Nc = p->Nc, Nd = p->Nd, N1D = p->N1D;
     int s = nstage - 1;
     int NLs = c->NLs;
-    /* initialize_smoothness_indicator!(NoShockCapture, PositivityBound) shock_capture.jl:4-6: nothing */
-    /* update_blending_factor!(NoShockCapture) shock_capture.jl:111-114 */
-    for (int k = 0; k < K; k++) c->blending_factor[k + (int64_t)K * s] = 1.0;
-    /* update_smoothness_factor!(PositivityBound) subcell.jl:927-930 */
-    for (int k = 0; k < K; k++) c->smooth_factor[k + (int64_t)K * s] = 0.0;
-    /* initialize_entropy_bounds!(PositivityBound) subcell.jl:4-6 */
-    memset(c->lbound_s_modified, 0, sizeof(double) * (size_t)Nq * K);
-    /* initialize_TVD_bounds!(PositivityBound) subcell.jl:82-84: nothing */
+    /* initialize_smoothness_indicator! shock_capture.jl:4-12: nothing for (NoShockCapture,
+     * PositivityBound), which is also the bound the Zhang-Shu limiter reports (traits.jl:61-63) */
+    if (p->shockcapture != 0 || (p->rhs_limiter == 0 && p->bound != 0)) smoothness_indicator(c, Uq, nthr);
+    update_blending_factor(c, s);
+    if (p->rhs_limiter == 1) {
+        apply_zhang_shu_limiter(c, Uq, dt, nstage, nthr);
+        return;
+    }
+    update_smoothness_factor(c, s);
+    initialize_entropy_bounds(c, Uq, t, nstage, nthr);
+    /* initialize_TVD_bounds! for the positivity-type bounds subcell.jl:82-84: nothing */
 
     /* accumulate_f_bar! subcell.jl:144-206 */
     OMP_FOR
@@ -1201,7 +1426,7 @@ int oracle_rhs(oracle_ctx *c, const double *Uq, double t, double dt, int nstage,
     entropy_projection(c, Uq, nstage, nthr);
     double dt_ret = rhs_low_graph_visc(c, Uq, t, dt, nstage, nthr);
     rhs_fluxdiff(c, Uq, nstage, nthr);
-    apply_rhs_limiter(c, Uq, dt, nstage, nthr);
+    apply_rhs_limiter(c, Uq, t, dt, nstage, nthr);
     if (dt_out) *dt_out = dt_ret;
     return 0;
 }

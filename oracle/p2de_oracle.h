@@ -11,7 +11,9 @@
  *   src/math/compressible_Navier_Stokes.jl, src/timestepping/SSPRK33.jl
  * and src/dg/filter.jl, src/math/nonlinear_solvers.jl
  * for the option set ESLimitedLowOrderPos(LaxFriedrichsOnNodalVal,
- * LaxFriedrichsOnProjectedVal) + SubcellLimiter(PositivityBound, NoShockCapture) with
+ * LaxFriedrichsOnProjectedVal) + SubcellLimiter(PositivityBound, PositivityAndMinEntropyBound or
+ * PositivityAndRelaxedMinEntropyBound) or ZhangShuLimiter, each with NoShockCapture or
+ * HennemannShockCapture, and
  * NoEntropyProjectionLimiter (Lobatto or Gauss operators) or NodewiseScaledExtrapolation
  * (Gauss operators), with the operators passed in (the reference reads them from
  * Solver.discrete_data).
@@ -41,6 +43,15 @@ typedef struct {
      * (filter.jl:6-130, rhs.jl:113-133, flux_differencing.jl:288-328; Gauss basis only).  The
      * basis itself enters through the operators: Vf, Vf_low, MinvVhT, MinvVfT, Srsh_db. */
     int entropyproj_limiter;
+    /* Param.rhs_limiter: 0 SubcellLimiter, 1 ZhangShuLimiter (zhangshu.jl:4-56) */
+    int rhs_limiter;
+    /* SubcellLimiter bound: 0 PositivityBound, 1 PositivityAndMinEntropyBound,
+     * 2 PositivityAndRelaxedMinEntropyBound (subcell.jl:4-75, 927-956, limiter_utils.jl:19-50) */
+    int bound;
+    /* shockcapture: 0 NoShockCapture, 1 HennemannShockCapture(shock_a, shock_c)
+     * (shock_capture.jl:10-132) */
+    int shockcapture;
+    double shock_a, shock_c;
     /* Operators (Solver.jl:192-212) */
     const double *Srsh_db;   /* Nd blocks of Nh x Nh */
     const double *Srs0;      /* Nd blocks of Nq x Nq */
@@ -56,6 +67,8 @@ typedef struct {
     const double *wq;        /* Nq */
     const int64_t *fq2q;     /* Nfp, 1-based */
     const int64_t *q2fq;     /* Nq x 2, 1-based, 0 = absent (column-major: q2fq[i + Nq*m]) */
+    const double *VDM_inv;   /* Nq x Nq, inverse of the orthonormal-Legendre Vandermonde (smoothness
+                                indicator; may be NULL when no option reads it) */
     /* GeomData (Solver.jl:185-190) */
     const double *Jq;        /* Nq x K */
     const double *GJh;       /* NGEO blocks of Nh x K: rxJh (, sxJh, ryJh, syJh) */
@@ -91,6 +104,12 @@ enum {
     ORACLE_FSTAR_H,    /* (Nfp, K) x Nd x Nc */
     ORACLE_FSTAR_L,
     ORACLE_THETA_LOCAL, /* (Nfp, K, 3) entropy-projection limiting parameters */
+    ORACLE_SMOOTH_INDICATOR, /* (K) */
+    ORACLE_BLENDING_FACTOR,  /* (K, 3) */
+    ORACLE_SMOOTH_FACTOR,    /* (K, 3) */
+    ORACLE_LBOUND_S,         /* (Nq, K) lower bound of the modified specific entropy */
+    ORACLE_L_ZHANGSHU,       /* (K, 3) Zhang-Shu limiting parameters */
+    ORACLE_S_MODIFIED_MIN,   /* (1) */
     ORACLE_NFIELDS
 };
 

@@ -35,14 +35,26 @@ import numpy as np
 from .dg import initialize_DG, make_periodic, primitive_to_conservative
 from .operators import BCData
 from .types import (CompressibleEulerIdealGas, Dim1, Dim2, ESLimitedLowOrderPos, GaussCollocation,
-                    GlobalConstant, LimitingParameter, LobattoCollocation, NodewiseScaledExtrapolation,
-                    NoEntropyProjectionLimiter, Param, SubcellLimiter, TimesteppingParameter)
+                    GlobalConstant, HennemannShockCapture, LimitingParameter, LobattoCollocation,
+                    NodewiseScaledExtrapolation, NoEntropyProjectionLimiter, Param,
+                    PositivityAndMinEntropyBound, PositivityAndRelaxedMinEntropyBound, PositivityBound,
+                    SubcellLimiter, TimesteppingParameter, ZhangShuLimiter)
 
 # approximation basis and entropy-projection limiter by name
 BASES = {
     "lobatto": (LobattoCollocation, NoEntropyProjectionLimiter),
     "gauss": (GaussCollocation, NodewiseScaledExtrapolation),        # examples/2D/kelvin-helmholtz.jl:52-53
     "gauss_noep": (GaussCollocation, NoEntropyProjectionLimiter),
+}
+
+# rhs limiters by name: the limiter set-ups of test/test_smoke.jl:44-52 that are implemented
+LIMITERS = {
+    "subcell": lambda: SubcellLimiter(bound=PositivityBound()),
+    "zhangshu": lambda: ZhangShuLimiter(),
+    "zhangshu_shock": lambda: ZhangShuLimiter(shockcapture=HennemannShockCapture()),
+    "subcell_shock": lambda: SubcellLimiter(bound=PositivityBound(), shockcapture=HennemannShockCapture()),
+    "minentropy": lambda: SubcellLimiter(bound=PositivityAndMinEntropyBound()),
+    "relaxed_minentropy": lambda: SubcellLimiter(bound=PositivityAndRelaxedMinEntropyBound()),
 }
 
 # dt0 = 0.9*dt_pos(U0)/CFL, recorded from the initial states (tests/golden/record_dt0.py)
@@ -54,7 +66,7 @@ RECORDED_DT0: Dict[str, float] = {
 }
 
 
-def _param(N, K, xL, xR, T, CFL, dt0, Nd, zeta=0.1, eta=1.0, basis="lobatto"):
+def _param(N, K, xL, xR, T, CFL, dt0, Nd, zeta=0.1, eta=1.0, basis="lobatto", limiter="subcell"):
     B, EPL = BASES[basis]
     return Param(N=N, K=K, xL=xL, xR=xR,
                  global_constants=GlobalConstant(POSTOL=1e-14, ZEROTOL=5e-16),
@@ -64,7 +76,7 @@ def _param(N, K, xL, xR, T, CFL, dt0, Nd, zeta=0.1, eta=1.0, basis="lobatto"):
                  rhs=ESLimitedLowOrderPos(),
                  approximation_basis=B(),
                  entropyproj_limiter=EPL(),
-                 rhs_limiter=SubcellLimiter())
+                 rhs_limiter=LIMITERS[limiter]())
 
 
 def _periodic_bc(param, md):
@@ -90,9 +102,9 @@ def sod_ic(param, x):
     return primitive_to_conservative(g, (rho, u, p))
 
 
-def sod_1d(K=100, N=3, T=0.2, CFL=0.5, dt0=None, basis="lobatto"):
+def sod_1d(K=100, N=3, T=0.2, CFL=0.5, dt0=None, basis="lobatto", limiter="subcell"):
     dt0 = RECORDED_DT0["sod_1d"] if dt0 is None else dt0
-    return _param(N, K, 0.0, 1.0, T, CFL, dt0, 1, basis=basis), sod_ic, _transmissive_bc
+    return _param(N, K, 0.0, 1.0, T, CFL, dt0, 1, basis=basis, limiter=limiter), sod_ic, _transmissive_bc
 
 
 # ------------------------------------------------------------------------------- vortex
@@ -116,9 +128,10 @@ def vortex_ic(param, x, y):
     return primitive_to_conservative(param.equation.gamma, vortex_exact_prim(x, y, t0, param.equation.gamma))
 
 
-def vortex_2d(K=64, N=3, T=1.0, CFL=0.5, dt0=None, basis="lobatto"):
+def vortex_2d(K=64, N=3, T=1.0, CFL=0.5, dt0=None, basis="lobatto", limiter="subcell"):
     dt0 = RECORDED_DT0["vortex_2d"] if dt0 is None else dt0
-    return _param(N, (K, K), (-5.0, -5.0), (5.0, 5.0), T, CFL, dt0, 2, basis=basis), vortex_ic, _periodic_bc
+    return (_param(N, (K, K), (-5.0, -5.0), (5.0, 5.0), T, CFL, dt0, 2, basis=basis, limiter=limiter),
+            vortex_ic, _periodic_bc)
 
 
 # ------------------------------------------------------------------------------- 4-state Riemann
@@ -140,9 +153,10 @@ def riemann4_ic(param, x, y):
     return primitive_to_conservative(param.equation.gamma, tuple(out))
 
 
-def riemann4_2d(K=256, N=3, T=0.25, CFL=0.5, dt0=None, basis="lobatto"):
+def riemann4_2d(K=256, N=3, T=0.25, CFL=0.5, dt0=None, basis="lobatto", limiter="subcell"):
     dt0 = RECORDED_DT0["riemann4_2d"] if dt0 is None else dt0
-    return _param(N, (K, K), (0.0, 0.0), (1.0, 1.0), T, CFL, dt0, 2, basis=basis), riemann4_ic, _transmissive_bc
+    return (_param(N, (K, K), (0.0, 0.0), (1.0, 1.0), T, CFL, dt0, 2, basis=basis, limiter=limiter),
+            riemann4_ic, _transmissive_bc)
 
 
 # ------------------------------------------------------------------------------- Kelvin-Helmholtz
@@ -183,9 +197,10 @@ def smoke_ic(param, x, y):
     return primitive_to_conservative(param.equation.gamma, smoke_exact_prim(x, y, param.timestepping_param.t0))
 
 
-def smoke_vortex(N=3, K=5, T=2e-2, CFL=1.0, dt0=1e-2, basis="lobatto"):
-    """test/test_smoke.jl:117-129 with SubcellLimiter(PositivityBound)."""
-    return (_param(N, (K, K), (0.0, 0.0), (10.0, 10.0), T, CFL, dt0, 2, zeta=0.1, eta=0.5, basis=basis),
+def smoke_vortex(N=3, K=5, T=2e-2, CFL=1.0, dt0=1e-2, basis="lobatto", limiter="subcell"):
+    """test/test_smoke.jl:54-66,117-129; ``limiter`` picks one of its limiter set-ups."""
+    return (_param(N, (K, K), (0.0, 0.0), (10.0, 10.0), T, CFL, dt0, 2, zeta=0.1, eta=0.5, basis=basis,
+                   limiter=limiter),
             smoke_ic, _periodic_bc)
 
 

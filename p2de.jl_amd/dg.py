@@ -18,7 +18,7 @@ import numpy as np
 
 from .operators import (BCData, MeshData, Operators, RefElem, SizeData, build_operators,
                         gauss_ref_elem, lobatto_ref_elem, uniform_mesh)
-from .types import GaussCollocation, NodewiseScaledExtrapolation, Param, check_supported
+from .types import GaussCollocation, NodewiseScaledExtrapolation, Param, check_supported, limiter_codes
 
 
 @dataclass
@@ -187,6 +187,12 @@ class FlatProblem:
     upper_bound_mode: int = 0
     basis: int = 0                  # 0 LobattoCollocation, 1 GaussCollocation
     entropyproj_limiter: int = 0    # 0 NoEntropyProjectionLimiter, 1 NodewiseScaledExtrapolation
+    rhs_limiter: int = 0            # 0 SubcellLimiter, 1 ZhangShuLimiter
+    bound: int = 0                  # 0 PositivityBound, 1 ...MinEntropyBound, 2 ...RelaxedMinEntropyBound
+    shockcapture: int = 0           # 0 NoShockCapture, 1 HennemannShockCapture(shock_a, shock_c)
+    shock_a: float = 0.5
+    shock_c: float = 1.8
+    VDM_inv: Optional[np.ndarray] = None
 
 
 def flat_problem(solver: Solver, state_param: StateParam, upper_bound_mode: int = 0) -> FlatProblem:
@@ -203,6 +209,7 @@ def flat_problem(solver: Solver, state_param: StateParam, upper_bound_mode: int 
     Kx, Ky = (sz.K, 1) if Nd == 1 else (int(p.K[0]), int(p.K[1]))
     Ival = np.asarray(bc.Ival, dtype=np.float64).reshape(-1, sz.Nc) if len(bc.mapI) else np.zeros((0, sz.Nc))
     tp = p.timestepping_param
+    lc = limiter_codes(p.rhs_limiter)
     return FlatProblem(
         Nd=Nd, Nc=sz.Nc, N1D=sz.N1D, Nq=sz.Nq, Nfp=sz.Nfp, Nh=sz.Nh, K=sz.K, Kx=Kx, Ky=Ky,
         gamma=float(p.equation.gamma), zeta=float(p.limiting_param.zeta), eta=float(p.limiting_param.eta),
@@ -224,4 +231,6 @@ def flat_problem(solver: Solver, state_param: StateParam, upper_bound_mode: int 
         mapO=np.ascontiguousarray(np.asarray(bc.mapO, dtype=np.int64)),
         upper_bound_mode=int(upper_bound_mode),
         basis=1 if isinstance(p.approximation_basis, GaussCollocation) else 0,
-        entropyproj_limiter=1 if isinstance(p.entropyproj_limiter, NodewiseScaledExtrapolation) else 0)
+        entropyproj_limiter=1 if isinstance(p.entropyproj_limiter, NodewiseScaledExtrapolation) else 0,
+        rhs_limiter=lc[0], bound=lc[1], shockcapture=lc[2], shock_a=lc[3], shock_c=lc[4],
+        VDM_inv=_colmajor(ops.VDM_inv))

@@ -105,6 +105,30 @@ def lagrange_diff_matrix(x: np.ndarray) -> np.ndarray:
     return D
 
 
+def legendre_orthonormal(x: np.ndarray, n: int) -> np.ndarray:
+    """P_n(x) scaled to unit L2 norm on [-1, 1] (NodesAndModes ``jacobiP(x, 0, 0, n)``)."""
+    p0 = np.ones_like(x)
+    if n == 0:
+        return p0 * np.sqrt(0.5)
+    p1 = x.copy()
+    for k in range(2, n + 1):
+        p0, p1 = p1, ((2 * k - 1) * x * p1 - (k - 1) * p0) / k
+    return p1 * np.sqrt((2 * n + 1) / 2.0)
+
+
+def vandermonde(rd) -> np.ndarray:
+    """``VDM`` of the element (init.jl:116,141): the orthonormal Legendre basis at the volume
+    nodes; in 2D mode (i, j) = P_i(r) P_j(s) in column j + (N+1) i (NodesAndModes' order)."""
+    N = rd.N
+    if rd.Nd == 1:
+        return np.stack([legendre_orthonormal(rd.r, i) for i in range(N + 1)], axis=1)
+    cols = []
+    for i in range(N + 1):
+        for j in range(N + 1):
+            cols.append(legendre_orthonormal(rd.r, i) * legendre_orthonormal(rd.s, j))
+    return np.stack(cols, axis=1)
+
+
 def droptol(A: np.ndarray, tol: float) -> np.ndarray:
     """SparseArrays ``droptol!``: keep entries with ``abs(x) > tol``."""
     B = np.array(A, dtype=np.float64, copy=True)
@@ -257,6 +281,7 @@ class Operators:
     wq: np.ndarray
     q2fq: List[List[int]]
     fq2q: np.ndarray
+    VDM_inv: Optional[np.ndarray] = None   # (Np, Np): inv(VDM) (Solver.jl:200, init.jl:224)
 
 
 @dataclass
@@ -339,7 +364,7 @@ def build_operators(rd: RefElem, ZEROTOL: float, quad: str = "lobatto") -> Tuple
     q2fq = [[int(f) + 1 for f in np.flatnonzero(Vf_low[:, i] == 1.0)] for i in range(Nq)]
     ops = Operators(Srsh_db=Srsh_db, Srs0=Srs0, Srsh_nnz=Srsh_nnz, Srs0_nnz=Srs0_nnz, Brs=Brs,
                     Vh=Vh, MinvVhT=MinvVhT, MinvVfT=MinvVfT, Vq=Vq, Vf=Vf, Vf_low=Vf_low, Pq=Pq,
-                    wq=rd.wq.copy(), q2fq=q2fq, fq2q=fq2q)
+                    wq=rd.wq.copy(), q2fq=q2fq, fq2q=fq2q, VDM_inv=np.linalg.inv(vandermonde(rd)))
     return ops, sizes
 
 

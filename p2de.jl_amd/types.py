@@ -101,6 +101,31 @@ class PositivityBound:
 
 
 @dataclass(frozen=True)
+class PositivityAndMinEntropyBound:
+    pass
+
+
+@dataclass(frozen=True)
+class PositivityAndRelaxedMinEntropyBound:
+    pass
+
+
+@dataclass(frozen=True)
+class PositivityAndCellEntropyBound:
+    pass
+
+
+@dataclass(frozen=True)
+class PositivityAndRelaxedCellEntropyBound:
+    beta: float = 0.5
+
+
+@dataclass(frozen=True)
+class TVDBound:
+    pass
+
+
+@dataclass(frozen=True)
 class NoShockCapture:
     pass
 
@@ -184,6 +209,22 @@ class Param:
         return int(self.K[0]) * int(self.K[1])
 
 
+BOUNDS = (PositivityBound, PositivityAndMinEntropyBound, PositivityAndRelaxedMinEntropyBound)
+
+
+def limiter_codes(lim) -> tuple:
+    """(rhs_limiter, bound, shockcapture, a, c) as the C-ABI enums take them."""
+    rl = 1 if isinstance(lim, ZhangShuLimiter) else 0
+    bd = 0
+    if isinstance(lim, SubcellLimiter):
+        bd = {PositivityBound: 0, PositivityAndMinEntropyBound: 1,
+              PositivityAndRelaxedMinEntropyBound: 2}.get(type(lim.bound), -1)
+    sc = lim.shockcapture
+    if isinstance(sc, HennemannShockCapture):
+        return rl, bd, 1, float(sc.a), float(sc.c)
+    return rl, bd, 0, 0.5, 1.8
+
+
 def check_supported(param: Param) -> None:
     """Reject option combinations outside the implemented hot path.
 
@@ -192,7 +233,10 @@ def check_supported(param: Param) -> None:
     ``LobattoCollocation`` + ``NoEntropyProjectionLimiter`` (the combination
     test/test_smoke.jl:107-108 exercises) or ``GaussCollocation`` with
     ``NodewiseScaledExtrapolation`` (examples/2D/kelvin-helmholtz.jl:50-55) or
-    ``NoEntropyProjectionLimiter``.
+    ``NoEntropyProjectionLimiter``; the limiter may also be ``ZhangShuLimiter``, the bound
+    ``PositivityAndMinEntropyBound`` or ``PositivityAndRelaxedMinEntropyBound``, and either
+    limiter may use ``HennemannShockCapture`` (the first five limiter set-ups of
+    test/test_smoke.jl:44-52).
     """
     rhs = param.rhs
     if not isinstance(rhs, LimitedDG):
@@ -214,11 +258,12 @@ def check_supported(param: Param) -> None:
         raise UnsupportedOption(f"entropyproj_limiter {epl!r}: NoEntropyProjectionLimiter or "
                                 "NodewiseScaledExtrapolation")
     lim = param.rhs_limiter
-    if not isinstance(lim, SubcellLimiter):
-        raise UnsupportedOption(f"rhs_limiter {lim!r}: only SubcellLimiter is implemented")
-    if not isinstance(lim.bound, PositivityBound):
-        raise UnsupportedOption("SubcellLimiter bound must be PositivityBound")
-    if not isinstance(lim.shockcapture, NoShockCapture):
-        raise UnsupportedOption("SubcellLimiter shockcapture must be NoShockCapture")
+    if not isinstance(lim, (SubcellLimiter, ZhangShuLimiter)):
+        raise UnsupportedOption(f"rhs_limiter {lim!r}: SubcellLimiter or ZhangShuLimiter")
+    if isinstance(lim, SubcellLimiter) and not isinstance(lim.bound, BOUNDS):
+        raise UnsupportedOption(f"SubcellLimiter bound {lim.bound!r}: PositivityBound, PositivityAndMinEntropyBound "
+                                "or PositivityAndRelaxedMinEntropyBound")
+    if not isinstance(lim.shockcapture, (NoShockCapture, HennemannShockCapture)):
+        raise UnsupportedOption(f"shockcapture {lim.shockcapture!r}: NoShockCapture or HennemannShockCapture")
     if not 1 <= param.N <= 4:
         raise UnsupportedOption(f"polynomial degree N={param.N}: 1..4 are implemented")

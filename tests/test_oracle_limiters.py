@@ -1,0 +1,224 @@
+"""The oracle's limiter options beyond SubcellLimiter(PositivityBound) (CPU): ZhangShuLimiter,
+HennemannShockCapture, PositivityAndMinEntropyBound and PositivityAndRelaxedMinEntropyBound
+(zhangshu.jl:4-56, shock_capture.jl:10-132, subcell.jl:4-75,927-956, limiter_utils.jl:4-50).
+
+The reference's smoke test runs these set-ups but checks no values (test/test_smoke.jl:44-52),
+so they are pinned by what the definitions fix:
+
+* VDM_inv inverts the orthonormal Legendre Vandermonde, which is orthonormal under Gauss
+  quadrature and equals the closed-form Legendre polynomials at the nodes;
+* the smoothness indicator vanishes for rho*p of degree <= N-2 per direction and is 1 for a
+  pure top mode; the blending factor at indicator 0 is 1 - alphaE0 (closed form);
+* Zhang-Shu: the limited nodal update U + dt*rhsU satisfies rho >= zeta*rho_L and
+  rhoe >= zeta*rhoe_L at every node, l is the largest such parameter, and rhsU is the convex
+  blend (1-l) rhsL + l rhsH;
+* min-entropy bounds: the lower bound is the minimum of s = rhoe*rho^-gamma over the node and
+  its low-order neighbours, found here by node coordinates (independent of mapP and fq2q);
+  the limited update keeps s >= lower bound - POSTOL at every node (quasi-concavity of s);
+  the relaxed bound blends it with the initial minimum by the smoothness factor;
+* all set-ups conserve and stay positive through the reference's smoke problem.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import PKG
+from oracle.oracle import Oracle
+
+C = PKG.configs
+ops_mod = PKG.operators
+
+
+def _setup(name="smoke_vortex", **kw):
+    s, st, sp = C.build(name, **kw)
+    return s, st.preallocation.Uq.copy(), PKG.flat_problem(s, sp)
+
+
+@pytest.mark.parametrize("Nd,N", [(1, 3), (2, 2), (2, 3), (2, 4)])
+def test_vandermonde_is_orthonormal_legendre(Nd, N):
+    rd = ops_mod.gauss_ref_elem(N, Nd)
+    V = ops_mod.vandermonde(rd)
+    assert np.allclose(V.T @ np.diag(rd.wq) @ V, np.eye(V.shape[0]), atol=1e-13)
+    ops, _ = ops_mod.build_operators(rd, 5e-16, quad="gauss")
+    assert np.allclose(ops.VDM_inv @ V, np.eye(V.shape[0]), atol=1e-12)
+    x = rd.r
+    closed = [np.full_like(x, math.sqrt(0.5)), math.sqrt(1.5) * x, math.sqrt(2.5) * 0.5 * (3 * x * x - 1)]
+    if Nd == 1:
+        for n in range(3):
+            assert np.allclose(V[:, n], closed[n], atol=1e-14)
+    else:
+        for i in range(3):   # column j + (N+1) i is P_i(r) P_j(s); j = 0
+            assert np.allclose(V[:, (N + 1) * i], closed[i] * math.sqrt(0.5), atol=1e-14)
+    rl = ops_mod.lobatto_ref_elem(N, Nd)
+    opl, _ = ops_mod.build_operators(rl, 5e-16)
+    assert np.allclose(opl.VDM_inv @ ops_mod.vandermonde(rl), np.eye(V.shape[0]), atol=1e-12)
+
+
+def _rho_p_state(s, fp, f):
+    """rho = 1, zero velocity, p = f(x, y): rho*p = f at the nodes."""
+    x, y = s.md.xq, s.md.yq
+    U = np.zeros((fp.K, fp.Nq, 4))
+    U[..., 0] = 1.0
+    U[..., 3] = (f(x, y) / 0.4).T
+    return U
+
+
+def test_smoothness_indicator_and_blending_closed_forms():
+    s, U0, fp = _setup(N=3, limiter="subcell_shock")
+    orc = Oracle(fp)
+    # degree <= N-2 = 1 per direction: the top two shells carry no energy
+    U = _rho_p_state(s, fp, lambda x, y: 2.0 + 0.01 * x + 0.02 * y + 0.001 * x * y)
+    orc.rhs(U, 0.0, 1e-3, 2)
+    ind = orc.field("smooth_indicator")
+    assert np.max(ind) < 1e-24
+    alphaE0 = 1e-4
+    blend = orc.field("blending_factor").reshape(3, fp.K)[1]
+    assert np.allclose(blend, 1.0 - alphaE0, rtol=0, atol=1e-12)
+    # a pure top mode P_3(r) of each element (its modal energy sits in the N shell)
+    rd = s.rd
+    top = ops_mod.legendre_orthonormal(rd.r, 3)
+    U = _rho_p_state(s, fp, lambda x, y: np.zeros_like(x))
+    U[..., 3] = (1.0 + 0.0 * top[None, :]) / 0.4
+    U[..., 0] = 1.0 + 0.5 * top[None, :]            # rho*p = rho = 1 + 0.5 P_3(r)
+    orc.rhs(U, 0.0, 1e-3, 2)
+    ind = orc.field("smooth_indicator")
+    # with P_0 = sqrt(1/2): 1 = 2 P_0(r) P_0(s) and 0.5 P_3(r) = (0.5 / sqrt(1/2)) P_3(r) P_0(s)
+    c00, c30 = 2.0, 0.5 / math.sqrt(0.5)
+    assert np.allclose(ind, c30 ** 2 / (c30 ** 2 + c00 ** 2), rtol=1e-10)
+
+
+def _limited_update(orc, U, t, dt, nstage):
+    orc.rhs(U, t, dt, nstage)
+    return U + dt * orc.field("rhsU").reshape(U.shape)
+
+
+def _rhoe(U):
+    return U[..., -1] - 0.5 * np.sum(U[..., 1:-1] ** 2, axis=-1) / U[..., 0]
+
+
+@pytest.mark.parametrize("N", [2, 3, 4])
+def test_zhang_shu_bounds_maximality_and_blend(N):
+    s, U, fp = _setup(N=N, limiter="zhangshu")
+    U[7, N + 2, 1:3] = 0.0
+    U[7, N + 2, 3] = 1e-4 / 0.4                     # a cold node
+    orc = Oracle(fp)
+    dt = 1e-3
+    Un = _limited_update(orc, U, 0.0, dt, 2)
+    rL = orc.field("rhsL").reshape(U.shape)
+    rH = orc.field("rhsH").reshape(U.shape)
+    L = orc.field("L_zhangshu").reshape(3, fp.K)[1]
+    assert np.all((L >= 0) & (L <= 1)) and L.min() < 1
+    uL = U + dt * rL
+    zeta = fp.zeta
+    tol = 1e-12
+    assert np.all(Un[..., 0] >= zeta * uL[..., 0] * (1 - tol))
+    assert np.all(_rhoe(Un) >= zeta * _rhoe(uL) * (1 - tol) - tol * np.abs(_rhoe(uL)))
+    r = orc.field("rhsU").reshape(U.shape)
+    blend = (1 - L)[:, None, None] * rL + L[:, None, None] * rH
+    assert np.allclose(r, blend, rtol=0, atol=1e-13 * np.abs(rH).max())
+    # maximality: a slightly larger parameter breaks the bound on a limited element
+    k = int(np.argmin(L))
+    l2 = L[k] + 1e-6
+    Ut = uL[k] + l2 * dt * (rH[k] - rL[k])
+    viol = (Ut[:, 0] < zeta * uL[k, :, 0]) | (_rhoe(Ut) < zeta * _rhoe(uL[k]))
+    assert viol.any()
+
+
+def _coordinate_stencil_min(s, fp, svals):
+    """min of s over each node and its four (two in 1D) low-order neighbours, the neighbours
+    located by coordinates: the adjacent node of the same line, or at an element boundary the
+    neighbouring element's node at the same point (periodic box)."""
+    x, y = s.md.xq.T, s.md.yq.T                      # (K, Nq)
+    N1D = fp.N1D
+    Lx, Ly = 10.0, 10.0
+    out = np.empty_like(svals)
+    pts = {}
+    for k in range(fp.K):
+        for q in range(fp.Nq):
+            pts.setdefault((round(x[k, q] % Lx, 9) % Lx, round(y[k, q] % Ly, 9) % Ly), []).append((k, q))
+    for k in range(fp.K):
+        for q in range(fp.Nq):
+            si, sj = q % N1D, q // N1D
+            vals = [svals[k, q]]
+            for d, (a, b) in enumerate(((si, sj), (sj, si))):
+                for side in (-1, 1):
+                    na = a + side
+                    if 0 <= na < N1D:
+                        qq = (na + N1D * sj) if d == 0 else (si + N1D * na)
+                        vals.append(svals[k, qq])
+                    else:
+                        key = (round(x[k, q] % Lx, 9) % Lx, round(y[k, q] % Ly, 9) % Ly)
+                        others = [(kk, qq) for kk, qq in pts[key] if kk != k]
+                        # the neighbour across this side shares the point and lies on that side
+                        dx = [(x[kk].mean() - x[k].mean() + Lx / 2) % Lx - Lx / 2 for kk, qq in others]
+                        dy = [(y[kk].mean() - y[k].mean() + Ly / 2) % Ly - Ly / 2 for kk, qq in others]
+                        pick = [o for o, ddx, ddy in zip(others, dx, dy)
+                                if (d == 0 and abs(ddy) < 1e-9 and side * ddx > 0) or
+                                (d == 1 and abs(ddx) < 1e-9 and side * ddy > 0)]
+                        assert len(pick) == 1
+                        vals.append(svals[pick[0]])
+            out[k, q] = min(vals)
+    return out
+
+
+@pytest.mark.parametrize("N", [2, 3])
+def test_min_entropy_lower_bound_and_bound_property(N):
+    s, U0, fp = _setup(N=N, limiter="minentropy")
+    orc = Oracle(fp)
+    orc.rhs(U0, 0.0, 1e-2, 1)                        # t = t0, stage 1: the initial minimum
+    s0 = orc.field("s_modified_min")[0]
+    rho = U0[..., 0]
+    assert s0 == pytest.approx(np.min(_rhoe(U0) * rho ** -1.4), rel=1e-14)
+    U = U0.copy()
+    U[7, N + 2, 3] *= 1.5                            # a hot node: a local entropy extremum
+    dt = 5e-3
+    Un = _limited_update(orc, U, 0.01, dt, 2)
+    svals = _rhoe(U) * U[..., 0] ** -1.4
+    lb = orc.field("lbound_s_modified").reshape(fp.K, fp.Nq)
+    assert np.allclose(lb, _coordinate_stencil_min(s, fp, svals), rtol=1e-15, atol=0)
+    s_new = _rhoe(Un) * Un[..., 0] ** -1.4
+    assert np.all(s_new >= lb - fp.POSTOL - 1e-13 * np.abs(lb))
+    assert orc.field("L_local").min() < 1
+
+
+def test_relaxed_min_entropy_blends_with_the_initial_minimum():
+    s, U0, fp = _setup(N=3, limiter="relaxed_minentropy")
+    orc = Oracle(fp)
+    orc.rhs(U0, 0.0, 1e-2, 1)
+    U = U0.copy()
+    U[7, 5, 3] *= 1.5
+    orc.rhs(U, 0.01, 5e-3, 2)
+    sf = orc.field("smooth_factor").reshape(3, fp.K)[1]
+    ind = orc.field("smooth_indicator")
+    s0 = math.log10(3.0 ** -4)
+    sk = np.log10(ind)
+    expect = np.where(sk < s0 - 1, 0.0, np.where(sk > s0 + 1, 1.0, 0.5 - 0.5 * np.sin(np.pi * (sk - s0) / 2)))
+    assert np.allclose(sf, expect, rtol=0, atol=1e-14)
+    assert (sf > 0).any() and (sf < 1).any()
+    svals = _rhoe(U) * U[..., 0] ** -1.4
+    local = _coordinate_stencil_min(s, fp, svals)
+    smin = orc.field("s_modified_min")[0]
+    lb = orc.field("lbound_s_modified").reshape(fp.K, fp.Nq)
+    assert np.allclose(lb, sf[:, None] * local + (1 - sf[:, None]) * smin, rtol=1e-15, atol=1e-15)
+
+
+@pytest.mark.parametrize("limiter", ["zhangshu", "zhangshu_shock", "subcell_shock", "minentropy", "relaxed_minentropy"])
+@pytest.mark.parametrize("N", [1, 2, 3, 4])
+def test_smoke_setups_conserve_and_stay_positive(limiter, N):
+    """The reference's smoke problem (test/test_smoke.jl:44-66) to T = 2e-2 with each set-up."""
+    s, U, fp = _setup(N=N, limiter=limiter)
+    wJ = fp.wq[None, :] * fp.Jq.reshape(fp.K, fp.Nq)
+    m0 = np.einsum("kq,kqc->c", wJ, U)
+    a0 = np.einsum("kq,kqc->c", wJ, np.abs(U))
+    orc = Oracle(fp)
+    t = 0.0
+    while t < fp.T - 1e-14:
+        t, _ = orc.step(U, t)
+    m1 = np.einsum("kq,kqc->c", wJ, U)
+    assert np.max(np.abs(m1 - m0) / a0) < 1e-13
+    assert orc.count_nonpositive(U)[0] == 0
+    x, y = s.md.xq, s.md.yq
+    ex = np.transpose(C.primitive_to_conservative(1.4, C.smoke_exact_prim(x, y, t)), (2, 1, 0))
+    err = np.sum(wJ * np.abs(U[..., 0] - ex[..., 0])) / np.sum(wJ * np.abs(ex[..., 0]))
+    assert err < 5e-3
