@@ -10,7 +10,8 @@ FP64 drop-in (``p2de_create``) and the mixed path are timed on the same steps, w
 ratio and the density L1/Linf difference of the two results (the batched instances are also
 run one by one through the FP64 drop-in for that comparison).  The Gauss basis with the
 entropy-projection limiter is timed through the FP64 drop-in on the four-state Riemann problem
-(256x256, N=3) and on the reference's Kelvin-Helmholtz example (80x80, N=3).
+(256x256, N=3) and on the reference's Kelvin-Helmholtz example (80x80, N=3), and the other limiter
+set-ups (Zhang-Shu, shock capture, min-entropy bounds) on the four-state Riemann problem.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-compare]
     python bench.py --profile WORKLOAD [--precision mixed|fp64] [--basis B] [--variant V] [--steps K]
@@ -315,6 +316,19 @@ def main():
                            "update_kernel_ms_per_step": ms[1] / steps_t,
                            "positivity_violations": int(rep.violations), "min_p": rep.min_p}
         cmp["gauss_fp64"] = gauss
+        # the other limiter set-ups of test/test_smoke.jl:44-52, FP64 drop-in, four-state Riemann 256x256 N=3
+        lims = {}
+        for lim in ("zhangshu", "zhangshu_shock", "subcell_shock", "minentropy", "relaxed_minentropy"):
+            solver, state, sp = pkg.configs.build("riemann4_2d", limiter=lim)
+            fp = pkg.flat_problem(solver, sp)
+            U = state.preallocation.Uq
+            n = fp.K * fp.Nq
+            dev, w64 = time_fp64(DeviceSolver, fp, U, args.steps, args.warmup)
+            rep = dev.check_positivity()
+            del dev
+            lims[lim] = {"fp64_dropin": n * 3 * args.steps / w64, "ms_per_step": w64 / args.steps * 1e3,
+                         "positivity_violations": int(rep.violations)}
+        cmp["limiters_fp64_riemann4"] = lims
         out["config"]["comparison"] = cmp
     if not args.no_cpu:
         try:

@@ -38,8 +38,10 @@ enum { P2DE_CHANDRASHEKAR_FLUX = 0 };             /* ChandrashekarFlux */
 enum { P2DE_BASIS_LOBATTO = 0, P2DE_BASIS_GAUSS = 1 };
 enum { P2DE_NO_ENTROPY_PROJ_LIMITER = 0, P2DE_NODEWISE_SCALED_EXTRAPOLATION = 1 };
 enum { P2DE_SUBCELL_LIMITER = 0, P2DE_ZHANG_SHU_LIMITER = 1 };
-enum { P2DE_POSITIVITY_BOUND = 0 };
-enum { P2DE_NO_SHOCK_CAPTURE = 0 };
+enum { P2DE_POSITIVITY_BOUND = 0,                          /* PositivityBound */
+       P2DE_POSITIVITY_AND_MIN_ENTROPY_BOUND = 1,          /* PositivityAndMinEntropyBound */
+       P2DE_POSITIVITY_AND_RELAXED_MIN_ENTROPY_BOUND = 2 };/* PositivityAndRelaxedMinEntropyBound */
+enum { P2DE_NO_SHOCK_CAPTURE = 0, P2DE_HENNEMANN_SHOCK_CAPTURE = 1 };
 /* FP64: the drop-in, matching the reference's FP64 arithmetic.  MIXED (2D only): FP32 flux
  * differencing, surface and low-order fluxes and element assembly; FP64 node pressures,
  * subcell flux potentials, limiter bounds and coefficients, RK stages and state.  A context
@@ -64,11 +66,12 @@ typedef struct {
     int basis;                    /* P2DE_BASIS_LOBATTO or P2DE_BASIS_GAUSS */
     int entropyproj_limiter;      /* P2DE_NO_ENTROPY_PROJ_LIMITER; with Gauss also
                                      P2DE_NODEWISE_SCALED_EXTRAPOLATION */
-    int rhs_limiter;              /* P2DE_SUBCELL_LIMITER */
-    int bound;                    /* P2DE_POSITIVITY_BOUND */
-    int shockcapture;             /* P2DE_NO_SHOCK_CAPTURE */
+    int rhs_limiter;              /* P2DE_SUBCELL_LIMITER or P2DE_ZHANG_SHU_LIMITER */
+    int bound;                    /* P2DE_POSITIVITY*_BOUND (SubcellLimiter; Zhang-Shu: POSITIVITY) */
+    int shockcapture;             /* P2DE_NO_SHOCK_CAPTURE or P2DE_HENNEMANN_SHOCK_CAPTURE */
     int upper_bound_mode;         /* P2DE_UPPER_BOUND_* */
     int precision;                /* P2DE_PRECISION_* */
+    double shock_a, shock_c;      /* HennemannShockCapture(a, c) (Solver.jl:67-74) */
 } p2de_config;
 
 /* Operators{DIM} (Solver.jl:192-212), sizes from SizeData (Solver.jl:173-183). */
@@ -88,6 +91,8 @@ typedef struct {
     const double *wq;        /* Nq */
     const int64_t *fq2q;     /* Nfp, 1-based */
     const int64_t *q2fq;     /* Nq x 2 column-major, 1-based, 0 = absent */
+    const double *VDM_inv;   /* Nq x Nq (Operators.VDM_inv); read by HennemannShockCapture and the
+                                relaxed bound only, may be NULL otherwise */
 } p2de_operators;
 
 /* GeomData (Solver.jl:185-190). */
@@ -129,7 +134,8 @@ enum {
     P2DE_FIELD_RHSU = 0,     /* (Nq, K) x Nc */
     P2DE_FIELD_RHSH = 1,     /* (Nq, K) x Nc */
     P2DE_FIELD_RHSL = 2,     /* (Nq, K) x Nc */
-    P2DE_FIELD_L_LOCAL = 3,  /* (Nq+N1D, Nd, K) for the stage of the last call */
+    P2DE_FIELD_L_LOCAL = 3,  /* (Nq+N1D, Nd, K) for the stage of the last call; Zhang-Shu:
+                                the first K values are L[:, nstage] */
     P2DE_FIELD_THETA = 4     /* (Nfp, K) theta_local[:, :, nstage] (NodewiseScaledExtrapolation) */
 };
 

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstddef>
 #include <cstring>
 #include <fstream>
 #include <map>
@@ -47,6 +48,8 @@ struct p2de_ctx {
     double* d_theta = nullptr;             // entropy-projection parameters per face node (Gauss)
     int basis = 0;                         // P2DE_BASIS_*
     bool ep_limit = false;                 // NodewiseScaledExtrapolation
+    int optA = OPT_SUBCELL;                // element-kernel form for the limiter options
+    double *d_blend = nullptr, *d_lphi = nullptr, *d_spart = nullptr;   // limiter option buffers
     double* d_dtp = nullptr;               // per-workgroup CFL minima of phase A (stage 1)
     bool ut_valid = false;                 // d_ut matches d_U[0]
     double *d_dbg_rhsH = nullptr, *d_dbg_rhsL = nullptr, *d_dbg_L = nullptr;
@@ -117,7 +120,7 @@ void free_ctx(p2de_ctx* c) {
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
     void* dptrs[] = {c->d_tab, c->d_geo, c->d_nb_node, c->d_nb_face, c->d_bc, c->d_Ival, c->d_U[0], c->d_U[1],
                      c->d_rhsUA, c->d_lA, c->d_dfA, c->d_rhsU, c->d_ut, c->d_theta, c->d_dtp, c->d_dbg_rhsH,
-                     c->d_dbg_rhsL, c->d_dbg_L,
+                     c->d_dbg_rhsL, c->d_dbg_L, c->d_blend, c->d_lphi, c->d_spart,
                      c->d_sc, c->d_keys};
     for (void* p : dptrs)
         if (p) (void)hipFree(p);
@@ -226,6 +229,12 @@ int p2de_create(const p2de_config* cfg, const p2de_operators* ops, const p2de_ge
             if (!p) return fail(c, P2DE_E_INVALID, "NULL operator, geometry or mapP array"), bail(P2DE_E_INVALID);
         c->basis = cfg->basis;
         c->ep_limit = cfg->entropyproj_limiter == P2DE_NODEWISE_SCALED_EXTRAPOLATION;
+        const bool shock = cfg->shockcapture == P2DE_HENNEMANN_SHOCK_CAPTURE;
+        if ((shock || cfg->bound == P2DE_POSITIVITY_AND_RELAXED_MIN_ENTROPY_BOUND) && !ops->VDM_inv)
+            return fail(c, P2DE_E_INVALID, "the smoothness indicator needs VDM_inv"), bail(P2DE_E_INVALID);
+        c->optA = cfg->rhs_limiter == P2DE_ZHANG_SHU_LIMITER                        ? OPT_ZHANG_SHU
+                  : (shock || cfg->bound != P2DE_POSITIVITY_BOUND) ? OPT_SUBCELL_GENERAL
+                                                                    : OPT_SUBCELL;
         if ((st = setup_tables(c->tab, c->Nd, c->N1D, cfg->basis, ops, c->err))) return bail(st);
         MeshArrays mesh;
         if ((st = setup_mesh(c->Nd, c->N1D, c->K, c->tab, geom, bc, mesh, c->err))) return bail(st);
@@ -245,7 +254,8 @@ int p2de_create(const p2de_config* cfg, const p2de_operators* ops, const p2de_ge
             (st = dalloc(c, &c->d_U[0], nU)) || (st = dalloc(c, &c->d_U[1], nU)) || (st = dalloc(c, &c->d_rhsUA, nU)) ||
             (st = dalloc(c, &c->d_lA, (size_t)K * Nfp)) || (st = dalloc(c, &c->d_dfA, (size_t)K * Nfp * Nc)) ||
             (st = dalloc(c, &c->d_rhsU, nU)) || (st = dalloc(c, &c->d_ut, (size_t)K * Nfp * Nc)) ||
-            (st = dalloc(c, &c->d_theta, (size_t)K * Nfp)) ||
+            (st = dalloc(c, &c->d_theta, (size_t)K * Nfp)) || (st = dalloc(c, &c->d_blend, (size_t)K)) ||
+            (st = dalloc(c, &c->d_lphi, (size_t)K * Nq)) || (st = dalloc(c, &c->d_spart, ((size_t)K * Nq + 255) / 256)) ||
             (st = dalloc(c, &c->d_dtp, (size_t)K)) ||
             (st = dalloc(c, &c->d_sc, 1)) || (st = dalloc(c, &c->d_keys, 3)))
             return bail(st);
@@ -351,6 +361,10 @@ PhaseAArgs argsA(p2de_ctx* c, const double* U, int stage1) {
     a.lA = c->d_lA;
     a.dfA = c->d_dfA;
     a.dtp = c->d_dtp;
+    a.bound = c->cfg.bound;
+    a.POSTOL = c->cfg.POSTOL;
+    a.blend = c->cfg.shockcapture == P2DE_HENNEMANN_SHOCK_CAPTURE ? c->d_blend : nullptr;
+    a.lphi = c->cfg.bound != P2DE_POSITIVITY_BOUND ? c->d_lphi : nullptr;
     if (c->debug) {
         a.dbg_rhsH = c->d_dbg_rhsH;
         a.dbg_rhsL = c->d_dbg_rhsL;
@@ -375,7 +389,7 @@ PhaseBArgs argsB(p2de_ctx* c, int mode, const double* W, const double* Uin, doub
     b.Uin = Uin;
     b.Uout = Uout;
     b.rhsU = c->d_rhsU;
-    b.dbg_L = c->debug ? c->d_dbg_L : nullptr;
+    b.dbg_L = (c->debug && c->optA != OPT_ZHANG_SHU) ? c->d_dbg_L : nullptr;   // Zhang-Shu: L[k] stays
     b.ut = c->basis == P2DE_BASIS_GAUSS ? nullptr : c->d_ut;
     b.finalize = finalize;
     return b;
@@ -412,10 +426,45 @@ int project_faces(p2de_ctx* c, const double* U) {
     return P2DE_OK;
 }
 
-int stage_rhs(p2de_ctx* c, const double* U, int stage1) {
+// Limiter options: the entropy minimum of the state at t0 (first stage), then the smoothness
+// indicator, blending factor and entropy lower bounds of the stage input.  t is read from the
+// device scalars when tp is set (steps), else it is t_value (p2de_rhs).
+int limiter_prep(p2de_ctx* c, const double* U, int nstage, const double* tp, double t_value) {
+    const p2de_config& g = c->cfg;
+    const bool shock = g.shockcapture == P2DE_HENNEMANN_SHOCK_CAPTURE;
+    if (!shock && g.bound == P2DE_POSITIVITY_BOUND) return P2DE_OK;
+    if (g.bound != P2DE_POSITIVITY_BOUND && nstage == 1)
+        LAUNCHCHK(c, timed(c, 2, [&] {
+                      return launch_s_min(U, (int64_t)c->K * c->Nq, c->Nd, g.gamma, tp, t_value, g.t0, c->d_spart, c->d_sc,
+                                          c->stream);
+                  }));
+    PrepArgs p{};
+    p.U = U;
+    p.T = c->d_tab;
+    p.nb_node = c->d_nb_node;
+    p.sc = c->d_sc;
+    p.K = c->K;
+    p.N = c->N1D - 1;
+    p.stage = nstage;
+    p.shock = g.shockcapture;
+    p.bound = g.bound;
+    p.gamma = g.gamma;
+    p.shock_a = g.shock_a;
+    p.shock_c = g.shock_c;
+    p.blend = c->d_blend;
+    p.lphi = c->d_lphi;
+    LAUNCHCHK(c, timed(c, 0, [&] { return launch_limiter_prep(c->Nd, c->N1D, p, c->stream); }));
+    return P2DE_OK;
+}
+
+int stage_rhs(p2de_ctx* c, const double* U, int nstage, const double* tp, double t_value) {
+    const int stage1 = nstage == 1;
     int st = project_faces(c, U);
     if (st) return st;
-    LAUNCHCHK(c, timed(c, 0, [&] { return launch_phaseA(c->Nd, c->N1D, c->basis, c->variantA, argsA(c, U, stage1), c->stream); }));
+    if ((st = limiter_prep(c, U, nstage, tp, t_value))) return st;
+    LAUNCHCHK(c, timed(c, 0, [&] {
+                  return launch_phaseA(c->Nd, c->N1D, c->basis, c->variantA, c->optA, argsA(c, U, stage1), c->stream);
+              }));
     if (stage1)
         LAUNCHCHK(c, timed(c, 2, [&] {
                       return launch_dt_reduce(c->d_sc, c->d_dtp, phaseA_blocks(c->Nd, c->N1D, c->basis, c->variantA, c->K),
@@ -432,14 +481,15 @@ int enqueue_step(p2de_ctx* c) {
     int st = ensure_face_states(c);
     if (st) return st;
     LAUNCHCHK(c, timed(c, 2, [&] { return launch_step_init(c->d_sc, g.CFL, g.dt0, g.T, s); }));
+    const double* tp = (const double*)((const char*)c->d_sc + offsetof(StepScalars, t));
     // stage 1: U1 = W + dt*rhsU (SSPRK33.jl:32-33)
-    if ((st = stage_rhs(c, A, 1))) return st;
+    if ((st = stage_rhs(c, A, 1, tp, 0.0))) return st;
     LAUNCHCHK(c, timed(c, 1, [&] { return launch_phaseB(c->Nd, c->N1D, argsB(c, MODE_STAGE1, A, A, B, 0), s); }));
     // stage 2: U2 = 3/4 W + 1/4 (U1 + dt*rhsU) (SSPRK33.jl:34-36)
-    if ((st = stage_rhs(c, B, 0))) return st;
+    if ((st = stage_rhs(c, B, 2, tp, 0.0))) return st;
     LAUNCHCHK(c, timed(c, 1, [&] { return launch_phaseB(c->Nd, c->N1D, argsB(c, MODE_STAGE2, A, B, B, 0), s); }));
     // stage 3: U^{n+1} = 1/3 W + 2/3 (U2 + dt*rhsU) (SSPRK33.jl:37-39), t += dt
-    if ((st = stage_rhs(c, B, 0))) return st;
+    if ((st = stage_rhs(c, B, 3, tp, 0.0))) return st;
     LAUNCHCHK(c, timed(c, 1, [&] { return launch_phaseB(c->Nd, c->N1D, argsB(c, MODE_STAGE3, A, B, A, 1), s); }));
     return P2DE_OK;
 }
@@ -479,7 +529,7 @@ int p2de_rhs(p2de_ctx* ctx, double t, double dt, int nstage, double* dt_out, dou
     LAUNCHCHK(ctx, timed(ctx, 2, [&] { return launch_rhs_init(ctx->d_sc, dt, cap, s); }));
     int st0 = ensure_face_states(ctx);
     if (st0) return st0;
-    if ((st0 = stage_rhs(ctx, ctx->d_U[0], nstage == 1))) return st0;
+    if ((st0 = stage_rhs(ctx, ctx->d_U[0], nstage, nullptr, t))) return st0;
     LAUNCHCHK(ctx, timed(ctx, 1, [&] {
                   return launch_phaseB(ctx->Nd, ctx->N1D, argsB(ctx, MODE_RHS, ctx->d_U[0], ctx->d_U[0], nullptr, 0), s);
               }));

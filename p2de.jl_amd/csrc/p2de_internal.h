@@ -84,6 +84,11 @@ struct ElemTables {
     double Br[MAX_NFP], Bs[MAX_NFP];
     int8_t ld[MAX_NFP];           // direction the LF term is applied to (rhs_utils.jl:78-102)
     int8_t bsign[MAX_NFP];        // -1: left/bottom element-boundary subcell face, +1 right/top
+    // limiter options
+    double vdm_inv[MAX_NQ][MAX_NQ];   // VDM_inv[i][j] (smoothness indicator, shock_capture.jl:55-57)
+    // low-order stencil of each node (low_order_stencil limiter_utils.jl:214-231), in the order
+    // (left, right[, bottom, top]): >= 0 a node of the element, < 0 -(f+1) for the node across face f
+    int8_t stn[MAX_NQ][4];
 };
 
 // Device-side time-stepping scalars.
@@ -95,6 +100,7 @@ struct StepScalars {
     int steps;
     unsigned long long viol;      // positivity violations
     unsigned long long first_bad; // min over (stage<<40 | node) of offending nodes
+    double s_min;                 // minimum modified specific entropy of the state at t0 (min-entropy bounds)
 };
 
 struct PhaseAArgs {
@@ -119,7 +125,16 @@ struct PhaseAArgs {
     double* dbg_rhsL;
     double* dbg_L;                // L_local[:, :, :] of this stage, (Nq+N1D, Nd, K)
     double* dtp;                  // stage 1: per-workgroup minimum of the CFL time step
+    // limiter options (element-kernel variants OPT 1 and 2)
+    int bound;                    // P2DE_POSITIVITY*_BOUND
+    double POSTOL;
+    const double* blend;          // per element blending factor of the shock capture (nullptr: 1)
+    const double* lphi;           // per node lower bound of the modified specific entropy
 };
+
+// element-kernel forms: 0 SubcellLimiter(PositivityBound, NoShockCapture); 1 SubcellLimiter with
+// the min-entropy bounds and/or shock capture; 2 ZhangShuLimiter
+enum { OPT_SUBCELL = 0, OPT_SUBCELL_GENERAL = 1, OPT_ZHANG_SHU = 2 };
 
 enum { MODE_RHS = 0, MODE_STAGE1 = 1, MODE_STAGE2 = 2, MODE_STAGE3 = 3 };
 
@@ -143,6 +158,20 @@ struct PhaseBArgs {
     int finalize;                 // stage 3: advance t
 };
 
+// Per stage before the element kernel, for the limiter options: smoothness indicator, blending
+// factor, smoothness factor and the lower entropy bound of every node.
+struct PrepArgs {
+    const double* U;
+    const ElemTables* T;
+    const int* nb_node;
+    StepScalars* sc;
+    int K, N, stage;
+    int shock, bound;             // P2DE_*_SHOCK_CAPTURE, P2DE_POSITIVITY*_BOUND
+    double gamma, shock_a, shock_c;
+    double* blend;                // [K] (shock capture)
+    double* lphi;                 // [K][Nq] (min-entropy bounds)
+};
+
 // Entropy projection of the face states (Gauss basis): theta per face node and u_tilde.
 struct EPArgs {
     const double* U;
@@ -157,7 +186,11 @@ struct EPArgs {
 
 // launchers (rhs_kernels.hip); return hipError_t as int
 // basis: 0 Lobatto, 1 Gauss (phase-A variants are per basis)
-int launch_phaseA(int Nd, int N1D, int basis, int variant, const PhaseAArgs& a, void* stream);
+int launch_phaseA(int Nd, int N1D, int basis, int variant, int opt, const PhaseAArgs& a, void* stream);
+int launch_limiter_prep(int Nd, int N1D, const PrepArgs& a, void* stream);
+// s_min = min over all nodes of rhoe*rho^-gamma, when t == t0 (t from *tp or t_value) at stage 1
+int launch_s_min(const double* U, int64_t nodes, int Nd, double gamma, const double* tp, double t_value, double t0,
+                 double* part, StepScalars* sc, void* stream);
 int num_variants_A(int Nd, int N1D, int basis);
 int variant_shape_A(int Nd, int N1D, int basis, int variant, int* E, int* NT);
 int launch_entropy_projection(int Nd, int N1D, const EPArgs& a, void* stream);

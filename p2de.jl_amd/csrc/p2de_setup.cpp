@@ -31,9 +31,20 @@ int setup_config(const p2de_config* cfg, int* Nq, int* Nfp, int* Nh, int* K, std
     // option set (Solver.jl:1-121); anything else is not implemented here
     if (cfg->rhs != P2DE_RHS_LIMITED_DG || cfg->low_order_surface_flux != P2DE_LF_ON_NODAL_VAL ||
         cfg->high_order_surface_flux != P2DE_LF_ON_PROJECTED_VAL ||
-        cfg->high_order_volume_flux != P2DE_CHANDRASHEKAR_FLUX || cfg->rhs_limiter != P2DE_SUBCELL_LIMITER ||
-        cfg->bound != P2DE_POSITIVITY_BOUND || cfg->shockcapture != P2DE_NO_SHOCK_CAPTURE)
+        cfg->high_order_volume_flux != P2DE_CHANDRASHEKAR_FLUX)
         return fail_s(err, P2DE_E_UNSUPPORTED, "option set not implemented");
+    if (cfg->rhs_limiter != P2DE_SUBCELL_LIMITER && cfg->rhs_limiter != P2DE_ZHANG_SHU_LIMITER)
+        return fail_s(err, P2DE_E_UNSUPPORTED, "rhs limiter not implemented");
+    if (cfg->bound != P2DE_POSITIVITY_BOUND && cfg->bound != P2DE_POSITIVITY_AND_MIN_ENTROPY_BOUND &&
+        cfg->bound != P2DE_POSITIVITY_AND_RELAXED_MIN_ENTROPY_BOUND)
+        return fail_s(err, P2DE_E_UNSUPPORTED, "limiter bound not implemented (cell-entropy and TVD bounds)");
+    // bound(::ZhangShuLimiter) is PositivityBound (traits.jl:61-63)
+    if (cfg->rhs_limiter == P2DE_ZHANG_SHU_LIMITER && cfg->bound != P2DE_POSITIVITY_BOUND)
+        return fail_s(err, P2DE_E_UNSUPPORTED, "the Zhang-Shu limiter takes no subcell bound");
+    if (cfg->shockcapture != P2DE_NO_SHOCK_CAPTURE && cfg->shockcapture != P2DE_HENNEMANN_SHOCK_CAPTURE)
+        return fail_s(err, P2DE_E_UNSUPPORTED, "shock capture not implemented");
+    if (cfg->shockcapture == P2DE_HENNEMANN_SHOCK_CAPTURE && !(cfg->shock_a > 0.0))
+        return fail_s(err, P2DE_E_INVALID, "HennemannShockCapture needs a > 0");
     if (cfg->basis != P2DE_BASIS_LOBATTO && cfg->basis != P2DE_BASIS_GAUSS)
         return fail_s(err, P2DE_E_UNSUPPORTED, "approximation basis not implemented");
     if (cfg->entropyproj_limiter != P2DE_NO_ENTROPY_PROJ_LIMITER &&
@@ -54,6 +65,11 @@ int setup_config(const p2de_config* cfg, int* Nq, int* Nfp, int* Nh, int* K, std
         return fail_s(err, P2DE_E_UNSUPPORTED, "the mixed-precision path is 2D only");
     if (cfg->precision == P2DE_PRECISION_MIXED && cfg->basis != P2DE_BASIS_LOBATTO)
         return fail_s(err, P2DE_E_UNSUPPORTED, "the mixed-precision path runs the Lobatto basis");
+    if (cfg->precision == P2DE_PRECISION_MIXED &&
+        (cfg->rhs_limiter != P2DE_SUBCELL_LIMITER || cfg->bound != P2DE_POSITIVITY_BOUND ||
+         cfg->shockcapture != P2DE_NO_SHOCK_CAPTURE))
+        return fail_s(err, P2DE_E_UNSUPPORTED,
+                      "the mixed-precision path runs SubcellLimiter(PositivityBound, NoShockCapture)");
     const int N1D = cfg->N + 1;
     *Nq = (int)ipow(N1D, cfg->Nd);
     *Nfp = cfg->Nd == 1 ? 2 : 4 * N1D;
@@ -256,6 +272,31 @@ int setup_tables(ElemTables& T, int Nd, int N1D, int basis, const p2de_operators
     }
     T.ll_start[Nq] = (int16_t)pos;
     T.cl_start[Nq] = (int16_t)cpos;
+    // smoothness indicator operator (Operators.VDM_inv)
+    if (ops->VDM_inv)
+        for (int i = 0; i < Nq; i++)
+            for (int j = 0; j < Nq; j++) T.vdm_inv[i][j] = cm(ops->VDM_inv, Nq, i, j);
+    // low-order stencil (low_order_stencil limiter_utils.jl:214-231): the line neighbours, and at
+    // the element boundary the node across face q2fq[q][direction] (or the node's only face)
+    for (int q = 0; q < Nq; q++) {
+        const int si = Nd == 1 ? q : q % N1D, sj = Nd == 1 ? 0 : q / N1D;
+        const int nq = T.nq2fq[q];
+        int x = 0;
+        for (int d = 0; d < Nd; d++) {
+            const int posd = d == 0 ? si : sj;
+            for (int side = -1; side <= 1; side += 2) {
+                const int np = posd + side;
+                if (np >= 0 && np < N1D) {
+                    T.stn[q][x] = (int8_t)(d == 0 ? (Nd == 1 ? np : np + N1D * sj) : si + N1D * np);
+                } else {
+                    if (nq < 1) return fail_s(err, P2DE_E_UNSUPPORTED, "an element-boundary node without a face in q2fq");
+                    const int f = T.q2fq[q][nq == 1 ? 0 : d];
+                    T.stn[q][x] = (int8_t)(-(f + 1));
+                }
+                x++;
+            }
+        }
+    }
     return P2DE_OK;
 }
 

@@ -122,6 +122,51 @@ __device__ __forceinline__ double limiting_param(const double* uL, const double*
     return l;
 }
 
+// limiting_param_bound_rho_rhoe (limiter_utils.jl:26-40)
+template <int DIM>
+__device__ __forceinline__ double bound_rho_rhoe(const double* U, const double* P, double Lrho, double Lrhoe, double Urho,
+                                                 double Urhoe, double ZEROTOL) {
+    double l = 1.0;
+    if (U[0] + P[0] < Lrho) l = jmax((Lrho - U[0]) / P[0], 0.0);
+    if (U[0] + P[0] > Urho) l = jmin(l, jmax((Urho - U[0]) / P[0], 0.0));
+    l = jmin(jmin(l, rhoe_quadratic_solve<DIM>(U, P, Lrhoe, ZEROTOL)), rhoe_quadratic_solve<DIM>(U, P, Urhoe, ZEROTOL));
+    return l;
+}
+
+// s_modified_ufun (compressible_Navier_Stokes.jl:80-85): rhoe * rho^-gamma
+template <int DIM>
+__device__ __forceinline__ double s_modified(double g, const double* U) {
+    const double rho = U[0];
+    const double rhoe = Eq<DIM>::rhoe(U);
+    return rhoe * pow(rho, -g);
+}
+
+// limiting_param(::SubcellLimiter, <min-entropy bounds>) (limiter_utils.jl:19-24) with the bound
+// tuple (zeta*rho_L, zeta*rhoe(u_L), Lphi, Inf, Inf): the positivity parameter lpos, then
+// limiting_param_bound_phi (:42-50), the bisection (nonlinear_solvers.jl:3-20) between 0 and lpos
+// for s_modified(u_L + l P) >= Lphi - POSTOL
+template <int DIM>
+__device__ __noinline__ double limiting_param_phi(const double* uL, const double* P, double zeta, double ZEROTOL,
+                                                  double Lphi, double POSTOL, double g) {
+    constexpr int NC = DIM + 2;
+    const double Lrho = zeta * uL[0];
+    const double Lrhoe = zeta * Eq<DIM>::rhoe(uL);
+    const double lpos = bound_rho_rhoe<DIM>(uL, P, Lrho, Lrhoe, INFINITY, INFINITY, ZEROTOL);
+    auto ok = [&](double l) {
+        double Ul[NC];
+        for (int c = 0; c < NC; c++) Ul[c] = uL[c] + l * P[c];
+        return s_modified<DIM>(g, Ul) >= Lphi - POSTOL;
+    };
+    if (ok(lpos)) return lpos;
+    double x_valid = 0.0, x_invalid = lpos;
+    for (int it = 0; it <= 20; it++) {
+        const double x_new = 0.5 * (x_valid + x_invalid);
+        if (ok(x_new)) x_valid = x_new;
+        else x_invalid = x_new;
+    }
+    return x_valid;
+}
+
 // LDS of one workgroup.  Arrays whose lifetimes do not overlap share storage:
 // node/face primitives and fluxes (phases 1-3) with the directional RHS and u_L (phases 4-5),
 // pair results (phases 3-4) with the subcell flux potentials and parameters (phase 5).
@@ -186,7 +231,7 @@ __device__ __forceinline__ void project_face(double g, double coef, const double
     Eq<DIM>::u_of_v(g, Vt, ut);
 }
 
-template <int DIM, int N1D, int E, int NT, int BASIS>
+template <int DIM, int N1D, int E, int NT, int BASIS, int OPT>
 __global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
     using S = Sz<DIM, N1D, BASIS>;
     constexpr int NQ = S::NQ, NFP = S::NFP, NC = S::NC, NPH = S::NPH, NPL = S::NPL, NLINE = S::NLINE;
@@ -467,6 +512,55 @@ __global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
         a.dtp[blockIdx.x] = m;   // reduced by dt_reduce_kernel (one-address atomics serialise across XCDs)
     }
 
+    if (OPT == OPT_ZHANG_SHU) {
+        // apply_zhang_shu_limiter! (zhangshu.jl:4-24): one parameter per element, the largest l
+        // keeping u_L + l dt (rhsH - rhsL) within (zeta rho_L, zeta rhoe_L) at every node
+        // (zhang_shu_bound_limiter! :35-45, limiting_param(::ZhangShuLimiter) limiter_utils.jl:4-8)
+        for (int w = tid; w < E * NQ; w += NT) {
+            const int e = w / NQ, i = w - e * NQ;
+            if (k0 + e >= K) continue;
+            const double* uL = sm.a.g2.UL[e][i];
+            double P[NC];
+            for (int c = 0; c < NC; c++) {
+                double h = sm.a.g2.RH[e][i][0][c], lo = sm.a.g2.RL[e][i][0][c];
+                if (DIM == 2) {
+                    h = h + sm.a.g2.RH[e][i][DIM - 1][c];
+                    lo = lo + sm.a.g2.RL[e][i][DIM - 1][c];
+                }
+                P[c] = dt_lim * (h - lo);
+            }
+            sm.x.fb.l[e][i] = jmin(1.0, bound_rho_rhoe<DIM>(uL, P, a.zeta * uL[0], a.zeta * Eq<DIM>::rhoe(uL), INFINITY,
+                                                            INFINITY, a.ZEROTOL));
+        }
+        __syncthreads();
+        for (int w = tid; w < E * NQ; w += NT) {
+            const int e = w / NQ, i = w - e * NQ;
+            const int k = k0 + e;
+            if (k >= K) continue;
+            double Lk = 1.0;
+            for (int q = 0; q < NQ; q++) Lk = jmin(Lk, sm.x.fb.l[e][q]);
+            const double l = jmin(Lk, a.blend ? a.blend[k] : 1.0);
+            for (int c = 0; c < NC; c++) {
+                double h = sm.a.g2.RH[e][i][0][c], lo = sm.a.g2.RL[e][i][0][c];
+                if (DIM == 2) {
+                    h = h + sm.a.g2.RH[e][i][DIM - 1][c];
+                    lo = lo + sm.a.g2.RL[e][i][DIM - 1][c];
+                }
+                a.rhsUA[((size_t)k * NQ + i) * NC + c] = (1.0 - l) * lo + l * h;
+            }
+            if (a.dbg_L && i == 0) a.dbg_L[k] = Lk;
+        }
+        // no subcell faces: the update kernel's symmetrisation finds nothing to correct
+        for (int w = tid; w < E * NFP; w += NT) {
+            const int e = w / NFP, f = w - e * NFP;
+            const int k = k0 + e;
+            if (k >= K) continue;
+            a.lA[(size_t)k * NFP + f] = 1.0;
+            for (int c = 0; c < NC; c++) a.dfA[((size_t)k * NFP + f) * NC + c] = 0.0;
+        }
+        return;
+    }
+
     // ---- phase 5a: subcell flux potentials along every line (accumulate_f_bar! subcell.jl:163-206)
     for (int w = tid; w < E * NLINE; w += NT) {
         const int e = w / NLINE, line = w - e * NLINE;
@@ -507,20 +601,27 @@ __global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
         const double J = a.geo[k].J;
         double dfb[NC], P[NC];
         for (int c = 0; c < NC; c++) dfb[c] = sm.x.fb.fH[e][r][c] - sm.x.fb.fL[e][r][c];
+        // the bound's limiting_param (limiter_utils.jl:10-24); Lphi is the node's lower entropy bound
+        auto lim = [&](int q) {
+            if (OPT == OPT_SUBCELL_GENERAL && a.bound != 0)
+                return limiting_param_phi<DIM>(sm.a.g2.UL[e][q], P, zeta, ZEROTOL, a.lphi[(size_t)k * NQ + q], a.POSTOL, g);
+            return limiting_param<DIM>(sm.a.g2.UL[e][q], P, zeta, ZEROTOL, a.upper_mode);
+        };
         double l = 1.0;
         if (s < N1D) {   // node on the right of (above) the subcell face
             const int q = line_node<DIM, N1D>(line, s);
             const double wJ = T.wq[q] * J;
             for (int c = 0; c < NC; c++) P[c] = -fac * dt_lim * dfb[c] / wJ;
-            l = jmin(l, limiting_param<DIM>(sm.a.g2.UL[e][q], P, zeta, ZEROTOL, a.upper_mode));
+            l = jmin(l, lim(q));
         }
         if (s > 0) {     // node on the left of (below) the subcell face
             const int q = line_node<DIM, N1D>(line, s - 1);
             const double wJ = T.wq[q] * J;
             for (int c = 0; c < NC; c++) P[c] = fac * dt_lim * dfb[c] / wJ;
-            l = jmin(l, limiting_param<DIM>(sm.a.g2.UL[e][q], P, zeta, ZEROTOL, a.upper_mode));
+            l = jmin(l, lim(q));
         }
-        l = jmin(l, 1.0);   // blending factor of NoShockCapture (shock_capture.jl:111-114)
+        // blending factor of the shock capture (shock_capture.jl:111-132; 1 without)
+        l = jmin(l, (OPT == OPT_SUBCELL_GENERAL && a.blend) ? a.blend[k] : 1.0);
         sm.x.fb.l[e][r] = l;
         if (s == 0 || s == N1D) {
             const int f = line_face<DIM, N1D>(line, s);
@@ -566,6 +667,128 @@ __global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
             }
         }
         for (int c = 0; c < NC; c++) a.rhsUA[((size_t)k * NQ + i) * NC + c] = DIM == 2 ? rx[c] + ry[c] : rx[c];
+    }
+}
+
+// Limiter options, per stage before the element kernel (apply_rhs_limiter! limiter.jl:7-44):
+// initialize_smoothness_indicator! (shock_capture.jl:47-94: rho*p at the nodes, its modal
+// coefficients VDM_inv*(rho*p), the energy share of the top two mode shells), the Hennemann
+// blending factor (shock_capture.jl:117-132), the smoothness factor (subcell.jl:927-956) and the
+// lower bound of the modified specific entropy over each node's low-order stencil
+// (initialize_lower_bound! subcell.jl:37-75), which reaches into the face neighbour.
+template <int DIM, int N1D, int E, int NT>
+__global__ __launch_bounds__(NT) void limiter_prep_kernel(PrepArgs a) {
+    using S = Sz<DIM, N1D>;
+    constexpr int NQ = S::NQ, NFP = S::NFP, NC = S::NC;
+    __shared__ double sS[E][NQ];
+    __shared__ double sI[E][NQ];
+    __shared__ double sF[E];
+    if (a.sc->done) return;
+    const int tid = threadIdx.x;
+    const int k0 = blockIdx.x * E;
+    const int nel = min(E, a.K - k0);
+    const ElemTables& T = *a.T;
+    const double g = a.gamma;
+    const bool need_ind = a.shock != 0 || a.bound == 2;
+    for (int w = tid; w < nel * NQ; w += NT) {
+        const int e = w / NQ, i = w - e * NQ;
+        const double* src = a.U + ((size_t)(k0 + e) * NQ + i) * NC;
+        double u[NC];
+        for (int c = 0; c < NC; c++) u[c] = src[c];
+        if (a.bound != 0) sS[e][i] = s_modified<DIM>(g, u);
+        if (need_ind) sI[e][i] = u[0] * Eq<DIM>::p(g, u);   // initialize_indicator! :82-94
+    }
+    __syncthreads();
+    for (int e = tid; e < nel; e += NT) {
+        const int k = k0 + e;
+        double f = a.bound == 1 ? 1.0 : 0.0;
+        if (need_ind) {
+            double mod[NQ];
+            for (int i = 0; i < NQ; i++) mod[i] = 0.0;
+            for (int j = 0; j < NQ; j++)
+                for (int i = 0; i < NQ; i++) mod[i] += T.vdm_inv[i][j] * sI[e][j];
+            const int N = N1D - 1;
+            double modeN = 0.0, modeNm1 = 0.0, total = 0.0;
+            int count = 0;
+            for (int j = 0; j <= (DIM == 2 ? N : 0); j++)
+                for (int i = 0; i <= N; i++) {
+                    const double energy = mod[count] * mod[count];
+                    if (i == N || (DIM == 2 && j == N)) modeN += energy;
+                    if (i == N - 1 || (DIM == 2 && j == N - 1)) modeNm1 += energy;
+                    total += energy;
+                    count++;
+                }
+            const double ind = jmax(modeN / total, modeNm1 / total);
+            if (a.shock) {
+                const double TN = a.shock_a * pow(10.0, -a.shock_c * pow((double)(N + 1), 0.25));
+                const double alphamax = 0.5, alphaE0 = 0.0001;
+                const double s_factor = log((1.0 - alphaE0) / alphaE0);
+                const double alpha = 1.0 / (1.0 + exp(-s_factor / TN * (ind - TN)));
+                a.blend[k] = jmax(jmin(1.0 - alpha, 1.0), alphamax);
+            }
+            if (a.bound == 2) {
+                const double kappa = 1.0;
+                const double s0 = log(pow((double)N, -4.0)) / log(10.0);
+                const double sk = log(ind) / log(10.0);
+                if (sk < s0 - kappa) f = 0.0;
+                else if (sk > s0 + kappa) f = 1.0;
+                else f = 0.5 - 0.5 * sin(3.141592653589793 * (sk - s0) / (2.0 * kappa));
+            }
+        }
+        sF[e] = f;
+    }
+    __syncthreads();
+    if (a.bound == 0) return;
+    for (int w = tid; w < nel * NQ; w += NT) {
+        const int e = w / NQ, q = w - e * NQ;
+        const int k = k0 + e;
+        double lb = sS[e][q];
+        for (int x = 0; x < 2 * DIM; x++) {
+            const int st = T.stn[q][x];
+            if (st >= 0) {
+                lb = jmin(lb, sS[e][st]);
+            } else {
+                const double* src = a.U + (size_t)a.nb_node[(size_t)k * NFP + (-st - 1)] * NC;
+                double u[NC];
+                for (int c = 0; c < NC; c++) u[c] = src[c];
+                lb = jmin(lb, s_modified<DIM>(g, u));
+            }
+        }
+        a.lphi[(size_t)k * NQ + q] = sF[e] * lb + (1.0 - sF[e]) * a.sc->s_min;
+    }
+}
+
+// initialize_s_modified! (subcell.jl:19-35): at the first stage of t0, the minimum of
+// rhoe*rho^-gamma over every node, per workgroup then in one workgroup
+template <int DIM>
+__global__ __launch_bounds__(256) void s_min_partial_kernel(const double* U, int64_t nodes, double g, const double* tp,
+                                                            double tv, double t0, double* part) {
+    __shared__ double red[4];
+    const double t = tp ? *tp : tv;
+    if (!(t == t0)) return;
+    const size_t gidx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double m = INFINITY;
+    if (gidx < (size_t)nodes) m = s_modified<DIM>(g, U + gidx * (DIM + 2));
+    for (int off = 32; off > 0; off >>= 1) m = jmin(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = jmin(jmin(red[0], red[1]), jmin(red[2], red[3]));
+}
+
+__global__ __launch_bounds__(1024) void s_min_reduce_kernel(const double* part, int n, const double* tp, double tv,
+                                                            double t0, StepScalars* sc) {
+    __shared__ double red[16];
+    const double t = tp ? *tp : tv;
+    if (!(t == t0)) return;
+    double m = INFINITY;
+    for (int b = (int)threadIdx.x; b < n; b += 1024) m = jmin(m, part[b]);
+    for (int off = 32; off > 0; off >>= 1) m = jmin(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int x = 1; x < 16; x++) m = jmin(m, red[x]);
+        m = jmin(m, red[0]);
+        sc->s_min = m;
     }
 }
 
@@ -854,10 +1077,10 @@ __global__ void positivity_kernel(const double* U, int64_t nodes, double gamma, 
 }
 
 // --------------------------------------------------------------------------- launchers
-template <int DIM, int N1D, int E, int NT, int BASIS>
+template <int DIM, int N1D, int E, int NT, int BASIS, int OPT>
 static int launchA(const PhaseAArgs& a, hipStream_t s) {
     const int blocks = (a.K + E - 1) / E;
-    hipLaunchKernelGGL((phaseA_kernel<DIM, N1D, E, NT, BASIS>), dim3(blocks), dim3(NT), 0, s, a);
+    hipLaunchKernelGGL((phaseA_kernel<DIM, N1D, E, NT, BASIS, OPT>), dim3(blocks), dim3(NT), 0, s, a);
     return (int)hipGetLastError();
 }
 template <int DIM, int N1D>
@@ -873,9 +1096,9 @@ static int launchB(const PhaseBArgs& a, hipStream_t s) {
 // picks one per (Nd, N).  The first entry is the default.
 struct VariantA {
     int E, NT;
-    int (*launch)(const PhaseAArgs&, hipStream_t);
+    int (*launch[3])(const PhaseAArgs&, hipStream_t);   // by OPT_*
 };
-#define VA(D, N, E, T) {E, T, &launchA<D, N, E, T, 0>}
+#define VA(D, N, E, T) {E, T, {&launchA<D, N, E, T, 0, 0>, &launchA<D, N, E, T, 0, 1>, &launchA<D, N, E, T, 0, 2>}}
 static const VariantA kV1_2[] = {VA(1, 2, 32, 64), VA(1, 2, 32, 128)};
 static const VariantA kV1_3[] = {VA(1, 3, 21, 64), VA(1, 3, 21, 128)};
 static const VariantA kV1_4[] = {VA(1, 4, 16, 64), VA(1, 4, 16, 128)};
@@ -887,7 +1110,7 @@ static const VariantA kV2_4[] = {VA(2, 4, 4, 64), VA(2, 4, 4, 128), VA(2, 4, 4, 
 static const VariantA kV2_5[] = {VA(2, 5, 2, 64), VA(2, 5, 2, 128), VA(2, 5, 4, 128), VA(2, 5, 4, 256), VA(2, 5, 1, 64)};
 #undef VA
 // Gauss: more face-volume pairs per element, so fewer elements per workgroup
-#define VG(D, N, E, T) {E, T, &launchA<D, N, E, T, 1>}
+#define VG(D, N, E, T) {E, T, {&launchA<D, N, E, T, 1, 0>, &launchA<D, N, E, T, 1, 1>, &launchA<D, N, E, T, 1, 2>}}
 static const VariantA kG1_2[] = {VG(1, 2, 32, 64), VG(1, 2, 32, 128)};
 static const VariantA kG1_3[] = {VG(1, 3, 16, 64), VG(1, 3, 16, 128)};
 static const VariantA kG1_4[] = {VG(1, 4, 12, 64), VG(1, 4, 12, 128)};
@@ -928,11 +1151,54 @@ int variant_shape_A(int Nd, int N1D, int basis, int v, int* E, int* NT) {
     return 0;
 }
 
-int launch_phaseA(int Nd, int N1D, int basis, int variant, const PhaseAArgs& a, void* stream) {
+int launch_phaseA(int Nd, int N1D, int basis, int variant, int opt, const PhaseAArgs& a, void* stream) {
     int n = 0;
     const VariantA* t = variants(Nd, N1D, basis, &n);
-    if (!t || variant < 0 || variant >= n) return (int)hipErrorInvalidValue;
-    return t[variant].launch(a, (hipStream_t)stream);
+    if (!t || variant < 0 || variant >= n || opt < 0 || opt > 2) return (int)hipErrorInvalidValue;
+    return t[variant].launch[opt](a, (hipStream_t)stream);
+}
+
+template <int DIM, int N1D>
+static int launchP(const PrepArgs& a, hipStream_t s) {
+    constexpr int NQ = Sz<DIM, N1D>::NQ;
+    constexpr int E = 256 / NQ;
+    const int blocks = (a.K + E - 1) / E;
+    hipLaunchKernelGGL((limiter_prep_kernel<DIM, N1D, E, 256>), dim3(blocks), dim3(256), 0, s, a);
+    return (int)hipGetLastError();
+}
+
+int launch_limiter_prep(int Nd, int N1D, const PrepArgs& a, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (Nd == 1) {
+        switch (N1D) {
+            case 2: return launchP<1, 2>(a, s);
+            case 3: return launchP<1, 3>(a, s);
+            case 4: return launchP<1, 4>(a, s);
+            case 5: return launchP<1, 5>(a, s);
+        }
+    } else {
+        switch (N1D) {
+            case 2: return launchP<2, 2>(a, s);
+            case 3: return launchP<2, 3>(a, s);
+            case 4: return launchP<2, 4>(a, s);
+            case 5: return launchP<2, 5>(a, s);
+        }
+    }
+    return (int)hipErrorInvalidValue;
+}
+
+int launch_s_min(const double* U, int64_t nodes, int Nd, double gamma, const double* tp, double t_value, double t0,
+                 double* part, StepScalars* sc, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    const int blocks = (int)((nodes + 255) / 256);
+    if (Nd == 1)
+        hipLaunchKernelGGL(s_min_partial_kernel<1>, dim3(blocks), dim3(256), 0, s, U, nodes, gamma, tp, t_value, t0, part);
+    else
+        hipLaunchKernelGGL(s_min_partial_kernel<2>, dim3(blocks), dim3(256), 0, s, U, nodes, gamma, tp, t_value, t0, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL(s_min_reduce_kernel, dim3(1), dim3(1024), 0, s, part, blocks, tp, t_value, t0, sc);
+    return (int)hipGetLastError();
 }
 
 // entropy projection: about 128 face nodes per workgroup

@@ -44,14 +44,15 @@ class _Config(C.Structure):
                 ("rhs", C.c_int), ("low_order_surface_flux", C.c_int), ("high_order_surface_flux", C.c_int),
                 ("high_order_volume_flux", C.c_int), ("basis", C.c_int), ("entropyproj_limiter", C.c_int),
                 ("rhs_limiter", C.c_int), ("bound", C.c_int), ("shockcapture", C.c_int),
-                ("upper_bound_mode", C.c_int), ("precision", C.c_int)]
+                ("upper_bound_mode", C.c_int), ("precision", C.c_int),
+                ("shock_a", C.c_double), ("shock_c", C.c_double)]
 
 
 class _Operators(C.Structure):
     _fields_ = [("Nq", C.c_int), ("Nfp", C.c_int), ("Nh", C.c_int), ("Srsh_db", _dp), ("Srs0", _dp),
                 ("nnz_h", C.c_int), ("Srsh_nnz", _ip), ("nnz_0", C.c_int), ("Srs0_nnz", _ip), ("Brs", _dp),
                 ("MinvVhT", _dp), ("MinvVfT", _dp), ("Vf", _dp), ("Vf_low", _dp), ("wq", _dp), ("fq2q", _ip),
-                ("q2fq", _ip)]
+                ("q2fq", _ip), ("VDM_inv", _dp)]
 
 
 class _Geometry(C.Structure):
@@ -177,17 +178,23 @@ def _descriptors(fp: FlatProblem, upper_bound_mode: int = None, precision: int =
     cfg = _Config(Nd=fp.Nd, N=fp.N1D - 1, Kx=fp.Kx, Ky=fp.Ky, gamma=fp.gamma, zeta=fp.zeta, eta=fp.eta,
                   POSTOL=fp.POSTOL, ZEROTOL=fp.ZEROTOL, CFL=fp.CFL, dt0=fp.dt0, t0=fp.t0, T=fp.T,
                   high_order_surface_flux=1, upper_bound_mode=int(mode), precision=int(prec),
-                  basis=int(getattr(fp, "basis", 0)), entropyproj_limiter=int(getattr(fp, "entropyproj_limiter", 0)))
+                  basis=int(getattr(fp, "basis", 0)), entropyproj_limiter=int(getattr(fp, "entropyproj_limiter", 0)),
+                  rhs_limiter=int(getattr(fp, "rhs_limiter", 0)), bound=int(getattr(fp, "bound", 0)),
+                  shockcapture=int(getattr(fp, "shockcapture", 0)), shock_a=float(getattr(fp, "shock_a", 0.5)),
+                  shock_c=float(getattr(fp, "shock_c", 1.8)))
     keep = {}
     for n in ("Srsh_db", "Srs0", "Brs", "MinvVhT", "MinvVfT", "Vf", "Vf_low", "wq", "Jq", "GJh", "Ival"):
         keep[n] = np.ascontiguousarray(getattr(fp, n), dtype=np.float64)
+    vinv = getattr(fp, "VDM_inv", None)
+    keep["VDM_inv"] = None if vinv is None else np.ascontiguousarray(vinv, dtype=np.float64)
     for n in ("Srsh_nnz", "Srs0_nnz", "fq2q", "q2fq", "mapP", "mapI", "mapO"):
         keep[n] = np.ascontiguousarray(getattr(fp, n), dtype=np.int64)
     ops = _Operators(Nq=fp.Nq, Nfp=fp.Nfp, Nh=fp.Nh, Srsh_db=_d(keep["Srsh_db"]), Srs0=_d(keep["Srs0"]),
                      nnz_h=len(keep["Srsh_nnz"]) // 2, Srsh_nnz=_i(keep["Srsh_nnz"]),
                      nnz_0=len(keep["Srs0_nnz"]) // 2, Srs0_nnz=_i(keep["Srs0_nnz"]), Brs=_d(keep["Brs"]),
                      MinvVhT=_d(keep["MinvVhT"]), MinvVfT=_d(keep["MinvVfT"]), Vf=_d(keep["Vf"]),
-                     Vf_low=_d(keep["Vf_low"]), wq=_d(keep["wq"]), fq2q=_i(keep["fq2q"]), q2fq=_i(keep["q2fq"]))
+                     Vf_low=_d(keep["Vf_low"]), wq=_d(keep["wq"]), fq2q=_i(keep["fq2q"]), q2fq=_i(keep["q2fq"]),
+                     VDM_inv=_d(keep["VDM_inv"]) if keep["VDM_inv"] is not None else None)
     geo = _Geometry(Jq=_d(keep["Jq"]), GJh=_d(keep["GJh"]))
     bc = _BCData(mapP=_i(keep["mapP"]), nI=len(keep["mapI"]), mapI=_i(keep["mapI"]), Ival=_d(keep["Ival"]),
                  nO=len(keep["mapO"]), mapO=_i(keep["mapO"]))

@@ -38,11 +38,14 @@ def _create(fp, **cfg_over):
     """Call p2de_create like DeviceSolver does, with config overrides; return (status, msg)."""
     cfg = S._Config(Nd=fp.Nd, N=fp.N1D - 1, Kx=fp.Kx, Ky=fp.Ky, gamma=fp.gamma, zeta=fp.zeta, eta=fp.eta,
                     POSTOL=fp.POSTOL, ZEROTOL=fp.ZEROTOL, CFL=fp.CFL, dt0=fp.dt0, t0=fp.t0, T=fp.T,
-                    high_order_surface_flux=1, basis=fp.basis, entropyproj_limiter=fp.entropyproj_limiter)
+                    high_order_surface_flux=1, basis=fp.basis, entropyproj_limiter=fp.entropyproj_limiter,
+                    rhs_limiter=fp.rhs_limiter, bound=fp.bound, shockcapture=fp.shockcapture,
+                    shock_a=fp.shock_a, shock_c=fp.shock_c)
     for k, v in cfg_over.items():
         setattr(cfg, k, v)
-    keep = {n: np.ascontiguousarray(getattr(fp, n), dtype=np.float64)
-            for n in ("Srsh_db", "Srs0", "Brs", "MinvVhT", "MinvVfT", "Vf", "Vf_low", "wq", "Jq", "GJh", "Ival")}
+    keep = {n: None if getattr(fp, n) is None else np.ascontiguousarray(getattr(fp, n), dtype=np.float64)
+            for n in ("Srsh_db", "Srs0", "Brs", "MinvVhT", "MinvVfT", "Vf", "Vf_low", "wq", "Jq", "GJh", "Ival",
+                      "VDM_inv")}
     keep.update({n: np.ascontiguousarray(getattr(fp, n), dtype=np.int64)
                  for n in ("Srsh_nnz", "Srs0_nnz", "fq2q", "q2fq", "mapP", "mapI", "mapO")})
     return _create_raw(fp, cfg, keep)
@@ -54,7 +57,8 @@ def _create_raw(fp, cfg, keep):
                        nnz_h=len(keep["Srsh_nnz"]) // 2, Srsh_nnz=i(keep["Srsh_nnz"]),
                        nnz_0=len(keep["Srs0_nnz"]) // 2, Srs0_nnz=i(keep["Srs0_nnz"]), Brs=d(keep["Brs"]),
                        MinvVhT=d(keep["MinvVhT"]), MinvVfT=d(keep["MinvVfT"]), Vf=d(keep["Vf"]),
-                       Vf_low=d(keep["Vf_low"]), wq=d(keep["wq"]), fq2q=i(keep["fq2q"]), q2fq=i(keep["q2fq"]))
+                       Vf_low=d(keep["Vf_low"]), wq=d(keep["wq"]), fq2q=i(keep["fq2q"]), q2fq=i(keep["q2fq"]),
+                       VDM_inv=d(keep["VDM_inv"]))
     geo = S._Geometry(Jq=d(keep["Jq"]), GJh=d(keep["GJh"]))
     bc = S._BCData(mapP=i(keep["mapP"]), nI=len(keep["mapI"]), mapI=i(keep["mapI"]), Ival=d(keep["Ival"]),
                    nO=len(keep["mapO"]), mapO=i(keep["mapO"]))
@@ -72,8 +76,9 @@ def fp2():
 
 
 @pytest.mark.parametrize("field,value,code", [
-    ("basis", 1, S.P2DE_E_UNSUPPORTED), ("rhs_limiter", 1, S.P2DE_E_UNSUPPORTED),
-    ("entropyproj_limiter", 1, S.P2DE_E_UNSUPPORTED), ("shockcapture", 1, S.P2DE_E_UNSUPPORTED),
+    ("basis", 1, S.P2DE_E_UNSUPPORTED), ("rhs_limiter", 5, S.P2DE_E_UNSUPPORTED),
+    ("entropyproj_limiter", 1, S.P2DE_E_UNSUPPORTED), ("shockcapture", 3, S.P2DE_E_UNSUPPORTED),
+    ("bound", 3, S.P2DE_E_UNSUPPORTED),
     ("high_order_surface_flux", 0, S.P2DE_E_UNSUPPORTED), ("precision", 7, S.P2DE_E_INVALID),
     ("upper_bound_mode", 5, S.P2DE_E_INVALID), ("Nd", 3, S.P2DE_E_INVALID), ("N", 9, S.P2DE_E_UNSUPPORTED),
     ("Kx", 0, S.P2DE_E_INVALID), ("gamma", 1.0, S.P2DE_E_INVALID)])
@@ -89,7 +94,8 @@ def test_create_rejects_bad_operators_and_mesh(fp2):
 
     def base():
         keep = {n: np.ascontiguousarray(getattr(fp2, n), dtype=np.float64).copy()
-                for n in ("Srsh_db", "Srs0", "Brs", "MinvVhT", "MinvVfT", "Vf", "Vf_low", "wq", "Jq", "GJh", "Ival")}
+                for n in ("Srsh_db", "Srs0", "Brs", "MinvVhT", "MinvVfT", "Vf", "Vf_low", "wq", "Jq", "GJh", "Ival",
+                          "VDM_inv")}
         keep.update({n: np.ascontiguousarray(getattr(fp2, n), dtype=np.int64).copy()
                      for n in ("Srsh_nnz", "Srs0_nnz", "fq2q", "q2fq", "mapP", "mapI", "mapO")})
         return keep
@@ -182,3 +188,18 @@ def test_gauss_options_and_operators_are_checked():
     assert st == S.P2DE_E_UNSUPPORTED and "Lobatto" in msg
     st, msg = _create(fpg, basis=7)
     assert st == S.P2DE_E_UNSUPPORTED and msg
+
+
+def test_limiter_options_are_checked():
+    """Zhang-Shu takes no subcell bound; the shock indicator and the relaxed bound need VDM_inv;
+    the mixed-precision path runs SubcellLimiter(PositivityBound, NoShockCapture) only."""
+    _, _, _, zs = problem("smoke_vortex", limiter="zhangshu_shock")
+    st, msg = _create(zs, bound=1)
+    assert st == S.P2DE_E_UNSUPPORTED and "Zhang" in msg
+    _, _, _, rel = problem("smoke_vortex", limiter="relaxed_minentropy")
+    rel.VDM_inv = None
+    st, msg = _create(rel)
+    assert st == S.P2DE_E_INVALID and "VDM_inv" in msg
+    _, _, _, me = problem("smoke_vortex", limiter="minentropy")
+    st, msg = _create(me, precision=S.PRECISION_MIXED)
+    assert st == S.P2DE_E_UNSUPPORTED and "mixed" in msg
