@@ -60,7 +60,7 @@ typedef struct {
     double gamma, zeta, eta, POSTOL, ZEROTOL;
     double CFL, dt0, t0, T;
     int rhs;                      /* P2DE_RHS_LIMITED_DG */
-    int low_order_surface_flux;   /* P2DE_LF_ON_NODAL_VAL */
+    int low_order_surface_flux;   /* P2DE_LF_ON_NODAL_VAL or P2DE_LF_ON_PROJECTED_VAL */
     int high_order_surface_flux;  /* P2DE_LF_ON_PROJECTED_VAL */
     int high_order_volume_flux;   /* P2DE_CHANDRASHEKAR_FLUX */
     int basis;                    /* P2DE_BASIS_LOBATTO or P2DE_BASIS_GAUSS */

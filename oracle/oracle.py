@@ -37,6 +37,7 @@ class _Problem(C.Structure):
         ("ZEROTOL", C.c_double), ("CFL", C.c_double), ("dt0", C.c_double), ("t0", C.c_double), ("T", C.c_double),
         ("upper_bound_mode", C.c_int), ("entropyproj_limiter", C.c_int), ("rhs_limiter", C.c_int),
         ("bound", C.c_int), ("shockcapture", C.c_int), ("shock_a", C.c_double), ("shock_c", C.c_double),
+        ("low_order_surface_flux", C.c_int),
         ("Srsh_db", _dp), ("Srs0", _dp), ("nnz_h", C.c_int), ("Srsh_nnz", _ip), ("nnz_0", C.c_int),
         ("Srs0_nnz", _ip), ("Brs", _dp), ("MinvVhT", _dp), ("MinvVfT", _dp), ("Vf", _dp), ("Vf_low", _dp),
         ("wq", _dp), ("fq2q", _ip), ("q2fq", _ip), ("VDM_inv", _dp), ("Jq", _dp), ("GJh", _dp), ("mapP", _ip),
@@ -82,6 +83,8 @@ def lib():
         L.oracle_pt_wavespeed.restype = C.c_double
         L.oracle_pt_limiting_param.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp]
         L.oracle_pt_limiting_param.restype = C.c_double
+        L.oracle_pt_find_alpha.argtypes = [C.c_int, C.c_double, _dp, _dp]
+        L.oracle_pt_find_alpha.restype = C.c_double
         L.oracle_set_threads.argtypes = [C.c_int]
         L.oracle_get_threads.restype = C.c_int
         _lib = L
@@ -118,6 +121,7 @@ class Oracle:
             rhs_limiter=int(getattr(fp, "rhs_limiter", 0)), bound=int(getattr(fp, "bound", 0)),
             shockcapture=int(getattr(fp, "shockcapture", 0)), shock_a=float(getattr(fp, "shock_a", 0.5)),
             shock_c=float(getattr(fp, "shock_c", 1.8)), VDM_inv=_d(self._keep["VDM_inv"]),
+            low_order_surface_flux=int(getattr(fp, "low_order_surface_flux", 0)),
             Srsh_db=_d(k["Srsh_db"]), Srs0=_d(k["Srs0"]), nnz_h=len(k["Srsh_nnz"]) // 2,
             Srsh_nnz=_i(k["Srsh_nnz"]), nnz_0=len(k["Srs0_nnz"]) // 2, Srs0_nnz=_i(k["Srs0_nnz"]),
             Brs=_d(k["Brs"]), MinvVhT=_d(k["MinvVhT"]), MinvVfT=_d(k["MinvVfT"]), Vf=_d(k["Vf"]),
@@ -223,3 +227,8 @@ def pt_wavespeed(Nd, gamma, U, n):
 def pt_limiting_param(Nd, zeta, ZEROTOL, mode, uL, P):
     uL, P = _vec(uL), _vec(P)
     return lib().oracle_pt_limiting_param(Nd, zeta, ZEROTOL, mode, _d(uL), _d(P))
+
+
+def pt_find_alpha(Nd, POSTOL, ui, ut):
+    ui, ut = _vec(ui), _vec(ut)
+    return lib().oracle_pt_find_alpha(Nd, POSTOL, _d(ui), _d(ut))

@@ -224,6 +224,7 @@ struct oracle_ctx {
     double *epUf, *epVUf, *eprhoef, *theta_local;
     /* smoothness indicator (State.jl:23-25), entropy bounds (State.jl:148-153), Zhang-Shu L */
     double *indicator, *indicator_modal, *smooth_indicator, *s_modified, *L_zs;
+    double *alpha;           /* (Nfp, K) LaxFriedrichsOnProjectedVal CFL factor */
     double s_modified_min;
     int nthreads_alloc;
     double *stage_buf;       /* rhsU copy for the RK driver */
@@ -248,7 +249,7 @@ int oracle_create(const oracle_problem *p, oracle_ctx **out) {
     if (p->Nc != p->Nd + 2) return 2;
     if (p->entropyproj_limiter != 0 && p->entropyproj_limiter != 1) return 2;
     if (p->rhs_limiter < 0 || p->rhs_limiter > 1 || p->bound < 0 || p->bound > 2 || p->shockcapture < 0 ||
-        p->shockcapture > 1)
+        p->shockcapture > 1 || p->low_order_surface_flux < 0 || p->low_order_surface_flux > 1)
         return 2;
     if (!p->VDM_inv && (p->shockcapture != 0 || p->bound == 2)) return 2;
     oracle_ctx *c = (oracle_ctx *)calloc(1, sizeof(oracle_ctx));
@@ -314,6 +315,7 @@ int oracle_create(const oracle_problem *p, oracle_ctx **out) {
     c->smooth_indicator = zalloc(K);
     c->s_modified = zalloc(Nq * K);
     c->L_zs = zalloc(K * 3);
+    c->alpha = zalloc(Nfp * K);
     c->s_modified_min = 0.0;   /* SubcellLimiterCache: zeros(Float64, 1) (State.jl:180) */
     *out = c;
     return 0;
@@ -329,7 +331,8 @@ void oracle_destroy(oracle_ctx *c) {
                     &c->MinvVhTQF1, &c->MinvVfTBF1, &c->f_bar_H, &c->f_bar_L, &c->f_bar_lim,
                     &c->lbound_s_modified, &c->uL, &c->blending_factor, &c->smooth_factor,
                     &c->dtarr, &c->stage_buf, &c->epUf, &c->epVUf, &c->eprhoef, &c->theta_local,
-                    &c->indicator, &c->indicator_modal, &c->smooth_indicator, &c->s_modified, &c->L_zs};
+                    &c->indicator, &c->indicator_modal, &c->smooth_indicator, &c->s_modified, &c->L_zs,
+                    &c->alpha};
     for (size_t i = 0; i < sizeof(f) / sizeof(f[0]); i++) free(*f[i]);
     free(c);
 }
@@ -560,6 +563,29 @@ static void entropy_projection(oracle_ctx *c, const double *Uq, int nstage, int 
     }
 }
 
+/* find_alpha(::CompressibleIdealGas) low_order_graph_viscosity.jl:299-328: the smallest alpha
+ * (50 bisection steps after doubling) with alpha*u_i - u~ of positive density and internal energy.
+ * The doubling loop of the reference has no exit when no alpha works; here it stops once alpha
+ * overflows to Inf (returned). */
+static int alpha_ok(const oracle_ctx *c, const double *ui, const double *ut, double a) {
+    double sub[NC_MAX];
+    for (int cc = 0; cc < c->p.Nc; cc++) sub[cc] = a * ui[cc] - ut[cc];
+    return sub[0] > c->p.POSTOL && rhoe_ufun(c->p.Nd, sub) > c->p.POSTOL;
+}
+static double find_alpha(const oracle_ctx *c, const double *ui, const double *ut) {
+    double alphaL = 0.0, alphaR = 1.0;
+    while (!alpha_ok(c, ui, ut, alphaR)) {
+        alphaR = 2.0 * alphaR;
+        if (isinf(alphaR)) return alphaR;
+    }
+    for (int iter = 0; iter < 50; iter++) {
+        double alphaM = (alphaL + alphaR) / 2.0;
+        if (alpha_ok(c, ui, ut, alphaM)) alphaR = alphaM;
+        else alphaL = alphaM;
+    }
+    return alphaR;
+}
+
 /* ------------------------------------------------------------------------------- */
 /* rhs_low_graph_visc! low_order_graph_viscosity.jl:4-41                            */
 /* ------------------------------------------------------------------------------- */
@@ -570,10 +596,12 @@ static double rhs_low_graph_visc(oracle_ctx *c, const double *Uq, double t, doub
     /* calculate_wavespeed_and_inviscid_flux! :43-49 */
     OMP_FOR
     for (int k = 0; k < K; k++) {
-        /* update_face_values!(::LaxFriedrichsOnNodalVal) :51-61 */
+        /* update_face_values!(::LaxFriedrichsOnNodalVal) :51-61, (::LaxFriedrichsOnProjectedVal) :63-71 */
         for (int i = 0; i < Nfp; i++) {
             int iq = (int)p->fq2q[i] - 1;
-            for (int cc = 0; cc < Nc; cc++) c->Uf[IDX_V(i, k, Nfp, Nc) + cc] = Uq[IDX_V(iq, k, Nq, Nc) + cc];
+            const double *src = p->low_order_surface_flux == 1 ? c->u_tilde + IDX_V(Nq + i, k, Nh, Nc)
+                                                               : Uq + IDX_V(iq, k, Nq, Nc);
+            for (int cc = 0; cc < Nc; cc++) c->Uf[IDX_V(i, k, Nfp, Nc) + cc] = src[cc];
         }
         /* update_wavespeed_and_inviscid_flux! :73-92 */
         for (int i = 0; i < Nq; i++) fluxes(Nd, g, Uq + IDX_V(i, k, Nq, Nc), c->flux + IDX_XY(0, i, k, Nh, Nd, Nc));
@@ -693,6 +721,13 @@ static double rhs_low_graph_visc(oracle_ctx *c, const double *Uq, double t, doub
         double dt0 = jmin(p->CFL * p->dt0, p->T - t);
         OMP_FOR
         for (int k = 0; k < K; k++) {
+            /* accumulate_alpha!(::LaxFriedrichsOnProjectedVal) :249-261 */
+            if (p->low_order_surface_flux == 1)
+                for (int i = 0; i < Nfp; i++) {
+                    int iq = (int)p->fq2q[i] - 1;
+                    c->alpha[i + (int64_t)Nfp * k] =
+                        find_alpha(c, Uq + IDX_V(iq, k, Nq, Nc), c->u_tilde + IDX_V(Nq + i, k, Nh, Nc));
+                }
             double dtk = dt0;
             for (int i = 0; i < Nq; i++) {
                 double wq_i = p->wq[i];
@@ -703,7 +738,15 @@ static double rhs_low_graph_visc(oracle_ctx *c, const double *Uq, double t, doub
                 for (int m = 0; m < 2; m++) {
                     int64_t f = p->q2fq[i + (int64_t)Nq * m];
                     if (f <= 0) continue;
-                    li += c->lambdaB[(f - 1) + (int64_t)Nfp * k];
+                    /* lambda_B_CFL :283-291 */
+                    if (p->low_order_surface_flux == 1) {
+                        double B[2] = {0.0, 0.0};
+                        double nrm = Bx_with_n(c, (int)f - 1, k, B);
+                        int64_t fk = (f - 1) + (int64_t)Nfp * k;
+                        li += c->alpha[fk] * c->lambdaB[fk] + 0.5 * nrm * c->wavespeed_f[fk];
+                    } else {
+                        li += c->lambdaB[(f - 1) + (int64_t)Nfp * k];
+                    }
                 }
                 dtk = jmin(dtk, p->CFL * 0.5 * wJq_i / li);
             }
@@ -981,6 +1024,14 @@ static double limiting_param(const oracle_ctx *c, const double *uL, const double
 /* Pointwise hooks for the tests                                                    */
 /* ------------------------------------------------------------------------------- */
 void oracle_pt_v_of_u(int Nd, double gamma, const double *U, double *V) { v_ufun(Nd, gamma, U, V); }
+double oracle_pt_find_alpha(int Nd, double POSTOL, const double *ui, const double *ut) {
+    oracle_ctx c;
+    memset(&c, 0, sizeof(c));
+    c.p.Nd = Nd;
+    c.p.Nc = Nd + 2;
+    c.p.POSTOL = POSTOL;
+    return find_alpha(&c, ui, ut);
+}
 void oracle_pt_u_of_v(int Nd, double gamma, const double *V, double *U) { u_vfun(Nd, gamma, V, U); }
 void oracle_pt_flux(int Nd, double gamma, const double *U, double *F) { fluxes(Nd, gamma, U, F); }
 double oracle_pt_logmean(double aL, double aR) { return logmean(aL, aR, log(aL), log(aR)); }

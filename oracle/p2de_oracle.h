@@ -52,6 +52,9 @@ typedef struct {
      * (shock_capture.jl:10-132) */
     int shockcapture;
     double shock_a, shock_c;
+    /* ESLimitedLowOrderPos low_order_surface_flux: 0 LaxFriedrichsOnNodalVal, 1
+     * LaxFriedrichsOnProjectedVal (low_order_graph_viscosity.jl:63-71, 249-327) */
+    int low_order_surface_flux;
     /* Operators (Solver.jl:192-212) */
     const double *Srsh_db;   /* Nd blocks of Nh x Nh */
     const double *Srs0;      /* Nd blocks of Nq x Nq */
@@ -144,6 +147,7 @@ double oracle_pt_logmean(double aL, double aR);
 double oracle_pt_wavespeed(int Nd, double gamma, const double *U, const double *n);
 double oracle_pt_limiting_param(int Nd, double zeta, double ZEROTOL, int upper_bound_mode,
                                 const double *uL, const double *P);
+double oracle_pt_find_alpha(int Nd, double POSTOL, const double *ui, const double *ut);
 
 /* OpenMP threads used by the loops (0 = runtime default). */
 void oracle_set_threads(int n);

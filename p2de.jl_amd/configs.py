@@ -12,6 +12,8 @@ Problems (γ = 1.4, ζ = 0.1, POSTOL = 1e-14, ZEROTOL = 5e-16 unless stated):
 * ``kelvin_helmholtz``-- periodic [-1,1]^2, 512x512, N=4 (examples/2D/kelvin-helmholtz.jl:9-18).
 * ``kelvin_helmholtz_gauss`` -- the reference's KH example as written: 80x80, N=3, Gauss
                          collocation with NodewiseScaledExtrapolation, dt0 1e-3.
+* ``astro_jet``       -- the reference's Mach-2000-class jet (examples/2D/astro-jet.jl): inflow on
+                         the left, outflow on the right, gamma 5/3, Gauss + relaxed min-entropy bound.
 * ``smoke_vortex``    -- the reference's own smoke problem (test/test_smoke.jl:6-20,117-129).
 * ``periodic_wave``   -- one instance of the batched workload: periodic [0,1]^2, density
                          1 + 0.5 sin(2 pi kx x) sin(2 pi ky y), constant velocity, p = 1, with an
@@ -35,7 +37,8 @@ import numpy as np
 from .dg import initialize_DG, make_periodic, primitive_to_conservative
 from .operators import BCData
 from .types import (CompressibleEulerIdealGas, Dim1, Dim2, ESLimitedLowOrderPos, GaussCollocation,
-                    GlobalConstant, HennemannShockCapture, LimitingParameter, LobattoCollocation,
+                    GlobalConstant, HennemannShockCapture, LaxFriedrichsOnNodalVal, LaxFriedrichsOnProjectedVal,
+                    LimitingParameter, LobattoCollocation,
                     NodewiseScaledExtrapolation, NoEntropyProjectionLimiter, Param,
                     PositivityAndMinEntropyBound, PositivityAndRelaxedMinEntropyBound, PositivityBound,
                     SubcellLimiter, TimesteppingParameter, ZhangShuLimiter)
@@ -66,14 +69,15 @@ RECORDED_DT0: Dict[str, float] = {
 }
 
 
-def _param(N, K, xL, xR, T, CFL, dt0, Nd, zeta=0.1, eta=1.0, basis="lobatto", limiter="subcell"):
+def _param(N, K, xL, xR, T, CFL, dt0, Nd, zeta=0.1, eta=1.0, basis="lobatto", limiter="subcell", low_flux="nodal"):
     B, EPL = BASES[basis]
     return Param(N=N, K=K, xL=xL, xR=xR,
                  global_constants=GlobalConstant(POSTOL=1e-14, ZEROTOL=5e-16),
                  timestepping_param=TimesteppingParameter(T=T, CFL=CFL, dt0=dt0, t0=0.0),
                  limiting_param=LimitingParameter(zeta=zeta, eta=eta),
                  equation=CompressibleEulerIdealGas(Dim1() if Nd == 1 else Dim2(), 1.4),
-                 rhs=ESLimitedLowOrderPos(),
+                 rhs=ESLimitedLowOrderPos(low_order_surface_flux=(LaxFriedrichsOnProjectedVal() if low_flux == "projected"
+                                                                  else LaxFriedrichsOnNodalVal())),
                  approximation_basis=B(),
                  entropyproj_limiter=EPL(),
                  rhs_limiter=LIMITERS[limiter]())
@@ -102,9 +106,10 @@ def sod_ic(param, x):
     return primitive_to_conservative(g, (rho, u, p))
 
 
-def sod_1d(K=100, N=3, T=0.2, CFL=0.5, dt0=None, basis="lobatto", limiter="subcell"):
+def sod_1d(K=100, N=3, T=0.2, CFL=0.5, dt0=None, basis="lobatto", limiter="subcell", low_flux="nodal"):
     dt0 = RECORDED_DT0["sod_1d"] if dt0 is None else dt0
-    return _param(N, K, 0.0, 1.0, T, CFL, dt0, 1, basis=basis, limiter=limiter), sod_ic, _transmissive_bc
+    return (_param(N, K, 0.0, 1.0, T, CFL, dt0, 1, basis=basis, limiter=limiter, low_flux=low_flux),
+            sod_ic, _transmissive_bc)
 
 
 # ------------------------------------------------------------------------------- vortex
@@ -153,9 +158,10 @@ def riemann4_ic(param, x, y):
     return primitive_to_conservative(param.equation.gamma, tuple(out))
 
 
-def riemann4_2d(K=256, N=3, T=0.25, CFL=0.5, dt0=None, basis="lobatto", limiter="subcell"):
+def riemann4_2d(K=256, N=3, T=0.25, CFL=0.5, dt0=None, basis="lobatto", limiter="subcell", low_flux="nodal"):
     dt0 = RECORDED_DT0["riemann4_2d"] if dt0 is None else dt0
-    return (_param(N, (K, K), (0.0, 0.0), (1.0, 1.0), T, CFL, dt0, 2, basis=basis, limiter=limiter),
+    return (_param(N, (K, K), (0.0, 0.0), (1.0, 1.0), T, CFL, dt0, 2, basis=basis, limiter=limiter,
+                   low_flux=low_flux),
             riemann4_ic, _transmissive_bc)
 
 
@@ -180,6 +186,68 @@ def kelvin_helmholtz_gauss(K=80, N=3, T=10.0, CFL=0.5, dt0=1e-3, basis="gauss"):
     return _param(N, (K, K), (-1.0, -1.0), (1.0, 1.0), T, CFL, dt0, 2, basis=basis), kh_ic, _periodic_bc
 
 
+# ------------------------------------------------------------------------------- astrophysical jet
+def astro_jet_ic(param, x, y):
+    g = param.equation.gamma
+    return primitive_to_conservative(g, (np.full_like(x, 0.5), np.zeros_like(x), np.zeros_like(x),
+                                         np.full_like(x, 0.4127)))
+
+
+def _astro_jet_bc(param, md):
+    """examples/2D/astro-jet.jl:20-59: periodic connectivity, then on the left boundary an inflow
+    state (the jet rho 5, u 800 where |y| < 0.05, rest state elsewhere) and outflow on the right."""
+    md = make_periodic(md)
+    g = param.equation.gamma
+    ZEROTOL = param.global_constants.ZEROTOL
+    Kx, Ky = int(param.K[0]), int(param.K[1])
+    N1D = param.N + 1
+    Nfp = 4 * N1D
+    K = Kx * Ky
+    VX = -1.0 + 2.0 * np.arange(Kx + 1) / Kx
+    VX[0], VX[-1] = -1.0, 1.0
+    VX = (VX + 1.0) / 2.0 * (param.xR[0] - param.xL[0]) + param.xL[0]
+    VY = -1.0 + 2.0 * np.arange(Ky + 1) / Ky
+    VY[0], VY[-1] = -1.0, 1.0
+    VY = (VY + 1.0) / 2.0 * (param.xR[1] - param.xL[1]) + param.xL[1]
+    r1D = md_face_points(param)
+    jet = primitive_to_conservative(g, (5.0, 800.0, 0.0, 0.4127))
+    rest = primitive_to_conservative(g, (0.5, 0.0, 0.0, 0.4127))
+    mapI, mapO, Ival = [], [], []
+    for k in range(K):
+        ex, ey = k % Kx, k // Kx
+        for i in range(Nfp):
+            face, m = divmod(i, N1D)
+            rf, sf = [(-1.0, r1D[m]), (1.0, r1D[m]), (r1D[m], -1.0), (r1D[m], 1.0)][face]
+            xf = 0.5 * (1.0 - rf) * VX[ex] + 0.5 * (1.0 + rf) * VX[ex + 1]
+            yf = 0.5 * (1.0 - sf) * VY[ey] + 0.5 * (1.0 + sf) * VY[ey + 1]
+            idx = i + Nfp * k + 1
+            if abs(xf - param.xL[0]) < ZEROTOL:
+                mapI.append(idx)
+                Ival.append(jet if abs(yf) < 0.05 + ZEROTOL else rest)
+            elif abs(xf - param.xR[0]) < ZEROTOL:
+                mapO.append(idx)
+    return BCData(mapP=md.mapP, mapI=np.array(mapI, dtype=np.int64), mapO=np.array(mapO, dtype=np.int64),
+                  Ival=np.array(Ival, dtype=np.float64).reshape(-1, 4)), md
+
+
+def md_face_points(param):
+    """1D face quadrature points of the basis (LGL or Gauss nodes)."""
+    from .operators import gauss_nodes_weights, lgl_nodes_weights
+    if isinstance(param.approximation_basis, GaussCollocation):
+        return gauss_nodes_weights(param.N)[0]
+    return lgl_nodes_weights(param.N)[0]
+
+
+def astro_jet(K=100, N=3, T=1e-3, CFL=0.5, dt0=1e-6, basis="gauss", limiter="relaxed_minentropy"):
+    """The reference's astrophysical jet (examples/2D/astro-jet.jl:61-75): [-0.5,0.5]^2, 100x100,
+    N=3, gamma 5/3, zeta 0.5, eta 0.1, Gauss + NodewiseScaledExtrapolation,
+    SubcellLimiter(PositivityAndRelaxedMinEntropyBound), inflow left, outflow right."""
+    param = _param(N, (K, K), (-0.5, -0.5), (0.5, 0.5), T, CFL, dt0, 2, zeta=0.5, eta=0.1, basis=basis,
+                   limiter=limiter)
+    param = replace(param, equation=CompressibleEulerIdealGas(Dim2(), 5.0 / 3.0))
+    return param, astro_jet_ic, _astro_jet_bc
+
+
 # ------------------------------------------------------------------------------- reference smoke
 def smoke_exact_prim(x, y, t, gamma=1.4):
     """test/test_smoke.jl:6-20."""
@@ -197,10 +265,10 @@ def smoke_ic(param, x, y):
     return primitive_to_conservative(param.equation.gamma, smoke_exact_prim(x, y, param.timestepping_param.t0))
 
 
-def smoke_vortex(N=3, K=5, T=2e-2, CFL=1.0, dt0=1e-2, basis="lobatto", limiter="subcell"):
+def smoke_vortex(N=3, K=5, T=2e-2, CFL=1.0, dt0=1e-2, basis="lobatto", limiter="subcell", low_flux="nodal"):
     """test/test_smoke.jl:54-66,117-129; ``limiter`` picks one of its limiter set-ups."""
     return (_param(N, (K, K), (0.0, 0.0), (10.0, 10.0), T, CFL, dt0, 2, zeta=0.1, eta=0.5, basis=basis,
-                   limiter=limiter),
+                   limiter=limiter, low_flux=low_flux),
             smoke_ic, _periodic_bc)
 
 
@@ -293,6 +361,7 @@ CONFIGS: Dict[str, Callable] = {
     "riemann4_2d": riemann4_2d,
     "kelvin_helmholtz": kelvin_helmholtz,
     "kelvin_helmholtz_gauss": kelvin_helmholtz_gauss,
+    "astro_jet": astro_jet,
     "smoke_vortex": smoke_vortex,
     "periodic_wave": periodic_wave,
 }

@@ -365,6 +365,7 @@ PhaseAArgs argsA(p2de_ctx* c, const double* U, int stage1) {
     a.POSTOL = c->cfg.POSTOL;
     a.blend = c->cfg.shockcapture == P2DE_HENNEMANN_SHOCK_CAPTURE ? c->d_blend : nullptr;
     a.lphi = c->cfg.bound != P2DE_POSITIVITY_BOUND ? c->d_lphi : nullptr;
+    a.lo_proj = c->cfg.low_order_surface_flux == P2DE_LF_ON_PROJECTED_VAL;
     if (c->debug) {
         a.dbg_rhsH = c->d_dbg_rhsH;
         a.dbg_rhsL = c->d_dbg_rhsL;

@@ -130,6 +130,7 @@ struct PhaseAArgs {
     double POSTOL;
     const double* blend;          // per element blending factor of the shock capture (nullptr: 1)
     const double* lphi;           // per node lower bound of the modified specific entropy
+    int lo_proj;                  // low-order surface flux LaxFriedrichsOnProjectedVal (else OnNodalVal)
 };
 
 // element-kernel forms: 0 SubcellLimiter(PositivityBound, NoShockCapture); 1 SubcellLimiter with

@@ -29,7 +29,8 @@ int64_t ipow(int a, int b) {
 
 int setup_config(const p2de_config* cfg, int* Nq, int* Nfp, int* Nh, int* K, std::string& err) {
     // option set (Solver.jl:1-121); anything else is not implemented here
-    if (cfg->rhs != P2DE_RHS_LIMITED_DG || cfg->low_order_surface_flux != P2DE_LF_ON_NODAL_VAL ||
+    if (cfg->rhs != P2DE_RHS_LIMITED_DG ||
+        (cfg->low_order_surface_flux != P2DE_LF_ON_NODAL_VAL && cfg->low_order_surface_flux != P2DE_LF_ON_PROJECTED_VAL) ||
         cfg->high_order_surface_flux != P2DE_LF_ON_PROJECTED_VAL ||
         cfg->high_order_volume_flux != P2DE_CHANDRASHEKAR_FLUX)
         return fail_s(err, P2DE_E_UNSUPPORTED, "option set not implemented");
@@ -67,7 +68,7 @@ int setup_config(const p2de_config* cfg, int* Nq, int* Nfp, int* Nh, int* K, std
         return fail_s(err, P2DE_E_UNSUPPORTED, "the mixed-precision path runs the Lobatto basis");
     if (cfg->precision == P2DE_PRECISION_MIXED &&
         (cfg->rhs_limiter != P2DE_SUBCELL_LIMITER || cfg->bound != P2DE_POSITIVITY_BOUND ||
-         cfg->shockcapture != P2DE_NO_SHOCK_CAPTURE))
+         cfg->shockcapture != P2DE_NO_SHOCK_CAPTURE || cfg->low_order_surface_flux != P2DE_LF_ON_NODAL_VAL))
         return fail_s(err, P2DE_E_UNSUPPORTED,
                       "the mixed-precision path runs SubcellLimiter(PositivityBound, NoShockCapture)");
     const int N1D = cfg->N + 1;

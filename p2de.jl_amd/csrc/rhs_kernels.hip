@@ -167,6 +167,30 @@ __device__ __noinline__ double limiting_param_phi(const double* uL, const double
     return x_valid;
 }
 
+// find_alpha (low_order_graph_viscosity.jl:299-328): the smallest alpha, by doubling and then 50
+// bisection steps, with alpha*u_i - u~ of positive density and internal energy.  The reference's
+// doubling has no exit when no alpha works; here it ends when alpha overflows (alpha = Inf).
+template <int DIM>
+__device__ __noinline__ double find_alpha(const double* ui, const double* ut, double POSTOL) {
+    constexpr int NC = DIM + 2;
+    auto ok = [&](double al) {
+        double sub[NC];
+        for (int c = 0; c < NC; c++) sub[c] = al * ui[c] - ut[c];
+        return sub[0] > POSTOL && Eq<DIM>::rhoe(sub) > POSTOL;
+    };
+    double alphaL = 0.0, alphaR = 1.0;
+    while (!ok(alphaR)) {
+        alphaR = 2.0 * alphaR;
+        if (isinf(alphaR)) return alphaR;
+    }
+    for (int it = 0; it < 50; it++) {
+        const double alphaM = (alphaL + alphaR) / 2.0;
+        if (ok(alphaM)) alphaR = alphaM;
+        else alphaL = alphaM;
+    }
+    return alphaR;
+}
+
 // LDS of one workgroup.  Arrays whose lifetimes do not overlap share storage:
 // node/face primitives and fluxes (phases 1-3) with the directional RHS and u_L (phases 4-5),
 // pair results (phases 3-4) with the subcell flux potentials and parameters (phase 5).
@@ -324,18 +348,25 @@ __global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
             sm.BFH[e][f][c] = BF - lf;
         }
 
-        // low-order surface term (low_order_graph_viscosity.jl:51-61, 85-91, 184-198)
-        const double wsf = Eq<DIM>::wavespeed(g, Uq, n);
-        const double wsP = Eq<DIM>::wavespeed(g, Ut, nP);
+        // low-order surface term (low_order_graph_viscosity.jl:51-71, 85-91, 94-124, 184-198): the face
+        // states are the nodal values (LaxFriedrichsOnNodalVal) or the projected ones
+        // (LaxFriedrichsOnProjectedVal); outflow takes the node's state, inflow Ival
+        const bool lproj = a.lo_proj != 0;
+        const double* Uf = lproj ? ut : Uq;
+        if (lproj && bcc == 0)
+            for (int c = 0; c < NC; c++) uPL[c] = utP[c];
+        const double wsf = Eq<DIM>::wavespeed(g, Uf, n);
+        const double wsP = Eq<DIM>::wavespeed(g, lproj ? utP : Ut, nP);
         const double lB = 0.5 * nrm * jmax(wsf, wsP);
         Eq<DIM>::flux(g, uPL, FP);
         for (int c = 0; c < NC; c++) {
-            double fs = 0.5 * (sm.a.g1.nF[e][q][ld * NC + c] + FP[ld * NC + c]);
+            double fs = 0.5 * ((lproj ? Ff[ld * NC + c] : sm.a.g1.nF[e][q][ld * NC + c]) + FP[ld * NC + c]);
             double BF = B[ld] * fs;
-            double lf = lB * (uPL[c] - Uq[c]);
+            double lf = lB * (uPL[c] - Uf[c]);
             sm.BFL[e][f][c] = BF - lf;
         }
-        sm.lamB[e][f] = lB;
+        // the face term of the CFL sum (lambda_B_CFL :283-291)
+        sm.lamB[e][f] = (lproj && a.stage1) ? find_alpha<DIM>(Uq, ut, a.POSTOL) * lB + 0.5 * nrm * wsf : lB;
     }
     __syncthreads();
 

@@ -18,7 +18,8 @@ import numpy as np
 
 from .operators import (BCData, MeshData, Operators, RefElem, SizeData, build_operators,
                         gauss_ref_elem, lobatto_ref_elem, uniform_mesh)
-from .types import GaussCollocation, NodewiseScaledExtrapolation, Param, check_supported, limiter_codes
+from .types import (GaussCollocation, LaxFriedrichsOnProjectedVal, NodewiseScaledExtrapolation, Param,
+                    check_supported, limiter_codes)
 
 
 @dataclass
@@ -193,6 +194,7 @@ class FlatProblem:
     shock_a: float = 0.5
     shock_c: float = 1.8
     VDM_inv: Optional[np.ndarray] = None
+    low_order_surface_flux: int = 0   # 0 LaxFriedrichsOnNodalVal, 1 LaxFriedrichsOnProjectedVal
 
 
 def flat_problem(solver: Solver, state_param: StateParam, upper_bound_mode: int = 0) -> FlatProblem:
@@ -233,4 +235,5 @@ def flat_problem(solver: Solver, state_param: StateParam, upper_bound_mode: int 
         basis=1 if isinstance(p.approximation_basis, GaussCollocation) else 0,
         entropyproj_limiter=1 if isinstance(p.entropyproj_limiter, NodewiseScaledExtrapolation) else 0,
         rhs_limiter=lc[0], bound=lc[1], shockcapture=lc[2], shock_a=lc[3], shock_c=lc[4],
-        VDM_inv=_colmajor(ops.VDM_inv))
+        VDM_inv=_colmajor(ops.VDM_inv),
+        low_order_surface_flux=1 if isinstance(p.rhs.low_order_surface_flux, LaxFriedrichsOnProjectedVal) else 0)

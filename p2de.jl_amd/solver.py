@@ -178,6 +178,7 @@ def _descriptors(fp: FlatProblem, upper_bound_mode: int = None, precision: int =
     cfg = _Config(Nd=fp.Nd, N=fp.N1D - 1, Kx=fp.Kx, Ky=fp.Ky, gamma=fp.gamma, zeta=fp.zeta, eta=fp.eta,
                   POSTOL=fp.POSTOL, ZEROTOL=fp.ZEROTOL, CFL=fp.CFL, dt0=fp.dt0, t0=fp.t0, T=fp.T,
                   high_order_surface_flux=1, upper_bound_mode=int(mode), precision=int(prec),
+                  low_order_surface_flux=int(getattr(fp, "low_order_surface_flux", 0)),
                   basis=int(getattr(fp, "basis", 0)), entropyproj_limiter=int(getattr(fp, "entropyproj_limiter", 0)),
                   rhs_limiter=int(getattr(fp, "rhs_limiter", 0)), bound=int(getattr(fp, "bound", 0)),
                   shockcapture=int(getattr(fp, "shockcapture", 0)), shock_a=float(getattr(fp, "shock_a", 0.5)),

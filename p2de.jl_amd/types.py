@@ -243,8 +243,8 @@ def check_supported(param: Param) -> None:
         raise UnsupportedOption(f"rhs {rhs!r}: only LimitedDG (ESLimitedLowOrderPos) is implemented")
     if not isinstance(rhs.high_order_volume_flux, ChandrashekarFlux):
         raise UnsupportedOption("high_order_volume_flux must be ChandrashekarFlux")
-    if not isinstance(rhs.low_order_surface_flux, LaxFriedrichsOnNodalVal):
-        raise UnsupportedOption("low_order_surface_flux must be LaxFriedrichsOnNodalVal")
+    if not isinstance(rhs.low_order_surface_flux, (LaxFriedrichsOnNodalVal, LaxFriedrichsOnProjectedVal)):
+        raise UnsupportedOption("low_order_surface_flux: LaxFriedrichsOnNodalVal or LaxFriedrichsOnProjectedVal")
     if not isinstance(rhs.high_order_surface_flux, LaxFriedrichsOnProjectedVal):
         raise UnsupportedOption("high_order_surface_flux must be LaxFriedrichsOnProjectedVal")
     basis = param.approximation_basis

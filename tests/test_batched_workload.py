@@ -46,3 +46,16 @@ def test_dip_initial_condition_positive_and_near_vacuum():
     y = np.array([[0.11]])
     c = ic(param, x, y)
     assert math.isclose(c[0, 0, 0], 1.0 + 0.5 * math.sin(2 * math.pi * 4 * 0.87) * math.sin(2 * math.pi * 3 * 0.11), rel_tol=1e-12)
+
+
+def test_astro_jet_boundary_data():
+    """examples/2D/astro-jet.jl:20-59: every left face node is inflow (the jet state where
+    |y| < 0.05, the rest state elsewhere), every right face node outflow, the rest periodic."""
+    s, st, sp = PKG.configs.build("astro_jet", K=20)
+    bc = sp.bcdata
+    N1D = s.param.N + 1
+    assert len(bc.mapI) == len(bc.mapO) == 20 * N1D
+    Ival = np.asarray(bc.Ival)
+    jet = Ival[:, 0] == 5.0
+    assert jet.sum() >= 1 and np.all(Ival[~jet, 0] == 0.5)
+    assert np.allclose(Ival[jet, 1] / Ival[jet, 0], 800.0)

@@ -40,7 +40,7 @@ def _create(fp, **cfg_over):
                     POSTOL=fp.POSTOL, ZEROTOL=fp.ZEROTOL, CFL=fp.CFL, dt0=fp.dt0, t0=fp.t0, T=fp.T,
                     high_order_surface_flux=1, basis=fp.basis, entropyproj_limiter=fp.entropyproj_limiter,
                     rhs_limiter=fp.rhs_limiter, bound=fp.bound, shockcapture=fp.shockcapture,
-                    shock_a=fp.shock_a, shock_c=fp.shock_c)
+                    shock_a=fp.shock_a, shock_c=fp.shock_c, low_order_surface_flux=fp.low_order_surface_flux)
     for k, v in cfg_over.items():
         setattr(cfg, k, v)
     keep = {n: None if getattr(fp, n) is None else np.ascontiguousarray(getattr(fp, n), dtype=np.float64)

@@ -222,3 +222,51 @@ def test_smoke_setups_conserve_and_stay_positive(limiter, N):
     ex = np.transpose(C.primitive_to_conservative(1.4, C.smoke_exact_prim(x, y, t)), (2, 1, 0))
     err = np.sum(wJ * np.abs(U[..., 0] - ex[..., 0])) / np.sum(wJ * np.abs(ex[..., 0]))
     assert err < 5e-3
+
+
+def test_find_alpha_is_the_smallest_admissible_scaling():
+    """find_alpha (low_order_graph_viscosity.jl:299-328): alpha*u - u~ keeps rho and rhoe above
+    POSTOL at the returned alpha and not below it; the exact threshold is the larger of the
+    density root and the larger root of the rhoe quadratic, found here in closed form."""
+    from oracle.oracle import pt_find_alpha
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        rho, u, v, p = rng.uniform(0.2, 2), rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(0.2, 2)
+        ui = np.array([rho, rho * u, rho * v, p / 0.4 + 0.5 * rho * (u * u + v * v)])
+        ut = ui * rng.uniform(0.5, 3.0, size=4) + rng.uniform(-0.2, 0.2, size=4)
+        a = pt_find_alpha(2, 1e-14, ui, ut)
+        # rho(a u - ut) = a u0 - ut0; rhoe(a u - ut) * rho = quadratic q(a) = A a^2 + B a + C
+        A = ui[0] * ui[3] - 0.5 * (ui[1] ** 2 + ui[2] ** 2)
+        B = -(ui[0] * ut[3] + ut[0] * ui[3] - ui[1] * ut[1] - ui[2] * ut[2])
+        Cq = ut[0] * ut[3] - 0.5 * (ut[1] ** 2 + ut[2] ** 2)
+        a_rho = ut[0] / ui[0]
+        roots = np.roots([A, B, Cq])
+        roots = roots[np.isreal(roots)].real
+        a_e = roots.max() if roots.size else -np.inf
+        exact = max(a_rho, a_e, 0.0)
+        assert a >= exact * (1 - 1e-12) and a <= exact * (1 + 1e-8) + 1e-12
+
+
+def test_low_order_flux_on_projected_values():
+    """LaxFriedrichsOnProjectedVal as the low-order surface flux (low_order_graph_viscosity.jl:63-71,
+    249-291): on Lobatto the projected face state u(v(u)) is the nodal state to rounding, so
+    rhsL agrees with the nodal flux's to rounding; on Gauss the set-up of the reference's
+    convergence examples (Zhang-Shu, projected low-order flux) conserves and stays positive."""
+    s, U, fpn = _setup(N=3)
+    _, _, fpp = _setup(N=3, low_flux="projected")
+    on, op = Oracle(fpn), Oracle(fpp)
+    on.rhs(U, 0.0, 1e-3, 2)
+    op.rhs(U, 0.0, 1e-3, 2)
+    rn, rp = on.field("rhsL"), op.field("rhsL")
+    assert np.max(np.abs(rn - rp)) <= 1e-12 * np.max(np.abs(rn))
+    # Gauss, Zhang-Shu, projected low-order flux
+    s, U, fp = _setup(N=3, basis="gauss", limiter="zhangshu", low_flux="projected")
+    wJ = fp.wq[None, :] * fp.Jq.reshape(fp.K, fp.Nq)
+    m0 = np.einsum("kq,kqc->c", wJ, U)
+    a0 = np.einsum("kq,kqc->c", wJ, np.abs(U))
+    orc = Oracle(fp)
+    t = 0.0
+    while t < fp.T - 1e-14:
+        t, _ = orc.step(U, t)
+    m1 = np.einsum("kq,kqc->c", wJ, U)
+    assert np.max(np.abs(m1 - m0) / a0) < 1e-13 and orc.count_nonpositive(U)[0] == 0
