@@ -129,14 +129,22 @@ typedef struct {
     int64_t violations;      /* positivity violations seen (every stage) */
 } p2de_run_stats;
 
-/* Fields readable with p2de_get_field after p2de_rhs with debug output on. */
+/* Fields readable with p2de_get_field.  RHSH, RHSL and L_LOCAL need debug output on
+ * (p2de_set_debug); the others are always kept. */
 enum {
     P2DE_FIELD_RHSU = 0,     /* (Nq, K) x Nc */
     P2DE_FIELD_RHSH = 1,     /* (Nq, K) x Nc */
     P2DE_FIELD_RHSL = 2,     /* (Nq, K) x Nc */
     P2DE_FIELD_L_LOCAL = 3,  /* (Nq+N1D, Nd, K) for the stage of the last call; Zhang-Shu:
                                 the first K values are L[:, nstage] */
-    P2DE_FIELD_THETA = 4     /* (Nfp, K) theta_local[:, :, nstage] (NodewiseScaledExtrapolation) */
+    P2DE_FIELD_THETA = 4,    /* (Nfp, K) theta_local[:, :, nstage] of the last stage
+                                (NodewiseScaledExtrapolation) */
+    P2DE_FIELD_L = 5,        /* (K, 3) the reference's L (State.jl:17): the Zhang-Shu element
+                                parameters of each stage (zhangshu.jl:44), zeros with the
+                                subcell limiter, which does not write it */
+    P2DE_FIELD_THETA_MEAN = 6/* (K, 3) the reference's theta (State.jl:19): the mean of
+                                theta_local over the element's face nodes per stage (filter.jl:58),
+                                summed in node order; zeros without NodewiseScaledExtrapolation */
 };
 
 /* Replaces initialize_DG's products as rhs!/SSP33! consume them (init.jl:44-60):
@@ -175,6 +183,37 @@ int p2de_run(p2de_ctx *ctx, int max_steps, p2de_run_stats *stats);
 /* Device time control for p2de_run/p2de_ssprk3_step. */
 int p2de_set_time(p2de_ctx *ctx, double t);
 int p2de_get_time(p2de_ctx *ctx, double *t, double *last_dt);
+
+/* One segment of SSP33!'s time loop with its history output (SSPRK33.jl:28-56): steps from
+ * the device time, writing each step's dt to dthist, until after a step the reference would
+ * record a history entry -- mod(i, output_interval) == 0 or |t - T| < 1e-10, where i is the
+ * reference's step index (1 at t0, incremented by every step; *step_index in and out) -- or
+ * t >= T, or max_steps steps (<= 0: no limit) or dthist_cap steps were taken.  *steps_taken
+ * receives the count and *at_output 1 when the segment ended at a history entry; the caller
+ * then reads the state (Uhist), p2de_get_time (thist), P2DE_FIELD_L (Lhist),
+ * P2DE_FIELD_THETA_MEAN (thetahist) and p2de_check_conservation.  Launches stay queued
+ * between history entries (no host round trip per step in FP64).  With precision MIXED
+ * the steps are taken one per host round trip. */
+int p2de_run_segment(p2de_ctx *ctx, int output_interval, int max_steps, int64_t *step_index,
+                     double *dthist, int dthist_cap, int *steps_taken, int *at_output);
+
+/* check_conservation (src/dg/utils.jl:1-12): sum over the components of
+ * sum_k sum_i Jq[i,k] wq[i] Uq[i,k] for the device state, accumulated on the host in the
+ * reference's order. */
+int p2de_check_conservation(p2de_ctx *ctx, double *total);
+
+/* Checkpoints.  The reference keeps its history in memory and saves it with JLD2 after the
+ * run (postprocess.jl:131-142); it has no restart.  These three calls have no counterpart in
+ * it: they snapshot a context (state, device time, last dt, step counter, the t0 entropy
+ * minimum of the min-entropy bounds, and the caller's step index) as one self-describing
+ * byte block -- a header with the problem's sizes, the state double[K][Nq][Nc], a 64-bit
+ * FNV-1a checksum of the state -- that the caller stores where it likes.  Loading it into a
+ * context of the same problem (Nd, N, basis, K) and stepping on gives the trajectory of the
+ * run that was not stopped, bit for bit.  Load rejects a block of another problem, a
+ * truncated block and a checksum mismatch with P2DE_E_INVALID and changes nothing then. */
+int64_t p2de_checkpoint_size(const p2de_ctx *ctx);
+int p2de_checkpoint_save(p2de_ctx *ctx, int64_t step_index, void *buf, int64_t size);
+int p2de_checkpoint_load(p2de_ctx *ctx, const void *buf, int64_t size, int64_t *step_index);
 
 /* Positivity of the device state plus the count of violations seen in every stage
  * since the last call; then resets the count. */

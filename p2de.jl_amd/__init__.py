@@ -10,7 +10,9 @@ Host-side mirror of the reference interface:
   C-ABI library ``libp2de_hip.so`` (``include/p2de_hip.h``);
 * ``BatchSolver`` (``solver``), many independent 2D solves advanced together in mixed
   precision (FP32 bulk, FP64 limiter and RK stages), and the batched workload
-  (``configs.batched_instances``).
+  (``configs.batched_instances``);
+* ``postprocess``: ``calculate_error`` (src/dg/postprocess.jl:1-46), ``check_conservation``,
+  the error table and history files, VTK output and checkpoint files.
 
 Import it with ``__graft_entry__.load_package()``: the directory name is not a Python
 identifier, so it is registered as ``p2de_jl_amd``.
@@ -20,4 +22,6 @@ from .types import *  # noqa: F401,F403
 from .dg import (initialize_DG, make_periodic, primitive_to_conservative, flat_problem,  # noqa: F401
                  Solver, State, StateParam, TimeParam)
 from . import solver  # noqa: F401,E402
-from .solver import BatchSolver, DeviceSolver, P2DEError, SSP33, rhs  # noqa: F401,E402
+from .solver import BatchSolver, DataHistory, DeviceSolver, P2DEError, SSP33, rhs  # noqa: F401,E402
+from . import postprocess  # noqa: F401,E402
+from .postprocess import ErrorData, calculate_error, check_conservation  # noqa: F401,E402

@@ -112,6 +112,41 @@ def sod_1d(K=100, N=3, T=0.2, CFL=0.5, dt0=None, basis="lobatto", limiter="subce
             sod_ic, _transmissive_bc)
 
 
+# ------------------------------------------------------------------------------- Shu-Osher
+def shu_osher_prim(x):
+    """examples/1D/shu-osher.jl:8-14: the Mach 3 shock at x = -4 entering a density wave."""
+    left = x < -4.0
+    rho = np.where(left, 3.857143, 1 + 0.2 * np.sin(5 * x))
+    u = np.where(left, 2.629369, 0.0)
+    p = np.where(left, 10.3333, 1.0)
+    return rho, u, p
+
+
+def shu_osher_exact(equation, x, t):
+    """The example's ``exact_sol`` (the initial state, as there: not an exact solution)."""
+    return shu_osher_prim(x)
+
+
+def shu_osher_ic(param, x):
+    return primitive_to_conservative(param.equation.gamma, shu_osher_prim(x))
+
+
+def _shu_osher_bc(param, md):
+    """examples/1D/shu-osher.jl:16-31: inflow of the post-shock state at the left end
+    (mapI = [1]), outflow at the right end (mapO = [2K])."""
+    K = int(param.K)
+    Ival = primitive_to_conservative(param.equation.gamma, (3.857143, 2.629369, 10.3333))
+    return BCData(mapP=md.mapP, mapI=np.array([1], dtype=np.int64), mapO=np.array([2 * K], dtype=np.int64),
+                  Ival=np.asarray(Ival, dtype=np.float64).reshape(1, 3))
+
+
+def shu_osher(K=64, N=3, T=1.8, CFL=0.5, dt0=1e-4, basis="gauss", limiter="subcell"):
+    """examples/1D/shu-osher.jl:43-55: [-5, 5], K = 64, N = 3, zeta 0.1, eta 1, Gauss +
+    NodewiseScaledExtrapolation, SubcellLimiter(PositivityBound)."""
+    return (_param(N, K, -5.0, 5.0, T, CFL, dt0, 1, zeta=0.1, eta=1.0, basis=basis, limiter=limiter),
+            shu_osher_ic, _shu_osher_bc)
+
+
 # ------------------------------------------------------------------------------- vortex
 def vortex_exact_prim(x, y, t, gamma=1.4, beta=5.0, L=10.0, xc=0.0, yc=0.0):
     """Isentropic vortex advected with (1, 1) on a periodic box of side L (nearest image)."""
@@ -126,6 +161,11 @@ def vortex_exact_prim(x, y, t, gamma=1.4, beta=5.0, L=10.0, xc=0.0, yc=0.0):
     rho = (1.0 - (gamma - 1.0) * beta * beta * np.exp(2.0 * (1.0 - r2)) / (16.0 * gamma * math.pi ** 2)) ** (1.0 / (gamma - 1.0))
     p = rho ** gamma
     return rho, u, v, p
+
+
+def vortex_exact(equation, x, y, t):
+    """``exact_sol(equation, x, y, t)`` form, as the reference's examples pass it to calculate_error."""
+    return vortex_exact_prim(x, y, t, equation.gamma)
 
 
 def vortex_ic(param, x, y):
@@ -261,6 +301,11 @@ def smoke_exact_prim(x, y, t, gamma=1.4):
     return rho, u, v, p
 
 
+def smoke_exact(equation, x, y, t):
+    """``exact_sol(eqn, x, y, t)`` of test/test_smoke.jl:6-20."""
+    return smoke_exact_prim(x, y, t, equation.gamma)
+
+
 def smoke_ic(param, x, y):
     return primitive_to_conservative(param.equation.gamma, smoke_exact_prim(x, y, param.timestepping_param.t0))
 
@@ -363,6 +408,7 @@ CONFIGS: Dict[str, Callable] = {
     "kelvin_helmholtz_gauss": kelvin_helmholtz_gauss,
     "astro_jet": astro_jet,
     "smoke_vortex": smoke_vortex,
+    "shu_osher": shu_osher,
     "periodic_wave": periodic_wave,
 }
 

@@ -478,6 +478,21 @@ int batch_instance_stats(p2de_batch* b, int inst, double* t, double* last_dt, in
     return P2DE_OK;
 }
 
+// Device time, last step and step counter of one instance as a checkpoint recorded them.
+int batch_instance_restore(p2de_batch* b, int inst, double t, double last_dt, int steps) {
+    if (!b) return bfail(b, P2DE_E_INVALID, "NULL batch");
+    if (inst < 0 || inst >= b->n) return bfail(b, P2DE_E_INVALID, "instance out of range");
+    int st = read_sc(b);
+    if (st) return st;
+    StepScalars& S = b->h_sc[inst];
+    S.t = t;
+    S.done = 0;
+    S.hold = 0;
+    S.steps = steps;
+    std::memcpy(&S.dt_bits, &last_dt, sizeof(double));
+    return write_sc(b);
+}
+
 extern "C" {
 
 int p2de_batch_create(int n, const p2de_config* cfg, const p2de_operators* ops, const p2de_geometry* geom,

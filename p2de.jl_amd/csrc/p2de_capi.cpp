@@ -45,7 +45,9 @@ struct p2de_ctx {
     double* d_U[2] = {nullptr, nullptr};
     double *d_rhsUA = nullptr, *d_lA = nullptr, *d_dfA = nullptr, *d_rhsU = nullptr;
     double* d_ut = nullptr;                // projected face states of d_U[0] (Lobatto) or of the stage input (Gauss)
-    double* d_theta = nullptr;             // entropy-projection parameters per face node (Gauss)
+    double* d_theta = nullptr;             // entropy-projection parameters per face node and stage (Gauss), [3][K][Nfp]
+    double* d_Lz = nullptr;                // Zhang-Shu element parameters per stage, [3][K] (the reference's L)
+    int last_stage = 0;                    // stage of the last element-kernel launch
     int basis = 0;                         // P2DE_BASIS_*
     bool ep_limit = false;                 // NodewiseScaledExtrapolation
     int optA = OPT_SUBCELL;                // element-kernel form for the limiter options
@@ -68,6 +70,12 @@ struct p2de_ctx {
     int64_t time_count[3] = {0, 0, 0};
     std::string err;
     p2de_batch* mix = nullptr;             // precision MIXED: a one-instance batch does the work
+    std::vector<double> h_Jq, h_wq;        // check_conservation's weights (Nq x K, Nq)
+    // history segment of p2de_run_segment while it runs (out_interval 0: none)
+    double* d_dtlog = nullptr;
+    int dtlog_cap = 0;
+    int seg_interval = 0, seg_cap = 0, seg_base = 0;
+    long long seg_i0 = 0;
 };
 
 namespace {
@@ -121,7 +129,7 @@ void free_ctx(p2de_ctx* c) {
     void* dptrs[] = {c->d_tab, c->d_geo, c->d_nb_node, c->d_nb_face, c->d_bc, c->d_Ival, c->d_U[0], c->d_U[1],
                      c->d_rhsUA, c->d_lA, c->d_dfA, c->d_rhsU, c->d_ut, c->d_theta, c->d_dtp, c->d_dbg_rhsH,
                      c->d_dbg_rhsL, c->d_dbg_L, c->d_blend, c->d_lphi, c->d_spart,
-                     c->d_sc, c->d_keys};
+                     c->d_sc, c->d_keys, c->d_Lz, c->d_dtlog};
     for (void* p : dptrs)
         if (p) (void)hipFree(p);
     if (c->h_sc) (void)hipHostFree(c->h_sc);
@@ -206,6 +214,9 @@ int p2de_create(const p2de_config* cfg, const p2de_operators* ops, const p2de_ge
         c->cfg = *cfg;
         int st = setup_config(cfg, &c->Nq, &c->Nfp, &c->Nh, &c->K, c->err);
         if (st) return bail(st);
+        if (!ops->wq || !geom->Jq) return fail(c, P2DE_E_INVALID, "NULL wq or Jq"), bail(P2DE_E_INVALID);
+        c->h_wq.assign(ops->wq, ops->wq + c->Nq);
+        c->h_Jq.assign(geom->Jq, geom->Jq + (size_t)c->Nq * c->K);
         if (cfg->precision == P2DE_PRECISION_MIXED) {
             c->Nd = cfg->Nd;
             c->Nc = cfg->Nd + 2;
@@ -254,11 +265,12 @@ int p2de_create(const p2de_config* cfg, const p2de_operators* ops, const p2de_ge
             (st = dalloc(c, &c->d_U[0], nU)) || (st = dalloc(c, &c->d_U[1], nU)) || (st = dalloc(c, &c->d_rhsUA, nU)) ||
             (st = dalloc(c, &c->d_lA, (size_t)K * Nfp)) || (st = dalloc(c, &c->d_dfA, (size_t)K * Nfp * Nc)) ||
             (st = dalloc(c, &c->d_rhsU, nU)) || (st = dalloc(c, &c->d_ut, (size_t)K * Nfp * Nc)) ||
-            (st = dalloc(c, &c->d_theta, (size_t)K * Nfp)) || (st = dalloc(c, &c->d_blend, (size_t)K)) ||
+            (st = dalloc(c, &c->d_theta, (size_t)3 * K * Nfp)) || (st = dalloc(c, &c->d_blend, (size_t)K)) ||
             (st = dalloc(c, &c->d_lphi, (size_t)K * Nq)) || (st = dalloc(c, &c->d_spart, ((size_t)K * Nq + 255) / 256)) ||
             (st = dalloc(c, &c->d_dtp, (size_t)K)) ||
             (st = dalloc(c, &c->d_sc, 1)) || (st = dalloc(c, &c->d_keys, 3)))
             return bail(st);
+        if (c->optA == OPT_ZHANG_SHU && (st = dalloc(c, &c->d_Lz, (size_t)3 * K))) return bail(st);
         if (hipHostMalloc((void**)&c->h_sc, sizeof(StepScalars)) != hipSuccess ||
             hipHostMalloc((void**)&c->h_keys, 3 * sizeof(unsigned long long)) != hipSuccess)
             return fail(c, P2DE_E_ALLOC, "hipHostMalloc failed"), bail(P2DE_E_ALLOC);
@@ -271,7 +283,9 @@ int p2de_create(const p2de_config* cfg, const p2de_operators* ops, const p2de_ge
             (e = hipMemcpy(c->d_nb_node, nb_node.data(), sizeof(int) * nb_node.size(), hipMemcpyHostToDevice)) != hipSuccess ||
             (e = hipMemcpy(c->d_nb_face, nb_face.data(), sizeof(int) * nb_face.size(), hipMemcpyHostToDevice)) != hipSuccess ||
             (e = hipMemcpy(c->d_bc, bcode.data(), sizeof(int) * bcode.size(), hipMemcpyHostToDevice)) != hipSuccess ||
-            (e = hipMemcpy(c->d_sc, c->h_sc, sizeof(StepScalars), hipMemcpyHostToDevice)) != hipSuccess)
+            (e = hipMemcpy(c->d_sc, c->h_sc, sizeof(StepScalars), hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemset(c->d_theta, 0, sizeof(double) * 3 * K * Nfp)) != hipSuccess ||
+            (c->d_Lz && (e = hipMemset(c->d_Lz, 0, sizeof(double) * 3 * K)) != hipSuccess))
             return fail(c, P2DE_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e)), bail(P2DE_E_HIP);
         if (bc->nI > 0 &&
             (e = hipMemcpy(c->d_Ival, bc->Ival, sizeof(double) * bc->nI * Nc, hipMemcpyHostToDevice)) != hipSuccess)
@@ -338,11 +352,15 @@ int p2de_set_debug(p2de_ctx* ctx, int on) {
 
 namespace {
 
-PhaseAArgs argsA(p2de_ctx* c, const double* U, int stage1) {
+double* theta_slot(p2de_ctx* c, int nstage) { return c->d_theta + (size_t)(nstage - 1) * c->K * c->Nfp; }
+
+PhaseAArgs argsA(p2de_ctx* c, const double* U, int nstage) {
+    const int stage1 = nstage == 1;
     PhaseAArgs a{};
     a.U = U;
     a.ut = c->d_ut;
-    a.theta = c->ep_limit ? c->d_theta : nullptr;
+    a.theta = c->ep_limit ? theta_slot(c, nstage) : nullptr;
+    a.zsL = c->d_Lz ? c->d_Lz + (size_t)(nstage - 1) * c->K : nullptr;
     a.T = c->d_tab;
     a.geo = c->d_geo;
     a.nb_node = c->d_nb_node;
@@ -393,6 +411,14 @@ PhaseBArgs argsB(p2de_ctx* c, int mode, const double* W, const double* Uin, doub
     b.dbg_L = (c->debug && c->optA != OPT_ZHANG_SHU) ? c->d_dbg_L : nullptr;   // Zhang-Shu: L[k] stays
     b.ut = c->basis == P2DE_BASIS_GAUSS ? nullptr : c->d_ut;
     b.finalize = finalize;
+    if (finalize && c->seg_interval > 0) {
+        b.dt_log = c->d_dtlog;
+        b.dt_cap = c->seg_cap;
+        b.steps_base = c->seg_base;
+        b.i0 = c->seg_i0;
+        b.out_interval = c->seg_interval;
+        b.T_end = c->cfg.T;
+    }
     return b;
 }
 
@@ -409,7 +435,7 @@ int ensure_face_states(p2de_ctx* c) {
 
 // Gauss: entropy projection of the stage input (init_rhs! and entropy_projection!, rhs.jl:19-23,
 // 41-44): theta and u_tilde of every face node, before phase A reads them.
-int project_faces(p2de_ctx* c, const double* U) {
+int project_faces(p2de_ctx* c, const double* U, int nstage) {
     if (c->basis != P2DE_BASIS_GAUSS) return P2DE_OK;
     EPArgs e{};
     e.U = U;
@@ -422,7 +448,7 @@ int project_faces(p2de_ctx* c, const double* U) {
     e.eta = c->cfg.eta;
     e.POSTOL = c->cfg.POSTOL;
     e.ut = c->d_ut;
-    e.theta = c->d_theta;
+    e.theta = theta_slot(c, nstage);
     LAUNCHCHK(c, timed(c, 0, [&] { return launch_entropy_projection(c->Nd, c->N1D, e, c->stream); }));
     return P2DE_OK;
 }
@@ -460,11 +486,12 @@ int limiter_prep(p2de_ctx* c, const double* U, int nstage, const double* tp, dou
 
 int stage_rhs(p2de_ctx* c, const double* U, int nstage, const double* tp, double t_value) {
     const int stage1 = nstage == 1;
-    int st = project_faces(c, U);
+    int st = project_faces(c, U, nstage);
     if (st) return st;
     if ((st = limiter_prep(c, U, nstage, tp, t_value))) return st;
+    c->last_stage = nstage;
     LAUNCHCHK(c, timed(c, 0, [&] {
-                  return launch_phaseA(c->Nd, c->N1D, c->basis, c->variantA, c->optA, argsA(c, U, stage1), c->stream);
+                  return launch_phaseA(c->Nd, c->N1D, c->basis, c->variantA, c->optA, argsA(c, U, nstage), c->stream);
               }));
     if (stage1)
         LAUNCHCHK(c, timed(c, 2, [&] {
@@ -545,6 +572,32 @@ int p2de_rhs(p2de_ctx* ctx, double t, double dt, int nstage, double* dt_out, dou
 
 int p2de_get_field(p2de_ctx* ctx, int field, double* out) {
     if (!ctx || !out) return fail(ctx, P2DE_E_INVALID, "NULL argument");
+    const size_t K = (size_t)ctx->K;
+    if (field == P2DE_FIELD_L || field == P2DE_FIELD_THETA_MEAN) {
+        // the reference writes L with the Zhang-Shu limiter only (zhangshu.jl:44) and theta with
+        // NodewiseScaledExtrapolation only (filter.jl:58); both start at zero (init.jl:20-22)
+        const bool have = field == P2DE_FIELD_L ? ctx->d_Lz != nullptr : (ctx->ep_limit && !ctx->mix);
+        if (!have) {
+            std::fill(out, out + 3 * K, 0.0);
+            return P2DE_OK;
+        }
+        if (field == P2DE_FIELD_L) {
+            HIPCHK(ctx, hipMemcpyAsync(out, ctx->d_Lz, sizeof(double) * 3 * K, hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+            return P2DE_OK;
+        }
+        const int Nfp = ctx->Nfp;
+        std::vector<double> th((size_t)3 * K * Nfp);
+        HIPCHK(ctx, hipMemcpyAsync(th.data(), ctx->d_theta, sizeof(double) * th.size(), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        for (size_t sk = 0; sk < 3 * K; sk++) {          // theta[k, nstage] = sum(theta_local[:, k, nstage]) / Nfp
+            double sum = 0.0;
+            for (int i = 0; i < Nfp; i++) sum += th[sk * Nfp + i];
+            out[sk] = sum / Nfp;
+        }
+        return P2DE_OK;
+    }
     if (ctx->mix) return fwd(ctx, p2de_batch_get_field(ctx->mix, 0, field, out));
     const size_t nU = (size_t)ctx->K * ctx->Nq * ctx->Nc;
     const double* src = nullptr;
@@ -560,7 +613,7 @@ int p2de_get_field(p2de_ctx* ctx, int field, double* out) {
         case P2DE_FIELD_THETA:
             if (!ctx->ep_limit)
                 return fail(ctx, P2DE_E_INVALID, "theta exists with NodewiseScaledExtrapolation only");
-            src = ctx->d_theta;
+            src = theta_slot(ctx, std::max(ctx->last_stage, 1));
             n = (size_t)ctx->Nfp * ctx->K;
             break;
         default: return fail(ctx, P2DE_E_INVALID, "unknown field");
@@ -578,6 +631,7 @@ int p2de_set_time(p2de_ctx* ctx, double t) {
     if (st) return st;
     ctx->h_sc->t = t;
     ctx->h_sc->done = 0;
+    ctx->h_sc->hold = 0;
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_sc, ctx->h_sc, sizeof(StepScalars), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return P2DE_OK;
@@ -672,6 +726,210 @@ int p2de_run(p2de_ctx* ctx, int max_steps, p2de_run_stats* stats) {
         stats->last_dt = bits_to_double_h(ctx->h_sc->dt_bits);
         stats->violations = (int64_t)ctx->h_sc->viol;
     }
+    return P2DE_OK;
+}
+
+int p2de_run_segment(p2de_ctx* ctx, int output_interval, int max_steps, int64_t* step_index, double* dthist,
+                     int dthist_cap, int* steps_taken, int* at_output) {
+    if (!ctx || !step_index || !dthist || !steps_taken || !at_output)
+        return fail(ctx, P2DE_E_INVALID, "NULL argument");
+    if (output_interval < 1) return fail(ctx, P2DE_E_INVALID, "output_interval must be >= 1");
+    if (dthist_cap < 1) return fail(ctx, P2DE_E_INVALID, "dthist_cap must be >= 1");
+    const int limit = max_steps > 0 ? std::min(max_steps, dthist_cap) : dthist_cap;
+    const double T = ctx->cfg.T;
+    // SSPRK33.jl:42-46: after a step, i = i + 1; a history point when mod(i, output_interval) == 0
+    // or |t - T| < 1e-10
+    auto history_point = [&](long long i, double t) { return i % output_interval == 0 || std::fabs(t - T) < 1e-10; };
+    *steps_taken = 0;
+    *at_output = 0;
+    if (ctx->mix) {
+        // the batched kernels keep no history: one step per call
+        long long i = *step_index;
+        for (int n = 0; n < limit; n++) {
+            int n0 = 0, n1 = 0, st;
+            double t = 0.0, dt = 0.0;
+            if ((st = fwd(ctx, p2de_batch_get_time(ctx->mix, 0, nullptr, nullptr, &n0))) ||
+                (st = fwd(ctx, p2de_batch_enqueue_steps(ctx->mix, 1))) ||
+                (st = fwd(ctx, p2de_batch_get_time(ctx->mix, 0, &t, &dt, &n1))))
+                return st;
+            if (n1 == n0) break;
+            dthist[n] = dt;
+            *steps_taken = n + 1;
+            *step_index = ++i;
+            if (history_point(i, t)) {
+                *at_output = 1;
+                break;
+            }
+        }
+        return P2DE_OK;
+    }
+    if (!ctx->state_set) return fail(ctx, P2DE_E_STATE, "no state uploaded (p2de_set_state)");
+    if (limit > ctx->dtlog_cap) {
+        if (ctx->d_dtlog) {
+            HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+            (void)hipFree(ctx->d_dtlog);
+            ctx->d_dtlog = nullptr;
+            ctx->dtlog_cap = 0;
+        }
+        int st = dalloc(ctx, &ctx->d_dtlog, (size_t)limit);
+        if (st) return st;
+        ctx->dtlog_cap = limit;
+    }
+    int st = read_scalars(ctx);
+    if (st) return st;
+    ctx->h_sc->hold = 0;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_sc, ctx->h_sc, sizeof(StepScalars), hipMemcpyHostToDevice, ctx->stream));
+    ctx->seg_interval = output_interval;
+    ctx->seg_cap = limit;
+    ctx->seg_base = ctx->h_sc->steps;
+    ctx->seg_i0 = *step_index;
+    // steps after the history point (or after the limit) find hold set and do nothing
+    const int chunk = 32;
+    int launched = 0;
+    while (launched < limit) {
+        const int n = std::min(chunk, limit - launched);
+        for (int j = 0; j < n && !st; j++) st = enqueue_step(ctx);
+        if (!st) st = read_scalars(ctx);
+        if (st) break;
+        launched += n;
+        if (ctx->h_sc->hold || ctx->h_sc->done || !(ctx->h_sc->t < T)) break;
+    }
+    ctx->seg_interval = 0;
+    if (st) return st;
+    const int taken = ctx->h_sc->steps - ctx->seg_base;
+    if (taken > 0)
+        HIPCHK(ctx, hipMemcpyAsync(dthist, ctx->d_dtlog, sizeof(double) * taken, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->h_sc->hold = 0;
+    ctx->h_sc->done = 0;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_sc, ctx->h_sc, sizeof(StepScalars), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    *steps_taken = taken;
+    *step_index = ctx->seg_i0 + taken;
+    *at_output = taken > 0 && history_point(*step_index, ctx->h_sc->t);
+    return P2DE_OK;
+}
+
+int p2de_check_conservation(p2de_ctx* ctx, double* total) {
+    if (!ctx || !total) return fail(ctx, P2DE_E_INVALID, "NULL argument");
+    const int Nq = ctx->Nq, Nc = ctx->Nc, K = ctx->K;
+    std::vector<double> U((size_t)K * Nq * Nc);
+    int st = p2de_get_state(ctx, U.data());
+    if (st) return st;
+    // utils.jl:5-11: total += J[i, k] * wq[i] * Uq[i, k] over k, then i; then the sum of the components
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < K; k++)
+        for (int i = 0; i < Nq; i++) {
+            const double w = ctx->h_Jq[(size_t)k * Nq + i] * ctx->h_wq[i];
+            const double* u = &U[((size_t)k * Nq + i) * Nc];
+            for (int c = 0; c < Nc; c++) acc[c] = acc[c] + w * u[c];
+        }
+    double sum = acc[0];
+    for (int c = 1; c < Nc; c++) sum = sum + acc[c];
+    *total = sum;
+    return P2DE_OK;
+}
+
+namespace {
+
+// Checkpoint layout: header, then the state double[K][Nq][Nc]; integers and doubles little-endian.
+struct CheckpointHeader {
+    char magic[8];            // "P2DECKPT"
+    uint32_t version;         // 1
+    uint32_t header_bytes;
+    int32_t Nd, N, Nc, basis, precision, reserved;
+    int64_t K, Nq;
+    double t, last_dt, s_min;
+    int64_t steps, step_index;
+    uint64_t payload_bytes, checksum;
+};
+constexpr char kCkptMagic[8] = {'P', '2', 'D', 'E', 'C', 'K', 'P', 'T'};
+
+// FNV-1a over the payload bytes
+uint64_t fnv1a64(const unsigned char* p, size_t n) {
+    uint64_t h = 1469598103934665603ull;
+    for (size_t i = 0; i < n; i++) {
+        h ^= p[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+}  // namespace
+
+int64_t p2de_checkpoint_size(const p2de_ctx* ctx) {
+    if (!ctx) return 0;
+    return (int64_t)sizeof(CheckpointHeader) + (int64_t)sizeof(double) * ctx->K * ctx->Nq * ctx->Nc;
+}
+
+int p2de_checkpoint_save(p2de_ctx* ctx, int64_t step_index, void* buf, int64_t size) {
+    if (!ctx || !buf) return fail(ctx, P2DE_E_INVALID, "NULL argument");
+    if (size < p2de_checkpoint_size(ctx)) return fail(ctx, P2DE_E_INVALID, "buffer smaller than p2de_checkpoint_size");
+    CheckpointHeader h{};
+    std::memcpy(h.magic, kCkptMagic, 8);
+    h.version = 1;
+    h.header_bytes = sizeof(CheckpointHeader);
+    h.Nd = ctx->Nd;
+    h.N = ctx->N1D - 1;
+    h.Nc = ctx->Nc;
+    h.basis = ctx->cfg.basis;
+    h.precision = ctx->cfg.precision;
+    h.K = ctx->K;
+    h.Nq = ctx->Nq;
+    h.step_index = step_index;
+    double* U = (double*)((char*)buf + sizeof(CheckpointHeader));
+    int st = p2de_get_state(ctx, U);
+    if (st) return st;
+    if (ctx->mix) {
+        int steps = 0;
+        if ((st = fwd(ctx, batch_instance_stats(ctx->mix, 0, &h.t, &h.last_dt, &steps, nullptr)))) return st;
+        h.steps = steps;
+        h.s_min = 0.0;
+    } else {
+        if ((st = read_scalars(ctx))) return st;
+        h.t = ctx->h_sc->t;
+        h.last_dt = bits_to_double_h(ctx->h_sc->dt_bits);
+        h.s_min = ctx->h_sc->s_min;
+        h.steps = ctx->h_sc->steps;
+    }
+    h.payload_bytes = sizeof(double) * (uint64_t)ctx->K * ctx->Nq * ctx->Nc;
+    h.checksum = fnv1a64((const unsigned char*)U, h.payload_bytes);
+    std::memcpy(buf, &h, sizeof(h));
+    return P2DE_OK;
+}
+
+int p2de_checkpoint_load(p2de_ctx* ctx, const void* buf, int64_t size, int64_t* step_index) {
+    if (!ctx || !buf) return fail(ctx, P2DE_E_INVALID, "NULL argument");
+    if (size < (int64_t)sizeof(CheckpointHeader)) return fail(ctx, P2DE_E_INVALID, "checkpoint truncated (header)");
+    CheckpointHeader h;
+    std::memcpy(&h, buf, sizeof(h));
+    if (std::memcmp(h.magic, kCkptMagic, 8) != 0) return fail(ctx, P2DE_E_INVALID, "not a checkpoint (magic)");
+    if (h.version != 1 || h.header_bytes != sizeof(CheckpointHeader))
+        return fail(ctx, P2DE_E_INVALID, "unsupported checkpoint version");
+    if (h.Nd != ctx->Nd || h.N != ctx->N1D - 1 || h.Nc != ctx->Nc || h.basis != ctx->cfg.basis || h.K != ctx->K ||
+        h.Nq != ctx->Nq)
+        return fail(ctx, P2DE_E_INVALID, "checkpoint of another problem (dimension, degree, basis or mesh size)");
+    const uint64_t payload = sizeof(double) * (uint64_t)ctx->K * ctx->Nq * ctx->Nc;
+    if (h.payload_bytes != payload || (uint64_t)size < sizeof(CheckpointHeader) + payload)
+        return fail(ctx, P2DE_E_INVALID, "checkpoint truncated (state)");
+    const double* U = (const double*)((const char*)buf + sizeof(CheckpointHeader));
+    if (fnv1a64((const unsigned char*)U, payload) != h.checksum)
+        return fail(ctx, P2DE_E_INVALID, "checkpoint checksum mismatch");
+    int st = p2de_set_state(ctx, U);
+    if (st) return st;
+    if (ctx->mix) {
+        if ((st = fwd(ctx, batch_instance_restore(ctx->mix, 0, h.t, h.last_dt, (int)h.steps)))) return st;
+    } else {
+        if ((st = read_scalars(ctx))) return st;
+        ctx->h_sc->t = h.t;
+        std::memcpy(&ctx->h_sc->dt_bits, &h.last_dt, sizeof(double));
+        ctx->h_sc->s_min = h.s_min;
+        ctx->h_sc->steps = (int)h.steps;
+        ctx->h_sc->done = 0;
+        ctx->h_sc->hold = 0;
+        HIPCHK(ctx, hipMemcpyAsync(ctx->d_sc, ctx->h_sc, sizeof(StepScalars), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    if (step_index) *step_index = h.step_index;
     return P2DE_OK;
 }
 

@@ -98,6 +98,7 @@ struct StepScalars {
     unsigned long long dt_bits;   // CFL dt accumulator (positive double bit pattern), then the step dt
     int done;                     // t >= T: kernels of the step do nothing
     int steps;
+    int hold;                     // a history point was reached: later steps do nothing until cleared
     unsigned long long viol;      // positivity violations
     unsigned long long first_bad; // min over (stage<<40 | node) of offending nodes
     double s_min;                 // minimum modified specific entropy of the state at t0 (min-entropy bounds)
@@ -124,6 +125,7 @@ struct PhaseAArgs {
     double* dbg_rhsH;             // optional debug outputs (nullptr: off)
     double* dbg_rhsL;
     double* dbg_L;                // L_local[:, :, :] of this stage, (Nq+N1D, Nd, K)
+    double* zsL;                  // Zhang-Shu: L[:, nstage] of this stage, [K] (else nullptr)
     double* dtp;                  // stage 1: per-workgroup minimum of the CFL time step
     // limiter options (element-kernel variants OPT 1 and 2)
     int bound;                    // P2DE_POSITIVITY*_BOUND
@@ -157,6 +159,15 @@ struct PhaseBArgs {
     double* dbg_L;
     double* ut;                   // projected face states of Uout (stage modes; nullptr: not formed here)
     int finalize;                 // stage 3: advance t
+    // history segments (finalize only): each step's dt goes to dt_log[steps - steps_base - 1]
+    // while that index is below dt_cap; hold is set after the step whose index
+    // i = i0 + steps - steps_base is a history point of the time loop, or that fills dt_log
+    double* dt_log;
+    int dt_cap;
+    int steps_base;
+    long long i0;
+    int out_interval;             // 0: no history segment
+    double T_end;                 // final time T
 };
 
 // Per stage before the element kernel, for the limiter options: smoothness indicator, blending

@@ -115,3 +115,4 @@ int mix_launch_positivity(const double* U, int64_t nodes, double gamma, unsigned
 // time, last dt, steps and the positivity violations counted since the last check of one
 // instance, without resetting the count (for p2de_run on a mixed-precision context)
 int batch_instance_stats(p2de_batch* b, int inst, double* t, double* last_dt, int* steps, int64_t* viol);
+int batch_instance_restore(p2de_batch* b, int inst, double t, double last_dt, int steps);

@@ -580,6 +580,7 @@ __global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
                 a.rhsUA[((size_t)k * NQ + i) * NC + c] = (1.0 - l) * lo + l * h;
             }
             if (a.dbg_L && i == 0) a.dbg_L[k] = Lk;
+            if (a.zsL && i == 0) a.zsL[k] = Lk;
         }
         // no subcell faces: the update kernel's symmetrisation finds nothing to correct
         for (int w = tid; w < E * NFP; w += NT) {
@@ -944,8 +945,18 @@ __global__ __launch_bounds__(256) void phaseB_kernel(PhaseBArgs a) {
     const size_t gidx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const double dt = bits_to_double(a.sc->dt_bits);
     if (a.finalize && gidx == 0) {
-        a.sc->t = a.sc->t + dt;
-        a.sc->steps = a.sc->steps + 1;
+        const double t_new = a.sc->t + dt;
+        const int steps = a.sc->steps + 1;
+        a.sc->t = t_new;
+        a.sc->steps = steps;
+        if (a.out_interval > 0) {
+            // SSPRK33.jl:42-46: i = i + 1, then a history point when mod(i, output_interval) == 0
+            // or |t - T| < 1e-10
+            const int n = steps - a.steps_base;
+            if (n >= 1 && n <= a.dt_cap) a.dt_log[n - 1] = dt;
+            const long long i = a.i0 + n;
+            if (i % a.out_interval == 0 || fabs(t_new - a.T_end) < 1e-10 || n >= a.dt_cap) a.sc->hold = 1;
+        }
     }
     if (a.mode == MODE_STAGE1 && gidx == 0) a.sc->dt_lim = dt;  // stages 2 and 3 limit with the CFL dt
     if (gidx >= (size_t)a.K * NQ) return;
@@ -1038,7 +1049,7 @@ __global__ __launch_bounds__(256) void phaseB_kernel(PhaseBArgs a) {
 __global__ void step_init_kernel(StepScalars* sc, double CFL, double dt0, double T) {
     if (threadIdx.x != 0) return;
     const double t = sc->t;
-    if (!(t < T)) {
+    if (!(t < T) || sc->hold) {
         sc->done = 1;
         return;
     }

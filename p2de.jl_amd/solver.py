@@ -10,7 +10,7 @@ fails.
 from __future__ import annotations
 
 import ctypes as C
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 from pathlib import Path
 
 import numpy as np
@@ -20,7 +20,7 @@ from .dg import FlatProblem, State, StateParam, Solver, TimeParam, flat_problem
 LIB_PATH = Path(__file__).resolve().parent / "csrc" / "libp2de_hip.so"
 
 P2DE_OK, P2DE_E_INVALID, P2DE_E_UNSUPPORTED, P2DE_E_HIP, P2DE_E_ALLOC, P2DE_E_STATE, P2DE_E_POSITIVITY = range(7)
-FIELD_RHSU, FIELD_RHSH, FIELD_RHSL, FIELD_L_LOCAL, FIELD_THETA = range(5)
+FIELD_RHSU, FIELD_RHSH, FIELD_RHSL, FIELD_L_LOCAL, FIELD_THETA, FIELD_L, FIELD_THETA_MEAN = range(7)
 UPPER_BOUND_REFERENCE, UPPER_BOUND_INTENDED = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 BASIS_LOBATTO, BASIS_GAUSS = 0, 1
@@ -105,6 +105,13 @@ def lib():
         L.p2de_set_variant.argtypes = [vp, C.c_int]
         L.p2de_get_variant.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.p2de_autotune.argtypes = [vp, C.c_int, C.c_char_p, C.POINTER(C.c_int)]
+        L.p2de_run_segment.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int64), _dp, C.c_int,
+                                       C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.p2de_check_conservation.argtypes = [vp, _dp]
+        L.p2de_checkpoint_size.argtypes = [vp]
+        L.p2de_checkpoint_size.restype = C.c_int64
+        L.p2de_checkpoint_save.argtypes = [vp, C.c_int64, C.c_void_p, C.c_int64]
+        L.p2de_checkpoint_load.argtypes = [vp, C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]
         bp = C.c_void_p
         L.p2de_batch_create.argtypes = [C.c_int, C.POINTER(_Config), C.POINTER(_Operators), C.POINTER(_Geometry),
                                         C.POINTER(_BCData), C.POINTER(bp)]
@@ -152,6 +159,8 @@ EXPORTED_SYMBOLS = (
     "p2de_check_positivity", "p2de_enqueue_steps", "p2de_synchronize", "p2de_set_kernel_timing",
     "p2de_get_kernel_timing", "p2de_num_dofs", "p2de_device_state", "p2de_last_error", "p2de_status_string",
     "p2de_num_variants", "p2de_set_variant", "p2de_get_variant", "p2de_autotune",
+    "p2de_run_segment", "p2de_check_conservation", "p2de_checkpoint_size", "p2de_checkpoint_save",
+    "p2de_checkpoint_load",
     "p2de_batch_create", "p2de_batch_destroy", "p2de_batch_num_instances", "p2de_batch_num_dofs",
     "p2de_batch_set_state", "p2de_batch_get_state", "p2de_batch_device_state", "p2de_batch_set_time",
     "p2de_batch_get_time", "p2de_batch_rhs", "p2de_batch_set_debug", "p2de_batch_get_field",
@@ -264,8 +273,12 @@ class DeviceSolver:
             n = (self.Nq + self.fp.N1D) * self.fp.Nd * self.K
         elif which == FIELD_THETA:
             n = self.fp.Nfp * self.K
+        elif which in (FIELD_L, FIELD_THETA_MEAN):
+            n = 3 * self.K
         out = np.empty(n)
         self._check(lib().p2de_get_field(self.h, int(which), _d(out)))
+        if which in (FIELD_L, FIELD_THETA_MEAN):
+            return out.reshape(3, self.K).T      # (K, 3) as the reference's L and theta
         return out
 
     # --------------------------------------------------------------- time stepping
@@ -288,6 +301,36 @@ class DeviceSolver:
         st = RunStats()
         self._check(lib().p2de_run(self.h, int(max_steps), C.byref(st)))
         return st
+
+    def run_segment(self, output_interval: int, step_index: int, max_steps: int = 0, cap: int = 4096):
+        """Steps until the next history entry of SSP33!'s loop (``p2de_run_segment``); returns
+        (dts of the steps taken, new step index, whether the segment ended at a history entry)."""
+        dts = np.empty(max(1, int(cap)))
+        i = C.c_int64(int(step_index))
+        n, out = C.c_int(0), C.c_int(0)
+        self._check(lib().p2de_run_segment(self.h, int(output_interval), int(max_steps), C.byref(i), _d(dts),
+                                           int(dts.size), C.byref(n), C.byref(out)))
+        return dts[:n.value].copy(), int(i.value), bool(out.value)
+
+    def check_conservation(self) -> float:
+        """``check_conservation`` (src/dg/utils.jl:1-12) of the device state."""
+        tot = C.c_double(0.0)
+        self._check(lib().p2de_check_conservation(self.h, C.byref(tot)))
+        return tot.value
+
+    def checkpoint(self, step_index: int = 1) -> bytes:
+        """The context's resumable snapshot (``p2de_checkpoint_save``) as bytes."""
+        n = int(lib().p2de_checkpoint_size(self.h))
+        buf = C.create_string_buffer(n)
+        self._check(lib().p2de_checkpoint_save(self.h, int(step_index), buf, n))
+        return buf.raw
+
+    def restore(self, blob: bytes) -> int:
+        """Loads a snapshot taken by ``checkpoint``; returns its step index."""
+        i = C.c_int64(0)
+        buf = C.create_string_buffer(bytes(blob), len(blob))
+        self._check(lib().p2de_checkpoint_load(self.h, buf, len(blob), C.byref(i)))
+        return int(i.value)
 
     def enqueue_steps(self, n: int):
         self._check(lib().p2de_enqueue_steps(self.h, int(n)))
@@ -474,18 +517,57 @@ def rhs(state: State, solver: Solver, state_param: StateParam, time_param: TimeP
 
 @dataclass
 class DataHistory:
-    """The part of ``DataHistory`` (PostprocessData.jl:1-7) the drop-in keeps."""
-    thist: list
-    dthist: list
-    positivity_violations: int
+    """``DataHistory`` (src/dg/PostprocessData.jl:1-7): the states, element limiting
+    parameters ``L``, element entropy-projection parameters ``theta`` and times of the history
+    entries, every step's dt, plus the conservation totals printed at each entry and the
+    positivity violations the device counted."""
+    Uhist: list = field(default_factory=list)
+    Lhist: list = field(default_factory=list)
+    thetahist: list = field(default_factory=list)
+    thist: list = field(default_factory=list)
+    dthist: list = field(default_factory=list)
+    conservation: list = field(default_factory=list)
+    positivity_violations: int = 0
 
 
-def SSP33(state: State, solver: Solver, state_param: StateParam, max_steps: int = 0) -> DataHistory:
+def SSP33(state: State, solver: Solver, state_param: StateParam, max_steps: int = 0, verbose: bool = False,
+          resume: bytes = None) -> DataHistory:
     """``SSP33!`` (src/timestepping/SSPRK33.jl:1-62): advances ``state.preallocation.Uq``
-    from t0 to T on the GPU (state resident on the device, time and dt too)."""
+    from t0 to T on the GPU and returns its ``DataHistory``.  Between history entries
+    (``mod(i, output_interval) == 0`` or ``|t - T| < 1e-10``, SSPRK33.jl:46) the steps stay
+    queued on the device; at each entry the state, ``L``, ``theta`` and time are copied out and
+    ``check_conservation`` taken, as the reference does.  ``resume``: a ``DeviceSolver.checkpoint``
+    blob to continue from instead of (Uq, t0); the history then starts at the checkpoint.
+    ``max_steps > 0`` stops early (the state is then the one reached)."""
+    T = solver.param.timestepping_param.T
+    interval = int(solver.param.postprocessing_param.output_interval)
     dev = _device(state, solver, state_param)
-    dev.set_state(state.preallocation.Uq)
-    dev.set_time(solver.param.timestepping_param.t0)
-    stats = dev.run(max_steps)
+    hist = DataHistory()
+    if resume is not None:
+        i = dev.restore(resume)
+    else:
+        dev.set_state(state.preallocation.Uq)
+        dev.set_time(solver.param.timestepping_param.t0)
+        i = 1
+    taken = 0
+    while max_steps <= 0 or taken < max_steps:
+        left = max_steps - taken if max_steps > 0 else 0
+        dts, i, at_output = dev.run_segment(interval, i, max_steps=left)
+        hist.dthist.extend(float(d) for d in dts)
+        taken += len(dts)
+        if at_output:
+            t, dt = dev.get_time()
+            hist.thist.append(t)
+            hist.Uhist.append(dev.get_state())
+            hist.Lhist.append(dev.field(FIELD_L))
+            hist.thetahist.append(dev.field(FIELD_THETA_MEAN))
+            total = dev.check_conservation()
+            hist.conservation.append(total)
+            if verbose:
+                print(f"Current time {t} with time step size {dt}, and final time {T}, step {i}", flush=True)
+                print(f"total_conservation = {total}", flush=True)
+        if len(dts) == 0 or not dev.get_time()[0] < T:
+            break
     state.preallocation.Uq[...] = dev.get_state()
-    return DataHistory(thist=[stats.t], dthist=[stats.last_dt], positivity_violations=int(stats.violations))
+    hist.positivity_violations = int(dev.check_positivity().violations)
+    return hist

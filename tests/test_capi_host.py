@@ -123,6 +123,11 @@ def test_null_arguments():
     assert L.p2de_rhs(None, 0.0, 0.0, 1, None, None) == S.P2DE_E_INVALID
     assert L.p2de_run(None, 1, None) == S.P2DE_E_INVALID
     assert L.p2de_num_dofs(None) == 0
+    assert L.p2de_run_segment(None, 1, 0, None, None, 1, None, None) == S.P2DE_E_INVALID
+    assert L.p2de_check_conservation(None, None) == S.P2DE_E_INVALID
+    assert L.p2de_checkpoint_size(None) == 0
+    assert L.p2de_checkpoint_save(None, 1, None, 0) == S.P2DE_E_INVALID
+    assert L.p2de_checkpoint_load(None, None, 0, None) == S.P2DE_E_INVALID
 
 
 # ----------------------------------------------------------------- batched C-ABI, host checks
