@@ -40,7 +40,9 @@ enum { P2DE_NO_ENTROPY_PROJ_LIMITER = 0, P2DE_NODEWISE_SCALED_EXTRAPOLATION = 1 
 enum { P2DE_SUBCELL_LIMITER = 0, P2DE_ZHANG_SHU_LIMITER = 1 };
 enum { P2DE_POSITIVITY_BOUND = 0,                          /* PositivityBound */
        P2DE_POSITIVITY_AND_MIN_ENTROPY_BOUND = 1,          /* PositivityAndMinEntropyBound */
-       P2DE_POSITIVITY_AND_RELAXED_MIN_ENTROPY_BOUND = 2 };/* PositivityAndRelaxedMinEntropyBound */
+       P2DE_POSITIVITY_AND_RELAXED_MIN_ENTROPY_BOUND = 2,  /* PositivityAndRelaxedMinEntropyBound */
+       P2DE_POSITIVITY_AND_CELL_ENTROPY_BOUND = 3,         /* PositivityAndCellEntropyBound (2D Lobatto) */
+       P2DE_POSITIVITY_AND_RELAXED_CELL_ENTROPY_BOUND = 4 };/* PositivityAndRelaxedCellEntropyBound(beta) */
 enum { P2DE_NO_SHOCK_CAPTURE = 0, P2DE_HENNEMANN_SHOCK_CAPTURE = 1 };
 /* FP64: the drop-in, matching the reference's FP64 arithmetic.  MIXED (2D only): FP32 flux
  * differencing, surface and low-order fluxes and element assembly; FP64 node pressures,
@@ -67,11 +69,13 @@ typedef struct {
     int entropyproj_limiter;      /* P2DE_NO_ENTROPY_PROJ_LIMITER; with Gauss also
                                      P2DE_NODEWISE_SCALED_EXTRAPOLATION */
     int rhs_limiter;              /* P2DE_SUBCELL_LIMITER or P2DE_ZHANG_SHU_LIMITER */
-    int bound;                    /* P2DE_POSITIVITY*_BOUND (SubcellLimiter; Zhang-Shu: POSITIVITY) */
+    int bound;                    /* P2DE_POSITIVITY*_BOUND (SubcellLimiter; Zhang-Shu: POSITIVITY);
+                                     the cell-entropy bounds in 2D with the Lobatto basis */
     int shockcapture;             /* P2DE_NO_SHOCK_CAPTURE or P2DE_HENNEMANN_SHOCK_CAPTURE */
     int upper_bound_mode;         /* P2DE_UPPER_BOUND_* */
     int precision;                /* P2DE_PRECISION_* */
     double shock_a, shock_c;      /* HennemannShockCapture(a, c) (Solver.jl:67-74) */
+    double cell_beta;             /* PositivityAndRelaxedCellEntropyBound(beta) (Solver.jl:52-54) */
 } p2de_config;
 
 /* Operators{DIM} (Solver.jl:192-212), sizes from SizeData (Solver.jl:173-183). */

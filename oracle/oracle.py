@@ -37,6 +37,7 @@ class _Problem(C.Structure):
         ("ZEROTOL", C.c_double), ("CFL", C.c_double), ("dt0", C.c_double), ("t0", C.c_double), ("T", C.c_double),
         ("upper_bound_mode", C.c_int), ("entropyproj_limiter", C.c_int), ("rhs_limiter", C.c_int),
         ("bound", C.c_int), ("shockcapture", C.c_int), ("shock_a", C.c_double), ("shock_c", C.c_double),
+        ("cell_beta", C.c_double),
         ("low_order_surface_flux", C.c_int),
         ("Srsh_db", _dp), ("Srs0", _dp), ("nnz_h", C.c_int), ("Srsh_nnz", _ip), ("nnz_0", C.c_int),
         ("Srs0_nnz", _ip), ("Brs", _dp), ("MinvVhT", _dp), ("MinvVfT", _dp), ("Vf", _dp), ("Vf_low", _dp),
@@ -120,7 +121,7 @@ class Oracle:
             entropyproj_limiter=int(getattr(fp, "entropyproj_limiter", 0)),
             rhs_limiter=int(getattr(fp, "rhs_limiter", 0)), bound=int(getattr(fp, "bound", 0)),
             shockcapture=int(getattr(fp, "shockcapture", 0)), shock_a=float(getattr(fp, "shock_a", 0.5)),
-            shock_c=float(getattr(fp, "shock_c", 1.8)), VDM_inv=_d(self._keep["VDM_inv"]),
+            shock_c=float(getattr(fp, "shock_c", 1.8)), cell_beta=float(getattr(fp, "cell_beta", 0.5)), VDM_inv=_d(self._keep["VDM_inv"]),
             low_order_surface_flux=int(getattr(fp, "low_order_surface_flux", 0)),
             Srsh_db=_d(k["Srsh_db"]), Srs0=_d(k["Srs0"]), nnz_h=len(k["Srsh_nnz"]) // 2,
             Srsh_nnz=_i(k["Srsh_nnz"]), nnz_0=len(k["Srs0_nnz"]) // 2, Srs0_nnz=_i(k["Srs0_nnz"]),

@@ -248,10 +248,13 @@ int oracle_create(const oracle_problem *p, oracle_ctx **out) {
     if (p->Nd != 1 && p->Nd != 2) return 2;
     if (p->Nc != p->Nd + 2) return 2;
     if (p->entropyproj_limiter != 0 && p->entropyproj_limiter != 1) return 2;
-    if (p->rhs_limiter < 0 || p->rhs_limiter > 1 || p->bound < 0 || p->bound > 2 || p->shockcapture < 0 ||
+    if (p->rhs_limiter < 0 || p->rhs_limiter > 1 || p->bound < 0 || p->bound > 4 || p->shockcapture < 0 ||
         p->shockcapture > 1 || p->low_order_surface_flux < 0 || p->low_order_surface_flux > 1)
         return 2;
-    if (!p->VDM_inv && (p->shockcapture != 0 || p->bound == 2)) return 2;
+    if (!p->VDM_inv && (p->shockcapture != 0 || p->bound == 2 || p->bound == 4)) return 2;
+    /* the cell-entropy bounds size their per-element scratch (Nq - N1D) x K (State.jl:182-183),
+     * which is empty in 1D, so the reference indexes out of bounds there (subcell.jl:503-505) */
+    if (p->bound >= 3 && p->Nd != 2) return 2;
     oracle_ctx *c = (oracle_ctx *)calloc(1, sizeof(oracle_ctx));
     if (!c) return 3;
     c->p = *p;
@@ -1010,8 +1013,9 @@ static double limiting_param(const oracle_ctx *c, const double *uL, const double
     double Lrho = c->p.zeta * uL[0];
     double Urho_i = INFINITY, Urhoe = INFINITY;
     double Lrhoe = c->p.zeta * rhoe_ufun(c->p.Nd, uL);
-    if (c->p.bound != 0) {
-        /* min-entropy bounds destructure all five values (limiter_utils.jl:20-22) */
+    if (c->p.bound == 1 || c->p.bound == 2) {
+        /* min-entropy bounds destructure all five values (limiter_utils.jl:20-22); the
+         * cell-entropy bounds take the positivity form (:10-14) */
         double lpos = limiting_param_bound_rho_rhoe(c, uL, P, Lrho, Lrhoe, Urho_i, Urhoe);
         return limiting_param_bound_phi(c, uL, P, Lphi, lpos);
     }
@@ -1131,7 +1135,7 @@ static void update_smoothness_factor(oracle_ctx *c, int s) {
     int K = p->K, N = p->N1D - 1;
     for (int k = 0; k < K; k++) {
         double f;
-        if (p->bound == 0) {
+        if (p->bound == 0 || p->bound == 3) {
             f = 0.0;
         } else if (p->bound == 1) {
             f = 1.0;
@@ -1165,7 +1169,7 @@ static void initialize_entropy_bounds(oracle_ctx *c, const double *Uq, double t,
     const oracle_problem *p = &c->p;
     int K = p->K, Nq = p->Nq, Nc = p->Nc, Nd = p->Nd, N1D = p->N1D;
     int s = nstage - 1;
-    if (p->bound == 0) {
+    if (p->bound == 0 || p->bound >= 3) {      /* subcell.jl:4-6 */
         memset(c->lbound_s_modified, 0, sizeof(double) * (size_t)Nq * K);
         return;
     }
@@ -1236,6 +1240,129 @@ static void apply_zhang_shu_limiter(oracle_ctx *c, const double *Uq, double dt, 
     }
 }
 
+/* Julia's isless on Float64 (NaN after every number, -0.0 before +0.0) and isequal */
+static int jisless(double a, double b) {
+    if (isnan(a)) return 0;
+    if (isnan(b)) return 1;
+    if (a < b) return 1;
+    return a == b && signbit(a) && !signbit(b);
+}
+static int jisequal(double a, double b) {
+    if (isnan(a) || isnan(b)) return isnan(a) && isnan(b);
+    return a == b && signbit(a) == signbit(b);
+}
+
+/* the positions 0..n-1 in the order of sort!(dvdf_order_k, rev=true) over (dvdf[i], i): descending
+ * by Julia's tuple isless, i.e. by value, then by index */
+static void es_order(const double *v, int n, int *order) {
+    int used[64] = {0};
+    for (int t = 0; t < n; t++) {
+        int best = -1;
+        for (int j = 0; j < n; j++) {
+            if (used[j]) continue;
+            if (best < 0 || jisless(v[best], v[j]) || (jisequal(v[best], v[j]) && best < j)) best = j;
+        }
+        used[best] = 1;
+        order[t] = best;
+    }
+}
+
+/* enforce_ES_subcell_volume!(::Dim2) subcell.jl:628-734 with initialize_ES_subcell_limiting!(::Dim2)
+ * :511-562 and rhs_es :736-743: per element and direction, dvdf = (v_{i-1} - v_i).(fbar_H - fbar_L)
+ * at the interior subcell faces; if sum(l dvdf) - rhs exceeds tol = max(0, sum dv.fbar_L - 1'B psi),
+ * zero the parameters of the faces with the largest dvdf, greedily, and scale the last one so that
+ * the sum reaches rhs + tol */
+static void enforce_es_subcell_volume(oracle_ctx *c, const double *Uq, int nstage, int nthr) {
+    const oracle_problem *p = &c->p;
+    int K = p->K, Nq = p->Nq, Nfp = p->Nfp, Nc = p->Nc, N1D = p->N1D, Nd = 2;
+    int s = nstage - 1;
+    int N1Dp1 = N1D + 1, N1Dm1 = N1D - 1, n = Nq - N1D;
+    double g = p->gamma;
+    OMP_FOR
+    for (int k = 0; k < K; k++) {
+        double V[25][NC_MAX];
+        for (int i = 0; i < Nq; i++) v_ufun(2, g, Uq + IDX_V(i, k, Nq, Nc), V[i]);
+        double sBpsi[2] = {0.0, 0.0};
+        for (int i = 0; i < Nfp; i++) {
+            const double *uf = Uq + IDX_V((int)p->fq2q[i] - 1, k, Nq, Nc);
+            double psi[2] = {(g - 1.0) * uf[1], (g - 1.0) * uf[2]};   /* psi_ufun :129-132 */
+            double B[2] = {0.0, 0.0};
+            Bx(c, i, k, B);
+            sBpsi[0] = sBpsi[0] + B[0] * psi[0];
+            sBpsi[1] = sBpsi[1] + B[1] * psi[1];
+        }
+        double dvdf[2][20], sdvfL[2] = {0.0, 0.0};
+        for (int sj = 0; sj < N1D; sj++)
+            for (int si = 1; si < N1D; si++) {
+                const double *fL = FB(c->f_bar_L, 0, si + N1Dp1 * sj, k), *fH = FB(c->f_bar_H, 0, si + N1Dp1 * sj, k);
+                const double *va = V[(si - 1) + N1D * sj], *vb = V[si + N1D * sj];
+                double a = 0.0, b = 0.0;
+                for (int cc = 0; cc < Nc; cc++) {
+                    double dv = va[cc] - vb[cc];
+                    double ta = dv * (fH[cc] - fL[cc]), tb = dv * fL[cc];
+                    a = cc == 0 ? ta : a + ta;
+                    b = cc == 0 ? tb : b + tb;
+                }
+                dvdf[0][(si - 1) + N1Dm1 * sj] = a;
+                sdvfL[0] += b;
+            }
+        for (int si = 0; si < N1D; si++)
+            for (int sj = 1; sj < N1D; sj++) {
+                const double *fL = FB(c->f_bar_L, 1, si + N1D * sj, k), *fH = FB(c->f_bar_H, 1, si + N1D * sj, k);
+                const double *va = V[si + N1D * (sj - 1)], *vb = V[si + N1D * sj];
+                double a = 0.0, b = 0.0;
+                for (int cc = 0; cc < Nc; cc++) {
+                    double dv = va[cc] - vb[cc];
+                    double ta = dv * (fH[cc] - fL[cc]), tb = dv * fL[cc];
+                    a = cc == 0 ? ta : a + ta;
+                    b = cc == 0 ? tb : b + tb;
+                }
+                dvdf[1][si + N1D * (sj - 1)] = a;
+                sdvfL[1] += b;
+            }
+        double epsk = c->smooth_factor[k + (int64_t)K * s];
+        for (int d = 0; d < 2; d++) {
+            /* the subcell face of dvdf[d][idx] in L_local (reshape (N1Dp1, N1D) / (N1D, N1Dp1)) */
+#define ES_LI(idx) (d == 0 ? ((idx) % N1Dm1 + 1) + N1Dp1 * ((idx) / N1Dm1) : ((idx) % N1D) + N1D * ((idx) / N1D + 1))
+            double poslim = 0.0;
+            if (d == 0) {
+                for (int sj = 0; sj < N1D; sj++)
+                    for (int si = 1; si < N1D; si++) {
+                        int idx = (si - 1) + N1Dm1 * sj;
+                        poslim += LLOC(ES_LI(idx), 0, k, s) * dvdf[0][idx];
+                    }
+            } else {
+                for (int si = 0; si < N1D; si++)
+                    for (int sj = 1; sj < N1D; sj++) {
+                        int idx = si + N1D * (sj - 1);
+                        poslim += LLOC(ES_LI(idx), 1, k, s) * dvdf[1][idx];
+                    }
+            }
+            double diff = sBpsi[d] - sdvfL[d];
+            double rhs = p->bound == 3 ? diff : (1.0 - p->cell_beta * epsk) * diff;
+            double est = poslim - rhs;
+            double tol = jmax(0.0, sdvfL[d] - sBpsi[d]);
+            if (!(est > tol)) continue;
+            int order[20];
+            es_order(dvdf[d], n, order);
+            int curr = 0;
+            double lhs = poslim;
+            while (lhs > rhs + tol && curr < n) {
+                int idx = order[curr];
+                if (dvdf[d][idx] < p->ZEROTOL) break;
+                lhs = lhs - LLOC(ES_LI(idx), d, k, s) * dvdf[d][idx];
+                curr++;
+            }
+            for (int i = 0; i < curr; i++) {
+                int idx = order[i];
+                double l_new = i == curr - 1 ? jmax((rhs + tol - lhs) / dvdf[d][idx], 0.0) : 0.0;
+                LLOC(ES_LI(idx), d, k, s) = jmin(LLOC(ES_LI(idx), d, k, s), l_new);
+            }
+#undef ES_LI
+        }
+    }
+}
+
 static void apply_rhs_limiter(oracle_ctx *c, const double *Uq, double t, double dt, int nstage, int nthr) {
     const oracle_problem *p = &c->p;
     int K = p->K, Nq = p->Nq, Nfp = p->Nfp, Nc = p->Nc, Nd = p->Nd, N1D = p->N1D;
@@ -1243,7 +1370,7 @@ static void apply_rhs_limiter(oracle_ctx *c, const double *Uq, double t, double 
     int NLs = c->NLs;
     /* initialize_smoothness_indicator! shock_capture.jl:4-12: nothing for (NoShockCapture,
      * PositivityBound), which is also the bound the Zhang-Shu limiter reports (traits.jl:61-63) */
-    if (p->shockcapture != 0 || (p->rhs_limiter == 0 && p->bound != 0)) smoothness_indicator(c, Uq, nthr);
+    if ((p->shockcapture != 0 || (p->rhs_limiter == 0 && p->bound != 0)) && p->VDM_inv) smoothness_indicator(c, Uq, nthr);
     update_blending_factor(c, s);
     if (p->rhs_limiter == 1) {
         apply_zhang_shu_limiter(c, Uq, dt, nstage, nthr);
@@ -1374,7 +1501,10 @@ static void apply_rhs_limiter(oracle_ctx *c, const double *Uq, double t, double 
         }
     }
 
-    /* enforce_ES_subcell!(PositivityBound) subcell.jl:458-460: nothing */
+    /* enforce_ES_subcell! subcell.jl:458-466: nothing for the positivity and min-entropy bounds;
+     * the cell-entropy bounds limit the interior subcell faces further (the interface step,
+     * :745-789, is the Gauss basis's; for Lobatto the interface fluxes coincide) */
+    if (p->bound >= 3) enforce_es_subcell_volume(c, Uq, nstage, nthr);
 
     /* symmetrize_limiting_parameters! */
     if (Nd == 1) {

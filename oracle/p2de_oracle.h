@@ -46,12 +46,15 @@ typedef struct {
     /* Param.rhs_limiter: 0 SubcellLimiter, 1 ZhangShuLimiter (zhangshu.jl:4-56) */
     int rhs_limiter;
     /* SubcellLimiter bound: 0 PositivityBound, 1 PositivityAndMinEntropyBound,
-     * 2 PositivityAndRelaxedMinEntropyBound (subcell.jl:4-75, 927-956, limiter_utils.jl:19-50) */
+     * 2 PositivityAndRelaxedMinEntropyBound (subcell.jl:4-75, 927-956, limiter_utils.jl:19-50),
+     * 3 PositivityAndCellEntropyBound, 4 PositivityAndRelaxedCellEntropyBound(cell_beta)
+     * (subcell.jl:458-823; 2D) */
     int bound;
     /* shockcapture: 0 NoShockCapture, 1 HennemannShockCapture(shock_a, shock_c)
      * (shock_capture.jl:10-132) */
     int shockcapture;
     double shock_a, shock_c;
+    double cell_beta;
     /* ESLimitedLowOrderPos low_order_surface_flux: 0 LaxFriedrichsOnNodalVal, 1
      * LaxFriedrichsOnProjectedVal (low_order_graph_viscosity.jl:63-71, 249-327) */
     int low_order_surface_flux;

@@ -40,7 +40,8 @@ from .types import (CompressibleEulerIdealGas, Dim1, Dim2, ESLimitedLowOrderPos,
                     GlobalConstant, HennemannShockCapture, LaxFriedrichsOnNodalVal, LaxFriedrichsOnProjectedVal,
                     LimitingParameter, LobattoCollocation,
                     NodewiseScaledExtrapolation, NoEntropyProjectionLimiter, Param,
-                    PositivityAndMinEntropyBound, PositivityAndRelaxedMinEntropyBound, PositivityBound,
+                    PositivityAndCellEntropyBound, PositivityAndMinEntropyBound,
+                    PositivityAndRelaxedCellEntropyBound, PositivityAndRelaxedMinEntropyBound, PositivityBound,
                     SubcellLimiter, TimesteppingParameter, ZhangShuLimiter)
 
 # approximation basis and entropy-projection limiter by name
@@ -58,6 +59,8 @@ LIMITERS = {
     "subcell_shock": lambda: SubcellLimiter(bound=PositivityBound(), shockcapture=HennemannShockCapture()),
     "minentropy": lambda: SubcellLimiter(bound=PositivityAndMinEntropyBound()),
     "relaxed_minentropy": lambda: SubcellLimiter(bound=PositivityAndRelaxedMinEntropyBound()),
+    "cellentropy": lambda: SubcellLimiter(bound=PositivityAndCellEntropyBound()),
+    "relaxed_cellentropy": lambda: SubcellLimiter(bound=PositivityAndRelaxedCellEntropyBound(beta=0.5)),
 }
 
 # dt0 = 0.9*dt_pos(U0)/CFL, recorded from the initial states (tests/golden/record_dt0.py)

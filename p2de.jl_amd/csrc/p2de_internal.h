@@ -126,6 +126,8 @@ struct PhaseAArgs {
     double* dbg_rhsL;
     double* dbg_L;                // L_local[:, :, :] of this stage, (Nq+N1D, Nd, K)
     double* zsL;                  // Zhang-Shu: L[:, nstage] of this stage, [K] (else nullptr)
+    const double* sfac;           // smoothness factor per element (relaxed cell-entropy bound), [K]
+    double cell_beta;             // PositivityAndRelaxedCellEntropyBound(beta)
     double* dtp;                  // stage 1: per-workgroup minimum of the CFL time step
     // limiter options (element-kernel variants OPT 1 and 2)
     int bound;                    // P2DE_POSITIVITY*_BOUND
@@ -182,6 +184,7 @@ struct PrepArgs {
     double gamma, shock_a, shock_c;
     double* blend;                // [K] (shock capture)
     double* lphi;                 // [K][Nq] (min-entropy bounds)
+    double* sfac;                 // [K] smoothness factor (relaxed cell-entropy bound), may be nullptr
 };
 
 // Entropy projection of the face states (Gauss basis): theta per face node and u_tilde.

@@ -167,6 +167,95 @@ __device__ __noinline__ double limiting_param_phi(const double* uL, const double
     return x_valid;
 }
 
+// Julia's isless on Float64 (NaN after every number, -0.0 before +0.0) and isequal
+__device__ __forceinline__ bool jisless(double a, double b) {
+    if (isnan(a)) return false;
+    if (isnan(b)) return true;
+    if (a < b) return true;
+    return a == b && signbit(a) && !signbit(b);
+}
+__device__ __forceinline__ bool jisequal(double a, double b) {
+    if (isnan(a) || isnan(b)) return isnan(a) && isnan(b);
+    return a == b && signbit(a) == signbit(b);
+}
+
+// enforce_ES_subcell_volume!(::Dim2) (subcell.jl:628-734) for one element and direction d, with
+// initialize_ES_subcell_limiting!(::Dim2) (:511-562) and rhs_es (:736-743): dvdf = (v_a - v_b).(fbar_H
+// - fbar_L) at the interior subcell faces of direction d (a before b along the line); when
+// sum(l dvdf) - rhs exceeds tol = max(0, sum dv.fbar_L - 1'B psi), the faces are taken in the order of
+// sort!((dvdf, idx), rev=true) and their parameters zeroed until the sum reaches rhs + tol, the last one
+// scaled to meet it.  L (the element's subcell-face parameters, LDS) is updated in place; the function
+// returns the number of faces it lowered and writes their positions to `changed`.
+template <int N1D>
+__device__ __noinline__ int es_subcell_volume(int d, double* L, const double* fH, const double* fL, const double* V,
+                                              const double* U, const ElemTables& T, const Geo& G, double g,
+                                              int bound, double beta, double epsk, double ZEROTOL, int* changed) {
+    constexpr int NC = 4, NQ = N1D * N1D, NFP = 4 * N1D, NS = NQ - N1D;
+    // 1'B psi over the face nodes, psi = (gamma - 1) (rho u, rho v) at the node fq2q
+    double sB = 0.0;
+    for (int f = 0; f < NFP; f++) {
+        const double B = d == 0 ? G.rxJ * T.Br[f] + G.sxJ * T.Bs[f] : G.ryJ * T.Br[f] + G.syJ * T.Bs[f];
+        const double psi = (g - 1.0) * U[T.fq2q[f] * NC + 1 + d];
+        sB = sB + B * psi;
+    }
+    double dd[NS];
+    int pos[NS];
+    double sL = 0.0;
+    for (int o = 0; o < N1D; o++)          // x: sj = o; y: si = o
+        for (int t = 1; t < N1D; t++) {    // x: si = t; y: sj = t
+            const int idx = d == 0 ? (t - 1) + (N1D - 1) * o : o + N1D * (t - 1);
+            const int r = d == 0 ? o * (N1D + 1) + t : (N1D + o) * (N1D + 1) + t;
+            const int qa = d == 0 ? (t - 1) + N1D * o : o + N1D * (t - 1);
+            const int qb = d == 0 ? t + N1D * o : o + N1D * t;
+            double a = 0.0, b = 0.0;
+            for (int c = 0; c < NC; c++) {
+                const double dv = V[qa * NC + c] - V[qb * NC + c];
+                const double ta = dv * (fH[r * NC + c] - fL[r * NC + c]), tb = dv * fL[r * NC + c];
+                a = c == 0 ? ta : a + ta;
+                b = c == 0 ? tb : b + tb;
+            }
+            dd[idx] = a;
+            pos[idx] = r;
+            sL += b;
+        }
+    double poslim = 0.0;
+    for (int o = 0; o < N1D; o++)
+        for (int t = 1; t < N1D; t++) {
+            const int idx = d == 0 ? (t - 1) + (N1D - 1) * o : o + N1D * (t - 1);
+            poslim += L[pos[idx]] * dd[idx];
+        }
+    const double diff = sB - sL;
+    const double rhs = bound == 3 ? diff : (1.0 - beta * epsk) * diff;
+    const double est = poslim - rhs;
+    const double tol = jmax(0.0, sL - sB);
+    if (!(est > tol)) return 0;
+    unsigned used = 0;
+    int order[NS];
+    int curr = 0;
+    double lhs = poslim;
+    while (lhs > rhs + tol && curr < NS) {
+        int best = -1;   // the next entry of the descending (dvdf, idx) order
+        for (int j = 0; j < NS; j++) {
+            if (used & (1u << j)) continue;
+            if (best < 0 || jisless(dd[best], dd[j]) || (jisequal(dd[best], dd[j]) && best < j)) best = j;
+        }
+        if (dd[best] < ZEROTOL) break;
+        used |= 1u << best;
+        order[curr] = best;
+        lhs = lhs - L[pos[best]] * dd[best];
+        curr++;
+    }
+    int n = 0;
+    for (int i = 0; i < curr; i++) {
+        const int idx = order[i];
+        const double l_new = i == curr - 1 ? jmax((rhs + tol - lhs) / dd[idx], 0.0) : 0.0;
+        const double l = jmin(L[pos[idx]], l_new);
+        if (!(l == L[pos[idx]])) changed[n++] = pos[idx];
+        L[pos[idx]] = l;
+    }
+    return n;
+}
+
 // find_alpha (low_order_graph_viscosity.jl:299-328): the smallest alpha, by doubling and then 50
 // bisection steps, with alpha*u_i - u~ of positive density and internal energy.  The reference's
 // doubling has no exit when no alpha works; here it ends when alpha overflows (alpha = Inf).
@@ -214,6 +303,9 @@ struct SmemA {
             double RL[E][S::NQ][DIM][S::NC];
             double UL[E][S::NQ][S::NC];
         } g2;
+        struct {
+            double V[E][S::NQ][S::NC];                   // entropy variables (cell-entropy bounds)
+        } g3;
     } a;
     union {
         struct {
@@ -635,7 +727,7 @@ __global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
         for (int c = 0; c < NC; c++) dfb[c] = sm.x.fb.fH[e][r][c] - sm.x.fb.fL[e][r][c];
         // the bound's limiting_param (limiter_utils.jl:10-24); Lphi is the node's lower entropy bound
         auto lim = [&](int q) {
-            if (OPT == OPT_SUBCELL_GENERAL && a.bound != 0)
+            if (OPT == OPT_SUBCELL_GENERAL && (a.bound == 1 || a.bound == 2))
                 return limiting_param_phi<DIM>(sm.a.g2.UL[e][q], P, zeta, ZEROTOL, a.lphi[(size_t)k * NQ + q], a.POSTOL, g);
             return limiting_param<DIM>(sm.a.g2.UL[e][q], P, zeta, ZEROTOL, a.upper_mode);
         };
@@ -670,6 +762,37 @@ __global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
         }
     }
     __syncthreads();
+
+    // ---- phase 5b': cell-entropy bounds (enforce_ES_subcell! subcell.jl:462-466; the interface step
+    //      :745-789 is the Gauss basis's, for Lobatto the interface fluxes coincide)
+    if constexpr (DIM == 2 && OPT == OPT_SUBCELL_GENERAL) {
+        if (a.bound >= 3) {
+            for (int w = tid; w < E * NQ; w += NT) {
+                const int e = w / NQ, i = w - e * NQ;
+                if (k0 + e >= K) continue;
+                Eq<DIM>::v_of_u(g, sm.U[e][i], sm.a.g3.V[e][i]);
+            }
+            __syncthreads();
+            for (int w = tid; w < E * DIM; w += NT) {
+                const int e = w / DIM, d = w - e * DIM;
+                const int k = k0 + e;
+                if (k >= K) continue;
+                int changed[N1D * N1D];
+                const int n = es_subcell_volume<N1D>(d, sm.x.fb.l[e], &sm.x.fb.fH[e][0][0], &sm.x.fb.fL[e][0][0],
+                                                     &sm.a.g3.V[e][0][0], &sm.U[e][0][0], T, a.geo[k], g, a.bound,
+                                                     a.cell_beta, a.sfac ? a.sfac[k] : 0.0, a.ZEROTOL, changed);
+                if (a.dbg_L)
+                    for (int m = 0; m < n; m++) {
+                        const int r = changed[m];
+                        const int line = r / (N1D + 1), sp = r - line * (N1D + 1);
+                        const int o = d == 0 ? line : line - N1D;
+                        const int idx = d == 0 ? sp + (N1D + 1) * o : o + N1D * sp;
+                        a.dbg_L[idx + (size_t)S::NLS * (d + DIM * (size_t)k)] = sm.x.fb.l[e][r];
+                    }
+            }
+            __syncthreads();
+        }
+    }
 
     // ---- phase 5c: rhsU with the element's own parameters (accumulate_f_bar_limited! :841-876,
     //      apply_subcell_limiter! :894-924)
@@ -721,7 +844,7 @@ __global__ __launch_bounds__(NT) void limiter_prep_kernel(PrepArgs a) {
     const int nel = min(E, a.K - k0);
     const ElemTables& T = *a.T;
     const double g = a.gamma;
-    const bool need_ind = a.shock != 0 || a.bound == 2;
+    const bool need_ind = a.shock != 0 || a.bound == 2 || a.bound == 4;
     for (int w = tid; w < nel * NQ; w += NT) {
         const int e = w / NQ, i = w - e * NQ;
         const double* src = a.U + ((size_t)(k0 + e) * NQ + i) * NC;
@@ -758,7 +881,7 @@ __global__ __launch_bounds__(NT) void limiter_prep_kernel(PrepArgs a) {
                 const double alpha = 1.0 / (1.0 + exp(-s_factor / TN * (ind - TN)));
                 a.blend[k] = jmax(jmin(1.0 - alpha, 1.0), alphamax);
             }
-            if (a.bound == 2) {
+            if (a.bound == 2 || a.bound == 4) {
                 const double kappa = 1.0;
                 const double s0 = log(pow((double)N, -4.0)) / log(10.0);
                 const double sk = log(ind) / log(10.0);
@@ -768,9 +891,10 @@ __global__ __launch_bounds__(NT) void limiter_prep_kernel(PrepArgs a) {
             }
         }
         sF[e] = f;
+        if (a.sfac) a.sfac[k] = f;
     }
     __syncthreads();
-    if (a.bound == 0) return;
+    if (a.bound == 0 || a.bound >= 3) return;
     for (int w = tid; w < nel * NQ; w += NT) {
         const int e = w / NQ, q = w - e * NQ;
         const int k = k0 + e;

@@ -189,10 +189,12 @@ class FlatProblem:
     basis: int = 0                  # 0 LobattoCollocation, 1 GaussCollocation
     entropyproj_limiter: int = 0    # 0 NoEntropyProjectionLimiter, 1 NodewiseScaledExtrapolation
     rhs_limiter: int = 0            # 0 SubcellLimiter, 1 ZhangShuLimiter
-    bound: int = 0                  # 0 PositivityBound, 1 ...MinEntropyBound, 2 ...RelaxedMinEntropyBound
+    bound: int = 0                  # 0 PositivityBound, 1 ...MinEntropyBound, 2 ...RelaxedMinEntropyBound,
+                                    # 3 ...CellEntropyBound, 4 ...RelaxedCellEntropyBound
     shockcapture: int = 0           # 0 NoShockCapture, 1 HennemannShockCapture(shock_a, shock_c)
     shock_a: float = 0.5
     shock_c: float = 1.8
+    cell_beta: float = 0.5          # PositivityAndRelaxedCellEntropyBound(beta)
     VDM_inv: Optional[np.ndarray] = None
     low_order_surface_flux: int = 0   # 0 LaxFriedrichsOnNodalVal, 1 LaxFriedrichsOnProjectedVal
 
@@ -234,6 +236,6 @@ def flat_problem(solver: Solver, state_param: StateParam, upper_bound_mode: int 
         upper_bound_mode=int(upper_bound_mode),
         basis=1 if isinstance(p.approximation_basis, GaussCollocation) else 0,
         entropyproj_limiter=1 if isinstance(p.entropyproj_limiter, NodewiseScaledExtrapolation) else 0,
-        rhs_limiter=lc[0], bound=lc[1], shockcapture=lc[2], shock_a=lc[3], shock_c=lc[4],
+        rhs_limiter=lc[0], bound=lc[1], shockcapture=lc[2], shock_a=lc[3], shock_c=lc[4], cell_beta=lc[5],
         VDM_inv=_colmajor(ops.VDM_inv),
         low_order_surface_flux=1 if isinstance(p.rhs.low_order_surface_flux, LaxFriedrichsOnProjectedVal) else 0)

@@ -45,7 +45,7 @@ class _Config(C.Structure):
                 ("high_order_volume_flux", C.c_int), ("basis", C.c_int), ("entropyproj_limiter", C.c_int),
                 ("rhs_limiter", C.c_int), ("bound", C.c_int), ("shockcapture", C.c_int),
                 ("upper_bound_mode", C.c_int), ("precision", C.c_int),
-                ("shock_a", C.c_double), ("shock_c", C.c_double)]
+                ("shock_a", C.c_double), ("shock_c", C.c_double), ("cell_beta", C.c_double)]
 
 
 class _Operators(C.Structure):
@@ -191,7 +191,7 @@ def _descriptors(fp: FlatProblem, upper_bound_mode: int = None, precision: int =
                   basis=int(getattr(fp, "basis", 0)), entropyproj_limiter=int(getattr(fp, "entropyproj_limiter", 0)),
                   rhs_limiter=int(getattr(fp, "rhs_limiter", 0)), bound=int(getattr(fp, "bound", 0)),
                   shockcapture=int(getattr(fp, "shockcapture", 0)), shock_a=float(getattr(fp, "shock_a", 0.5)),
-                  shock_c=float(getattr(fp, "shock_c", 1.8)))
+                  shock_c=float(getattr(fp, "shock_c", 1.8)), cell_beta=float(getattr(fp, "cell_beta", 0.5)))
     keep = {}
     for n in ("Srsh_db", "Srs0", "Brs", "MinvVhT", "MinvVfT", "Vf", "Vf_low", "wq", "Jq", "GJh", "Ival"):
         keep[n] = np.ascontiguousarray(getattr(fp, n), dtype=np.float64)

@@ -209,20 +209,24 @@ class Param:
         return int(self.K[0]) * int(self.K[1])
 
 
-BOUNDS = (PositivityBound, PositivityAndMinEntropyBound, PositivityAndRelaxedMinEntropyBound)
+BOUNDS = (PositivityBound, PositivityAndMinEntropyBound, PositivityAndRelaxedMinEntropyBound,
+          PositivityAndCellEntropyBound, PositivityAndRelaxedCellEntropyBound)
 
 
 def limiter_codes(lim) -> tuple:
-    """(rhs_limiter, bound, shockcapture, a, c) as the C-ABI enums take them."""
+    """(rhs_limiter, bound, shockcapture, a, c, cell_beta) as the C-ABI enums take them."""
     rl = 1 if isinstance(lim, ZhangShuLimiter) else 0
     bd = 0
+    beta = 0.5
     if isinstance(lim, SubcellLimiter):
-        bd = {PositivityBound: 0, PositivityAndMinEntropyBound: 1,
-              PositivityAndRelaxedMinEntropyBound: 2}.get(type(lim.bound), -1)
+        bd = {PositivityBound: 0, PositivityAndMinEntropyBound: 1, PositivityAndRelaxedMinEntropyBound: 2,
+              PositivityAndCellEntropyBound: 3, PositivityAndRelaxedCellEntropyBound: 4}.get(type(lim.bound), -1)
+        if isinstance(lim.bound, PositivityAndRelaxedCellEntropyBound):
+            beta = float(lim.bound.beta)
     sc = lim.shockcapture
     if isinstance(sc, HennemannShockCapture):
-        return rl, bd, 1, float(sc.a), float(sc.c)
-    return rl, bd, 0, 0.5, 1.8
+        return rl, bd, 1, float(sc.a), float(sc.c), beta
+    return rl, bd, 0, 0.5, 1.8, beta
 
 
 def check_supported(param: Param) -> None:
@@ -261,8 +265,16 @@ def check_supported(param: Param) -> None:
     if not isinstance(lim, (SubcellLimiter, ZhangShuLimiter)):
         raise UnsupportedOption(f"rhs_limiter {lim!r}: SubcellLimiter or ZhangShuLimiter")
     if isinstance(lim, SubcellLimiter) and not isinstance(lim.bound, BOUNDS):
-        raise UnsupportedOption(f"SubcellLimiter bound {lim.bound!r}: PositivityBound, PositivityAndMinEntropyBound "
-                                "or PositivityAndRelaxedMinEntropyBound")
+        raise UnsupportedOption(f"SubcellLimiter bound {lim.bound!r}: PositivityBound, PositivityAndMinEntropyBound, "
+                                "PositivityAndRelaxedMinEntropyBound, PositivityAndCellEntropyBound or "
+                                "PositivityAndRelaxedCellEntropyBound")
+    if isinstance(lim, SubcellLimiter) and isinstance(lim.bound, (PositivityAndCellEntropyBound,
+                                                                  PositivityAndRelaxedCellEntropyBound)):
+        # 1D: the reference sizes the bound's scratch (Nq - N1D) x K = 0 x K and indexes past it
+        # (State.jl:182-183, subcell.jl:503-505); Gauss: its interface step (subcell.jl:745-789)
+        # reads neighbour parameters the neighbour's own step may already have changed
+        if param.equation.Nd != 2 or not isinstance(basis, LobattoCollocation):
+            raise UnsupportedOption("the cell-entropy bounds are implemented for 2D LobattoCollocation")
     if not isinstance(lim.shockcapture, (NoShockCapture, HennemannShockCapture)):
         raise UnsupportedOption(f"shockcapture {lim.shockcapture!r}: NoShockCapture or HennemannShockCapture")
     if not 1 <= param.N <= 4:

@@ -40,7 +40,8 @@ def _create(fp, **cfg_over):
                     POSTOL=fp.POSTOL, ZEROTOL=fp.ZEROTOL, CFL=fp.CFL, dt0=fp.dt0, t0=fp.t0, T=fp.T,
                     high_order_surface_flux=1, basis=fp.basis, entropyproj_limiter=fp.entropyproj_limiter,
                     rhs_limiter=fp.rhs_limiter, bound=fp.bound, shockcapture=fp.shockcapture,
-                    shock_a=fp.shock_a, shock_c=fp.shock_c, low_order_surface_flux=fp.low_order_surface_flux)
+                    shock_a=fp.shock_a, shock_c=fp.shock_c, low_order_surface_flux=fp.low_order_surface_flux,
+                    cell_beta=fp.cell_beta)
     for k, v in cfg_over.items():
         setattr(cfg, k, v)
     keep = {n: None if getattr(fp, n) is None else np.ascontiguousarray(getattr(fp, n), dtype=np.float64)
@@ -78,7 +79,7 @@ def fp2():
 @pytest.mark.parametrize("field,value,code", [
     ("basis", 1, S.P2DE_E_UNSUPPORTED), ("rhs_limiter", 5, S.P2DE_E_UNSUPPORTED),
     ("entropyproj_limiter", 1, S.P2DE_E_UNSUPPORTED), ("shockcapture", 3, S.P2DE_E_UNSUPPORTED),
-    ("bound", 3, S.P2DE_E_UNSUPPORTED),
+    ("bound", 5, S.P2DE_E_UNSUPPORTED),
     ("high_order_surface_flux", 0, S.P2DE_E_UNSUPPORTED), ("precision", 7, S.P2DE_E_INVALID),
     ("upper_bound_mode", 5, S.P2DE_E_INVALID), ("Nd", 3, S.P2DE_E_INVALID), ("N", 9, S.P2DE_E_UNSUPPORTED),
     ("Kx", 0, S.P2DE_E_INVALID), ("gamma", 1.0, S.P2DE_E_INVALID)])
@@ -208,3 +209,14 @@ def test_limiter_options_are_checked():
     _, _, _, me = problem("smoke_vortex", limiter="minentropy")
     st, msg = _create(me, precision=S.PRECISION_MIXED)
     assert st == S.P2DE_E_UNSUPPORTED and "mixed" in msg
+    # the cell-entropy bounds: 2D Lobatto only (the reference cannot run them in 1D, and on Gauss
+    # their interface step depends on the thread schedule); the relaxed one reads the indicator
+    _, _, _, ce = problem("smoke_vortex", limiter="relaxed_cellentropy")
+    _, _, _, g1 = problem("sod_1d", K=8)
+    _, _, _, gg = problem("smoke_vortex", basis="gauss")
+    for fp in (g1, gg):
+        st, msg = _create(fp, bound=3)
+        assert st == S.P2DE_E_UNSUPPORTED and "cell-entropy" in msg
+    ce.VDM_inv = None
+    st, msg = _create(ce)
+    assert st == S.P2DE_E_INVALID and "VDM_inv" in msg

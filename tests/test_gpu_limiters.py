@@ -184,3 +184,105 @@ def test_projected_low_order_flux_parity(case):
         print(f"  step {step}: dt {dt_gpu:.17g} vs {dt_cpu:.17g} state err {err:.2e}", flush=True)
         assert abs(dt_gpu - dt_cpu) <= 1e-13 * dt_cpu
         assert err <= (1e-6 if fp.bound != 0 else 1e-12)
+
+
+# ---------------------------------------------------------------------------------------------
+# cell-entropy bounds (2D Lobatto).  The element's entropy estimate sum(l dvdf) - rhs is formed
+# from terms that the entropy-conservative volume term makes cancel, so with l = 1 it sits at the
+# threshold and rounding decides whether the greedy step runs; it then lowers a parameter by
+# about margin/dvdf, changing sum(l dvdf) by the rounding margin only.  The parity of L is
+# therefore held in that product (|dl| |dvdf| summed per element and direction, against the
+# rounding scale of the sums); rhsU, rhsH and rhsL meet the default bar (measured <= 1.8e-14);
+# and the GPU's own parameters must satisfy the cell entropy inequality wherever the estimate
+# clears the threshold by more than rounding.  Over steps those decisions amplify rounding: the
+# oracle run twice, from states 1e-15 apart, departs by 5.5e-11 after one step of the smoke
+# problem and 2e-9 after two; trajectories are held to 100 times that sensitivity.
+CELL = ["cellentropy", "relaxed_cellentropy"]
+
+
+def _cell_check(dev, orc, fp, U, t, dt, nstage):
+    import test_oracle_limiters as TOL_
+    from p2de_jl_amd.solver import FIELD_L_LOCAL, FIELD_RHSH, FIELD_RHSL
+    dt_gpu, r_gpu = dev.rhs(t, dt, nstage)
+    dt_cpu = orc.rhs(U, t, dt, nstage)
+    Nc, NLs = fp.Nc, fp.Nq + fp.N1D
+    e_h = normwise_rel(dev.field(FIELD_RHSH), orc.field("rhsH"), Nc)
+    e_l = normwise_rel(dev.field(FIELD_RHSL), orc.field("rhsL"), Nc)
+    e_u = normwise_rel(r_gpu, orc.field("rhsU"), Nc)
+    L_gpu = dev.field(FIELD_L_LOCAL).reshape(fp.K, 2, NLs)
+    L_cpu = orc.field("L_local").reshape(3, fp.K, 2, NLs)[nstage - 1]
+    fbH = orc.field("f_bar_H").reshape(2, fp.K, NLs, Nc)
+    fbL = orc.field("f_bar_L").reshape(2, fp.K, NLs, Nc)
+    dvdf, pos, sL, sB, mag = TOL_._cell_entropy_terms(fp, U, fbH, fbL)
+    eps = orc.field("smooth_factor").reshape(3, fp.K)[nstage - 1]
+    worst = 0.0
+    rounding_decided = 0
+    for d in range(2):
+        for k in range(fp.K):
+            lg, lc, dd = L_gpu[k, d, pos[d]], L_cpu[k, d, pos[d]], dvdf[k, d]
+            scale = mag[k, d] + 1e-300
+            worst = max(worst, float(np.sum(np.abs(lg - lc) * np.abs(dd)) / scale))
+            rhs = sB[k, d] - sL[k, d]
+            if fp.bound == 4:
+                rhs *= 1 - fp.cell_beta * eps[k]
+            tol = max(0.0, sL[k, d] - sB[k, d])
+            if np.any(lg != lc):
+                rounding_decided += 1
+            if np.sum(lc * dd) - rhs - tol > 1e-12 * scale:     # the oracle limited for real
+                small = np.sum((lg * dd)[dd < fp.ZEROTOL])
+                assert np.sum(lg * dd) <= max(rhs + tol, small) + 1e-12 * scale
+    print(f"  stage {nstage}: rhsU {e_u:.2e} rhsH {e_h:.2e} rhsL {e_l:.2e} sum|dl dvdf|/scale {worst:.1e} "
+          f"({rounding_decided} element-directions differ) dt {dt_gpu:.17g} vs {dt_cpu:.17g}", flush=True)
+    assert max(e_h, e_l) <= TOL_INF
+    assert worst <= 1e-12
+    assert abs(dt_gpu - dt_cpu) <= 1e-14 * abs(dt_cpu)
+    return e_u
+
+
+@pytest.mark.parametrize("mode", [1, 0], ids=["intended", "reference"])
+@pytest.mark.parametrize("N", [1, 2, 3, 4])
+@pytest.mark.parametrize("limiter", CELL)
+def test_cell_entropy_rhs_parity(limiter, N, mode):
+    import dataclasses
+    from p2de_jl_amd.solver import DeviceSolver
+    solver, state, sp, fp = problem("smoke_vortex", N=N, limiter=limiter)
+    fp = dataclasses.replace(fp, upper_bound_mode=mode)
+    print(f"case smoke_vortex N={N} limiter {limiter} upper_bound_mode {mode}", flush=True)
+    dev = DeviceSolver(fp)
+    dev.set_debug(True)
+    orc = Oracle(fp)
+    U0 = state.preallocation.Uq
+    worst_u = 0.0
+    for U in [perturbed_state(U0, seed=2), hot_node(U0, N)]:
+        dev.set_state(U)
+        for nstage, dt in [(1, 0.9 * fp.CFL * fp.dt0), (2, 0.5 * fp.CFL * fp.dt0), (3, 0.7 * fp.CFL * fp.dt0)]:
+            worst_u = max(worst_u, _cell_check(dev, orc, fp, U, 0.01, dt, nstage))
+    assert worst_u <= TOL_INF
+
+
+@pytest.mark.parametrize("limiter", CELL)
+def test_cell_entropy_steps(limiter):
+    """Two SSP-RK3 steps against the oracle, positivity after every stage; the state to 100 times
+    the oracle's own departure under a 1e-15 relative perturbation of the initial state."""
+    import dataclasses
+    from p2de_jl_amd.solver import DeviceSolver
+    solver, state, sp, fp = problem("smoke_vortex", N=3, limiter=limiter)
+    fp = dataclasses.replace(fp, upper_bound_mode=1)
+    dev = DeviceSolver(fp)
+    orc, twin = Oracle(fp), Oracle(fp)
+    U = state.preallocation.Uq.copy()
+    U2 = U * (1 + 1e-15 * np.random.default_rng(7).uniform(-1, 1, U.shape))
+    dev.set_state(U)
+    t_cpu = t_gpu = t2 = 0.0
+    for step in range(2):
+        t_cpu, dt_cpu = orc.step(U, t_cpu)
+        t2, _ = twin.step(U2, t2)
+        t_gpu, dt_gpu = dev.step(t_gpu)
+        err = normwise_rel(dev.get_state(), U, fp.Nc)
+        sens = normwise_rel(U2, U, fp.Nc)
+        print(f"  {limiter} step {step}: dt {dt_gpu:.17g} vs {dt_cpu:.17g} state err {err:.2e} "
+              f"(oracle sensitivity {sens:.1e})", flush=True)
+        assert abs(dt_gpu - dt_cpu) <= 1e-13 * dt_cpu
+        assert err <= max(1e-12, 100 * sens)
+        rep = dev.check_positivity()
+        assert rep.violations == 0 and rep.min_rho > 0 and rep.min_p > 0

@@ -128,7 +128,8 @@ def test_unsupported_options_are_rejected():
     T = PKG.types
     base = PKG.configs.vortex_2d(K=4)[0]
     from dataclasses import replace
-    for bad in (replace(base, rhs_limiter=T.SubcellLimiter(bound=T.PositivityAndCellEntropyBound())),
+    base1 = PKG.configs.sod_1d(K=4)[0]
+    for bad in (replace(base1, rhs_limiter=T.SubcellLimiter(bound=T.PositivityAndCellEntropyBound())),
                 replace(base, rhs_limiter=T.SubcellLimiter(bound=T.TVDBound())),
                 replace(base, entropyproj_limiter=T.NodewiseScaledExtrapolation()),
                 replace(base, N=5)):
