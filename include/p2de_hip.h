@@ -42,7 +42,13 @@ enum { P2DE_POSITIVITY_BOUND = 0,                          /* PositivityBound */
        P2DE_POSITIVITY_AND_MIN_ENTROPY_BOUND = 1,          /* PositivityAndMinEntropyBound */
        P2DE_POSITIVITY_AND_RELAXED_MIN_ENTROPY_BOUND = 2,  /* PositivityAndRelaxedMinEntropyBound */
        P2DE_POSITIVITY_AND_CELL_ENTROPY_BOUND = 3,         /* PositivityAndCellEntropyBound (2D Lobatto) */
-       P2DE_POSITIVITY_AND_RELAXED_CELL_ENTROPY_BOUND = 4 };/* PositivityAndRelaxedCellEntropyBound(beta) */
+       P2DE_POSITIVITY_AND_RELAXED_CELL_ENTROPY_BOUND = 4, /* PositivityAndRelaxedCellEntropyBound(beta) */
+       /* TVD density bounds (subcell.jl:87-141) with the same entropy parts: code - 5 */
+       P2DE_TVD_BOUND = 5,                                 /* TVDBound */
+       P2DE_TVD_AND_MIN_ENTROPY_BOUND = 6,                 /* TVDAndMinEntropyBound */
+       P2DE_TVD_AND_RELAXED_MIN_ENTROPY_BOUND = 7,         /* TVDAndRelaxedMinEntropyBound */
+       P2DE_TVD_AND_CELL_ENTROPY_BOUND = 8,                /* TVDAndCellEntropyBound (2D Lobatto) */
+       P2DE_TVD_AND_RELAXED_CELL_ENTROPY_BOUND = 9 };      /* TVDAndRelaxedCellEntropyBound(beta) */
 enum { P2DE_NO_SHOCK_CAPTURE = 0, P2DE_HENNEMANN_SHOCK_CAPTURE = 1 };
 /* FP64: the drop-in, matching the reference's FP64 arithmetic.  MIXED (2D only): FP32 flux
  * differencing, surface and low-order fluxes and element assembly; FP64 node pressures,
@@ -75,7 +81,7 @@ typedef struct {
     int upper_bound_mode;         /* P2DE_UPPER_BOUND_* */
     int precision;                /* P2DE_PRECISION_* */
     double shock_a, shock_c;      /* HennemannShockCapture(a, c) (Solver.jl:67-74) */
-    double cell_beta;             /* PositivityAndRelaxedCellEntropyBound(beta) (Solver.jl:52-54) */
+    double cell_beta;             /* (Positivity|TVD)AndRelaxedCellEntropyBound(beta) (Solver.jl:52-64) */
 } p2de_config;
 
 /* Operators{DIM} (Solver.jl:192-212), sizes from SizeData (Solver.jl:173-183). */

@@ -22,7 +22,7 @@ FIELDS = {
     "u_tilde": 8, "QF1": 9, "lambda": 10, "lambdaB": 11, "f_bar_H": 12, "f_bar_L": 13, "vq": 14,
     "psi_tilde": 15, "fstar_H": 16, "fstar_L": 17, "theta_local": 18, "smooth_indicator": 19,
     "blending_factor": 20, "smooth_factor": 21, "lbound_s_modified": 22, "L_zhangshu": 23,
-    "s_modified_min": 24,
+    "s_modified_min": 24, "lbound_rho": 25, "ubound_rho": 26,
 }
 
 _dp = C.POINTER(C.c_double)

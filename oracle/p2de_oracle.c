@@ -216,6 +216,7 @@ struct oracle_ctx {
     double *QF1, *MinvVhTQF1, *MinvVfTBF1;
     /* SubcellLimiterCache (State.jl:131-185) */
     double *f_bar_H, *f_bar_L, *f_bar_lim, *lbound_s_modified, *uL, *L_copy;
+    double *rhoL, *lbound_rho, *ubound_rho;   /* TVD bounds (SubcellLimiterCache State.jl:132,146-147) */
     /* ShockCaptureCache */
     double *blending_factor; /* (K, 3) */
     double *smooth_factor;   /* (K, 3) */
@@ -248,13 +249,13 @@ int oracle_create(const oracle_problem *p, oracle_ctx **out) {
     if (p->Nd != 1 && p->Nd != 2) return 2;
     if (p->Nc != p->Nd + 2) return 2;
     if (p->entropyproj_limiter != 0 && p->entropyproj_limiter != 1) return 2;
-    if (p->rhs_limiter < 0 || p->rhs_limiter > 1 || p->bound < 0 || p->bound > 4 || p->shockcapture < 0 ||
+    if (p->rhs_limiter < 0 || p->rhs_limiter > 1 || p->bound < 0 || p->bound > 9 || p->shockcapture < 0 ||
         p->shockcapture > 1 || p->low_order_surface_flux < 0 || p->low_order_surface_flux > 1)
         return 2;
-    if (!p->VDM_inv && (p->shockcapture != 0 || p->bound == 2 || p->bound == 4)) return 2;
+    if (!p->VDM_inv && (p->shockcapture != 0 || p->bound % 5 == 2 || p->bound % 5 == 4)) return 2;
     /* the cell-entropy bounds size their per-element scratch (Nq - N1D) x K (State.jl:182-183),
      * which is empty in 1D, so the reference indexes out of bounds there (subcell.jl:503-505) */
-    if (p->bound >= 3 && p->Nd != 2) return 2;
+    if (p->bound % 5 >= 3 && p->Nd != 2) return 2;
     oracle_ctx *c = (oracle_ctx *)calloc(1, sizeof(oracle_ctx));
     if (!c) return 3;
     c->p = *p;
@@ -303,6 +304,9 @@ int oracle_create(const oracle_problem *p, oracle_ctx **out) {
     c->f_bar_L = zalloc(Nd * NLs * K * Nc);
     c->f_bar_lim = zalloc(Nd * NLs * K * Nc);
     c->lbound_s_modified = zalloc(Nq * K);
+    c->rhoL = zalloc(Nq * K);
+    c->lbound_rho = zalloc(Nq * K);
+    c->ubound_rho = zalloc(Nq * K);
     c->uL = zalloc(Nq * K * Nc);
     c->blending_factor = zalloc(K * 3);
     c->smooth_factor = zalloc(K * 3);
@@ -335,7 +339,7 @@ void oracle_destroy(oracle_ctx *c) {
                     &c->lbound_s_modified, &c->uL, &c->blending_factor, &c->smooth_factor,
                     &c->dtarr, &c->stage_buf, &c->epUf, &c->epVUf, &c->eprhoef, &c->theta_local,
                     &c->indicator, &c->indicator_modal, &c->smooth_indicator, &c->s_modified, &c->L_zs,
-                    &c->alpha};
+                    &c->alpha, &c->rhoL, &c->lbound_rho, &c->ubound_rho};
     for (size_t i = 0; i < sizeof(f) / sizeof(f[0]); i++) free(*f[i]);
     free(c);
 }
@@ -359,6 +363,7 @@ int64_t oracle_field_size(const oracle_ctx *c, int field) {
     case ORACLE_BLENDING_FACTOR: case ORACLE_SMOOTH_FACTOR: case ORACLE_L_ZHANGSHU: return K * 3;
     case ORACLE_LBOUND_S: return Nq * K;
     case ORACLE_S_MODIFIED_MIN: return 1;
+    case ORACLE_LBOUND_RHO: case ORACLE_UBOUND_RHO: return Nq * K;
     default: return -1;
     }
 }
@@ -390,6 +395,8 @@ const double *oracle_field(const oracle_ctx *c, int field) {
     case ORACLE_LBOUND_S: return c->lbound_s_modified;
     case ORACLE_L_ZHANGSHU: return c->L_zs;
     case ORACLE_S_MODIFIED_MIN: return &c->s_modified_min;
+    case ORACLE_LBOUND_RHO: return c->lbound_rho;
+    case ORACLE_UBOUND_RHO: return c->ubound_rho;
     default: return NULL;
     }
 }
@@ -1009,11 +1016,17 @@ static double limiting_param_bound_phi(const oracle_ctx *c, const double *U, con
 /* limiting_param(::SubcellLimiter, bound) limiter_utils.jl:10-24 with the bound tuple of
  * subcell.jl:231,299 = (Lrho, Lrhoe, Lphi, Urho, Urhoe) and rho_bound(::Positivity...)
  * subcell.jl:370-387 (Lrho = zeta rho_L, Urho = Inf) */
-static double limiting_param(const oracle_ctx *c, const double *uL, const double *P, double Lphi) {
-    double Lrho = c->p.zeta * uL[0];
-    double Urho_i = INFINITY, Urhoe = INFINITY;
+/* limiting_param(::SubcellLimiter, bound) limiter_utils.jl:10-24 with the bound tuple of
+ * subcell.jl:299 (Lrho, zeta*rhoe(u_L), Lphi, Urho, Inf), where (Lrho, Urho) is rho_bound
+ * subcell.jl:350-387: (zeta*rho_L, Inf) for the positivity bounds, the TVD bounds of the node for
+ * the TVD ones (lrho_tvd, urho_tvd) */
+static double limiting_param(const oracle_ctx *c, const double *uL, const double *P, double Lphi, double lrho_tvd,
+                             double urho_tvd) {
+    const int tvd = c->p.bound >= 5, ent = c->p.bound % 5;
+    double Lrho = tvd ? lrho_tvd : c->p.zeta * uL[0];
+    double Urho_i = tvd ? urho_tvd : INFINITY, Urhoe = INFINITY;
     double Lrhoe = c->p.zeta * rhoe_ufun(c->p.Nd, uL);
-    if (c->p.bound == 1 || c->p.bound == 2) {
+    if (ent == 1 || ent == 2) {
         /* min-entropy bounds destructure all five values (limiter_utils.jl:20-22); the
          * cell-entropy bounds take the positivity form (:10-14) */
         double lpos = limiting_param_bound_rho_rhoe(c, uL, P, Lrho, Lrhoe, Urho_i, Urhoe);
@@ -1065,7 +1078,7 @@ double oracle_pt_limiting_param(int Nd, double zeta, double ZEROTOL, int upper_b
     c.p.zeta = zeta;
     c.p.ZEROTOL = ZEROTOL;
     c.p.upper_bound_mode = upper_bound_mode;
-    return limiting_param(&c, uL, P, 0.0);
+    return limiting_param(&c, uL, P, 0.0, 0.0, INFINITY);
 }
 
 #define LLOC(i, d, k, s) (c->L_local[(int64_t)(i) + (int64_t)c->NLs * ((d) + (int64_t)Nd * ((int64_t)(k) + (int64_t)K * (s)))])
@@ -1135,9 +1148,10 @@ static void update_smoothness_factor(oracle_ctx *c, int s) {
     int K = p->K, N = p->N1D - 1;
     for (int k = 0; k < K; k++) {
         double f;
-        if (p->bound == 0 || p->bound == 3) {
+        int ent = p->bound % 5;
+        if (ent == 0 || ent == 3) {
             f = 0.0;
-        } else if (p->bound == 1) {
+        } else if (ent == 1) {
             f = 1.0;
         } else {
             double kappa = 1.0;
@@ -1162,14 +1176,64 @@ static void node_across(const oracle_ctx *c, int i, int k, int m, int *iP, int *
     *kP = (int)(mp / Nfp);
 }
 
+/* low_order_stencil limiter_utils.jl:214-231: the node's neighbours (sl, sr[, sb, st]) along its
+ * lines, across the element boundary by face q2fq[q][direction] (a node on one face only takes its
+ * one face); returns their count */
+static int low_order_stencil(const oracle_ctx *c, int q, int k, int *nodes, int *elems) {
+    const oracle_problem *p = &c->p;
+    int Nq = p->Nq, Nd = p->Nd, N1D = p->N1D;
+    int si = Nd == 1 ? q : q % N1D, sj = Nd == 1 ? 0 : q / N1D;
+    int n = 0;
+    int nfq = p->q2fq[q + (int64_t)Nq * 1] > 0 ? 2 : 1;
+    for (int d = 0; d < Nd; d++) {
+        int pos = d == 0 ? si : sj;
+        for (int side = -1; side <= 1; side += 2) {
+            int np = pos + side;
+            if (np >= 0 && np < N1D) {
+                nodes[n] = d == 0 ? (Nd == 1 ? np : np + N1D * sj) : si + N1D * np;
+                elems[n] = k;
+            } else {
+                node_across(c, q, k, nfq == 1 ? 0 : d, &nodes[n], &elems[n]);
+            }
+            n++;
+        }
+    }
+    return n;
+}
+
+/* initialize_TVD_bounds! subcell.jl:87-141: rho_L = rho + dt*rhsL_rho at every node, then its
+ * minimum and maximum over the node and its low-order stencil */
+static void initialize_tvd_bounds(oracle_ctx *c, const double *Uq, double dt, int nthr) {
+    const oracle_problem *p = &c->p;
+    int K = p->K, Nq = p->Nq, Nc = p->Nc;
+    OMP_FOR
+    for (int k = 0; k < K; k++)
+        for (int i = 0; i < Nq; i++)
+            c->rhoL[i + (int64_t)Nq * k] = Uq[IDX_V(i, k, Nq, Nc)] + dt * c->rhsL[IDX_V(i, k, Nq, Nc)];
+    OMP_FOR
+    for (int k = 0; k < K; k++)
+        for (int q = 0; q < Nq; q++) {
+            int nodes[4], elems[4];
+            int n = low_order_stencil(c, q, k, nodes, elems);
+            double lb = c->rhoL[q + (int64_t)Nq * k], ub = lb;
+            for (int x = 0; x < n; x++) {
+                double r = c->rhoL[nodes[x] + (int64_t)Nq * elems[x]];
+                lb = jmin(lb, r);
+                ub = jmax(ub, r);
+            }
+            c->lbound_rho[q + (int64_t)Nq * k] = lb;
+            c->ubound_rho[q + (int64_t)Nq * k] = ub;
+        }
+}
+
 /* initialize_entropy_bounds! subcell.jl:4-75: s_modified at the nodes, its global minimum at the
  * first stage of t0, and the lower bound epsk*min(stencil) + (1-epsk)*min(s_modified(U0)) with
  * the low-order stencil of low_order_stencil limiter_utils.jl:214-231 */
 static void initialize_entropy_bounds(oracle_ctx *c, const double *Uq, double t, int nstage, int nthr) {
     const oracle_problem *p = &c->p;
-    int K = p->K, Nq = p->Nq, Nc = p->Nc, Nd = p->Nd, N1D = p->N1D;
+    int K = p->K, Nq = p->Nq, Nc = p->Nc, Nd = p->Nd;
     int s = nstage - 1;
-    if (p->bound == 0 || p->bound >= 3) {      /* subcell.jl:4-6 */
+    if (p->bound % 5 == 0 || p->bound % 5 >= 3) {      /* subcell.jl:4-6 */
         memset(c->lbound_s_modified, 0, sizeof(double) * (size_t)Nq * K);
         return;
     }
@@ -1186,24 +1250,8 @@ static void initialize_entropy_bounds(oracle_ctx *c, const double *Uq, double t,
     for (int k = 0; k < K; k++) {
         double epsk = c->smooth_factor[k + (int64_t)K * s];
         for (int q = 0; q < Nq; q++) {
-            int si = Nd == 1 ? q : q % N1D, sj = Nd == 1 ? 0 : q / N1D;
-            /* stencil (sl, sr[, sb, st]); across the element boundary by face q2fq[q][direction],
-             * where a node on one face only takes its one face */
-            int nodes[4], elems[4], n = 0;
-            int nfq = p->q2fq[q + (int64_t)Nq * 1] > 0 ? 2 : 1;
-            for (int d = 0; d < Nd; d++) {
-                int pos = d == 0 ? si : sj;
-                for (int side = -1; side <= 1; side += 2) {
-                    int np = pos + side;
-                    if (np >= 0 && np < N1D) {
-                        nodes[n] = d == 0 ? (Nd == 1 ? np : np + N1D * sj) : si + N1D * np;
-                        elems[n] = k;
-                    } else {
-                        node_across(c, q, k, nfq == 1 ? 0 : d, &nodes[n], &elems[n]);
-                    }
-                    n++;
-                }
-            }
+            int nodes[4], elems[4];
+            int n = low_order_stencil(c, q, k, nodes, elems);
             double lb = c->s_modified[q + (int64_t)Nq * k];
             for (int x = 0; x < n; x++) lb = jmin(lb, c->s_modified[nodes[x] + (int64_t)Nq * elems[x]]);
             c->lbound_s_modified[q + (int64_t)Nq * k] = epsk * lb + (1.0 - epsk) * c->s_modified_min;
@@ -1339,7 +1387,7 @@ static void enforce_es_subcell_volume(oracle_ctx *c, const double *Uq, int nstag
                     }
             }
             double diff = sBpsi[d] - sdvfL[d];
-            double rhs = p->bound == 3 ? diff : (1.0 - p->cell_beta * epsk) * diff;
+            double rhs = p->bound % 5 == 3 ? diff : (1.0 - p->cell_beta * epsk) * diff;
             double est = poslim - rhs;
             double tol = jmax(0.0, sdvfL[d] - sBpsi[d]);
             if (!(est > tol)) continue;
@@ -1378,7 +1426,8 @@ static void apply_rhs_limiter(oracle_ctx *c, const double *Uq, double t, double 
     }
     update_smoothness_factor(c, s);
     initialize_entropy_bounds(c, Uq, t, nstage, nthr);
-    /* initialize_TVD_bounds! for the positivity-type bounds subcell.jl:82-84: nothing */
+    /* initialize_TVD_bounds!: nothing for the positivity-type bounds (subcell.jl:82-84) */
+    if (p->bound >= 5) initialize_tvd_bounds(c, Uq, dt, nthr);
 
     /* accumulate_f_bar! subcell.jl:144-206 */
     OMP_FOR
@@ -1433,6 +1482,8 @@ static void apply_rhs_limiter(oracle_ctx *c, const double *Uq, double t, double 
     }
 
     /* subcell_bound_limiter! subcell.jl:208-246 (1D), :248-349 (2D) */
+#define TVD_LB(q) (c->lbound_rho[(q) + (int64_t)Nq * k])
+#define TVD_UB(q) (c->ubound_rho[(q) + (int64_t)Nq * k])
     for (int64_t x = 0; x < (int64_t)NLs * Nd * K; x++) c->L_local[x + (int64_t)NLs * Nd * K * s] = 1.0;
     OMP_FOR
     for (int k = 0; k < K; k++) {
@@ -1445,13 +1496,13 @@ static void apply_rhs_limiter(oracle_ctx *c, const double *Uq, double t, double 
                 double wJq_i = p->wq[i] * p->Jq[i + (int64_t)Nq * k];
                 double Lphi = c->lbound_s_modified[i + (int64_t)Nq * k];
                 for (int cc = 0; cc < Nc; cc++) P[cc] = -2.0 * dt * (FB(c->f_bar_H, 0, i, k)[cc] - FB(c->f_bar_L, 0, i, k)[cc]) / wJq_i;
-                LLOC(i, 0, k, s) = jmin(LLOC(i, 0, k, s), limiting_param(c, uLk + i * Nc, P, Lphi));
+                LLOC(i, 0, k, s) = jmin(LLOC(i, 0, k, s), limiting_param(c, uLk + i * Nc, P, Lphi, TVD_LB(i), TVD_UB(i)));
             }
             for (int i = 1; i < Nq + 1; i++) {
                 double wJq_im1 = p->wq[i - 1] * p->Jq[(i - 1) + (int64_t)Nq * k];
                 double Lphi = c->lbound_s_modified[(i - 1) + (int64_t)Nq * k];
                 for (int cc = 0; cc < Nc; cc++) P[cc] = 2.0 * dt * (FB(c->f_bar_H, 0, i, k)[cc] - FB(c->f_bar_L, 0, i, k)[cc]) / wJq_im1;
-                LLOC(i, 0, k, s) = jmin(LLOC(i, 0, k, s), limiting_param(c, uLk + (i - 1) * Nc, P, Lphi));
+                LLOC(i, 0, k, s) = jmin(LLOC(i, 0, k, s), limiting_param(c, uLk + (i - 1) * Nc, P, Lphi, TVD_LB((i - 1)), TVD_UB((i - 1))));
             }
             double l_shock = c->blending_factor[k + (int64_t)K * s];
             for (int i = 0; i < NLs; i++) LLOC(i, 0, k, s) = jmin(LLOC(i, 0, k, s), l_shock);
@@ -1465,7 +1516,7 @@ static void apply_rhs_limiter(oracle_ctx *c, const double *Uq, double t, double 
                 double Lphi = c->lbound_s_modified[q + (int64_t)Nq * k];
                 for (int cc = 0; cc < Nc; cc++) P[cc] = -4.0 * dt * (FB(c->f_bar_H, 0, si + N1Dp1 * sj, k)[cc] - FB(c->f_bar_L, 0, si + N1Dp1 * sj, k)[cc]) / wJq_i;
                 int li = si + N1Dp1 * sj;
-                LLOC(li, 0, k, s) = jmin(LLOC(li, 0, k, s), limiting_param(c, uLk + q * Nc, P, Lphi));
+                LLOC(li, 0, k, s) = jmin(LLOC(li, 0, k, s), limiting_param(c, uLk + q * Nc, P, Lphi, TVD_LB(q), TVD_UB(q)));
             }
             for (int si = 1; si < N1Dp1; si++) {         /* right subcell faces */
                 int q = (si - 1) + N1D * sj;
@@ -1473,7 +1524,7 @@ static void apply_rhs_limiter(oracle_ctx *c, const double *Uq, double t, double 
                 double Lphi = c->lbound_s_modified[q + (int64_t)Nq * k];
                 for (int cc = 0; cc < Nc; cc++) P[cc] = 4.0 * dt * (FB(c->f_bar_H, 0, si + N1Dp1 * sj, k)[cc] - FB(c->f_bar_L, 0, si + N1Dp1 * sj, k)[cc]) / wJq_i;
                 int li = si + N1Dp1 * sj;
-                LLOC(li, 0, k, s) = jmin(LLOC(li, 0, k, s), limiting_param(c, uLk + q * Nc, P, Lphi));
+                LLOC(li, 0, k, s) = jmin(LLOC(li, 0, k, s), limiting_param(c, uLk + q * Nc, P, Lphi, TVD_LB(q), TVD_UB(q)));
             }
         }
         for (int si = 0; si < N1D; si++) {
@@ -1483,7 +1534,7 @@ static void apply_rhs_limiter(oracle_ctx *c, const double *Uq, double t, double 
                 double Lphi = c->lbound_s_modified[q + (int64_t)Nq * k];
                 for (int cc = 0; cc < Nc; cc++) P[cc] = -4.0 * dt * (FB(c->f_bar_H, 1, si + N1D * sj, k)[cc] - FB(c->f_bar_L, 1, si + N1D * sj, k)[cc]) / wJq_i;
                 int li = si + N1D * sj;
-                LLOC(li, 1, k, s) = jmin(LLOC(li, 1, k, s), limiting_param(c, uLk + q * Nc, P, Lphi));
+                LLOC(li, 1, k, s) = jmin(LLOC(li, 1, k, s), limiting_param(c, uLk + q * Nc, P, Lphi, TVD_LB(q), TVD_UB(q)));
             }
             for (int sj = 1; sj < N1Dp1; sj++) {         /* top subcell faces */
                 int q = si + N1D * (sj - 1);
@@ -1491,7 +1542,7 @@ static void apply_rhs_limiter(oracle_ctx *c, const double *Uq, double t, double 
                 double Lphi = c->lbound_s_modified[q + (int64_t)Nq * k];
                 for (int cc = 0; cc < Nc; cc++) P[cc] = 4.0 * dt * (FB(c->f_bar_H, 1, si + N1D * sj, k)[cc] - FB(c->f_bar_L, 1, si + N1D * sj, k)[cc]) / wJq_i;
                 int li = si + N1D * sj;
-                LLOC(li, 1, k, s) = jmin(LLOC(li, 1, k, s), limiting_param(c, uLk + q * Nc, P, Lphi));
+                LLOC(li, 1, k, s) = jmin(LLOC(li, 1, k, s), limiting_param(c, uLk + q * Nc, P, Lphi, TVD_LB(q), TVD_UB(q)));
             }
         }
         double l_shock = c->blending_factor[k + (int64_t)K * s];
@@ -1501,10 +1552,12 @@ static void apply_rhs_limiter(oracle_ctx *c, const double *Uq, double t, double 
         }
     }
 
+#undef TVD_LB
+#undef TVD_UB
     /* enforce_ES_subcell! subcell.jl:458-466: nothing for the positivity and min-entropy bounds;
      * the cell-entropy bounds limit the interior subcell faces further (the interface step,
      * :745-789, is the Gauss basis's; for Lobatto the interface fluxes coincide) */
-    if (p->bound >= 3) enforce_es_subcell_volume(c, Uq, nstage, nthr);
+    if (p->bound % 5 >= 3) enforce_es_subcell_volume(c, Uq, nstage, nthr);
 
     /* symmetrize_limiting_parameters! */
     if (Nd == 1) {

@@ -48,7 +48,9 @@ typedef struct {
     /* SubcellLimiter bound: 0 PositivityBound, 1 PositivityAndMinEntropyBound,
      * 2 PositivityAndRelaxedMinEntropyBound (subcell.jl:4-75, 927-956, limiter_utils.jl:19-50),
      * 3 PositivityAndCellEntropyBound, 4 PositivityAndRelaxedCellEntropyBound(cell_beta)
-     * (subcell.jl:458-823; 2D) */
+     * (subcell.jl:458-823; 2D); 5..9 the same entropy parts with TVD density bounds instead of
+     * positivity ones: TVDBound, TVDAndMinEntropyBound, TVDAndRelaxedMinEntropyBound,
+     * TVDAndCellEntropyBound, TVDAndRelaxedCellEntropyBound (subcell.jl:87-141, 350-368) */
     int bound;
     /* shockcapture: 0 NoShockCapture, 1 HennemannShockCapture(shock_a, shock_c)
      * (shock_capture.jl:10-132) */
@@ -116,6 +118,8 @@ enum {
     ORACLE_LBOUND_S,         /* (Nq, K) lower bound of the modified specific entropy */
     ORACLE_L_ZHANGSHU,       /* (K, 3) Zhang-Shu limiting parameters */
     ORACLE_S_MODIFIED_MIN,   /* (1) */
+    ORACLE_LBOUND_RHO,       /* (Nq, K) TVD density bounds */
+    ORACLE_UBOUND_RHO,
     ORACLE_NFIELDS
 };
 

@@ -42,7 +42,8 @@ from .types import (CompressibleEulerIdealGas, Dim1, Dim2, ESLimitedLowOrderPos,
                     NodewiseScaledExtrapolation, NoEntropyProjectionLimiter, Param,
                     PositivityAndCellEntropyBound, PositivityAndMinEntropyBound,
                     PositivityAndRelaxedCellEntropyBound, PositivityAndRelaxedMinEntropyBound, PositivityBound,
-                    SubcellLimiter, TimesteppingParameter, ZhangShuLimiter)
+                    SubcellLimiter, TimesteppingParameter, TVDAndCellEntropyBound, TVDAndMinEntropyBound,
+                    TVDAndRelaxedCellEntropyBound, TVDAndRelaxedMinEntropyBound, TVDBound, ZhangShuLimiter)
 
 # approximation basis and entropy-projection limiter by name
 BASES = {
@@ -61,6 +62,12 @@ LIMITERS = {
     "relaxed_minentropy": lambda: SubcellLimiter(bound=PositivityAndRelaxedMinEntropyBound()),
     "cellentropy": lambda: SubcellLimiter(bound=PositivityAndCellEntropyBound()),
     "relaxed_cellentropy": lambda: SubcellLimiter(bound=PositivityAndRelaxedCellEntropyBound(beta=0.5)),
+    # TVD density bounds (subcell.jl:87-141), not in the smoke test's list
+    "tvd": lambda: SubcellLimiter(bound=TVDBound()),
+    "tvd_minentropy": lambda: SubcellLimiter(bound=TVDAndMinEntropyBound()),
+    "tvd_relaxed_minentropy": lambda: SubcellLimiter(bound=TVDAndRelaxedMinEntropyBound()),
+    "tvd_cellentropy": lambda: SubcellLimiter(bound=TVDAndCellEntropyBound()),
+    "tvd_relaxed_cellentropy": lambda: SubcellLimiter(bound=TVDAndRelaxedCellEntropyBound(beta=0.5)),
 }
 
 # dt0 = 0.9*dt_pos(U0)/CFL, recorded from the initial states (tests/golden/record_dt0.py)

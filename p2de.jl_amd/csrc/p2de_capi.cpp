@@ -52,7 +52,8 @@ struct p2de_ctx {
     bool ep_limit = false;                 // NodewiseScaledExtrapolation
     int optA = OPT_SUBCELL;                // element-kernel form for the limiter options
     double *d_blend = nullptr, *d_lphi = nullptr, *d_spart = nullptr;   // limiter option buffers
-    double* d_sfac = nullptr;              // smoothness factor per element (relaxed cell-entropy bound)
+    double* d_sfac = nullptr;              // smoothness factor per element (relaxed cell-entropy bounds)
+    double* d_rhoL = nullptr;              // rho + dt*rhsL_rho per node (TVD bounds)
     double* d_dtp = nullptr;               // per-workgroup CFL minima of phase A (stage 1)
     bool ut_valid = false;                 // d_ut matches d_U[0]
     double *d_dbg_rhsH = nullptr, *d_dbg_rhsL = nullptr, *d_dbg_L = nullptr;
@@ -130,7 +131,7 @@ void free_ctx(p2de_ctx* c) {
     void* dptrs[] = {c->d_tab, c->d_geo, c->d_nb_node, c->d_nb_face, c->d_bc, c->d_Ival, c->d_U[0], c->d_U[1],
                      c->d_rhsUA, c->d_lA, c->d_dfA, c->d_rhsU, c->d_ut, c->d_theta, c->d_dtp, c->d_dbg_rhsH,
                      c->d_dbg_rhsL, c->d_dbg_L, c->d_blend, c->d_lphi, c->d_spart,
-                     c->d_sc, c->d_keys, c->d_Lz, c->d_dtlog, c->d_sfac};
+                     c->d_sc, c->d_keys, c->d_Lz, c->d_dtlog, c->d_sfac, c->d_rhoL};
     for (void* p : dptrs)
         if (p) (void)hipFree(p);
     if (c->h_sc) (void)hipHostFree(c->h_sc);
@@ -242,8 +243,8 @@ int p2de_create(const p2de_config* cfg, const p2de_operators* ops, const p2de_ge
         c->basis = cfg->basis;
         c->ep_limit = cfg->entropyproj_limiter == P2DE_NODEWISE_SCALED_EXTRAPOLATION;
         const bool shock = cfg->shockcapture == P2DE_HENNEMANN_SHOCK_CAPTURE;
-        if ((shock || cfg->bound == P2DE_POSITIVITY_AND_RELAXED_MIN_ENTROPY_BOUND ||
-             cfg->bound == P2DE_POSITIVITY_AND_RELAXED_CELL_ENTROPY_BOUND) && !ops->VDM_inv)
+        const int ent = cfg->bound % 5;
+        if ((shock || ent == 2 || ent == 4) && !ops->VDM_inv)
             return fail(c, P2DE_E_INVALID, "the smoothness indicator needs VDM_inv"), bail(P2DE_E_INVALID);
         c->optA = cfg->rhs_limiter == P2DE_ZHANG_SHU_LIMITER                        ? OPT_ZHANG_SHU
                   : (shock || cfg->bound != P2DE_POSITIVITY_BOUND) ? OPT_SUBCELL_GENERAL
@@ -273,8 +274,8 @@ int p2de_create(const p2de_config* cfg, const p2de_operators* ops, const p2de_ge
             (st = dalloc(c, &c->d_sc, 1)) || (st = dalloc(c, &c->d_keys, 3)))
             return bail(st);
         if (c->optA == OPT_ZHANG_SHU && (st = dalloc(c, &c->d_Lz, (size_t)3 * K))) return bail(st);
-        if (cfg->bound == P2DE_POSITIVITY_AND_RELAXED_CELL_ENTROPY_BOUND && (st = dalloc(c, &c->d_sfac, (size_t)K)))
-            return bail(st);
+        if (cfg->bound % 5 == 4 && (st = dalloc(c, &c->d_sfac, (size_t)K))) return bail(st);
+        if (cfg->bound >= P2DE_TVD_BOUND && (st = dalloc(c, &c->d_rhoL, (size_t)K * Nq))) return bail(st);
         if (hipHostMalloc((void**)&c->h_sc, sizeof(StepScalars)) != hipSuccess ||
             hipHostMalloc((void**)&c->h_keys, 3 * sizeof(unsigned long long)) != hipSuccess)
             return fail(c, P2DE_E_ALLOC, "hipHostMalloc failed"), bail(P2DE_E_ALLOC);
@@ -386,11 +387,11 @@ PhaseAArgs argsA(p2de_ctx* c, const double* U, int nstage) {
     a.bound = c->cfg.bound;
     a.POSTOL = c->cfg.POSTOL;
     a.blend = c->cfg.shockcapture == P2DE_HENNEMANN_SHOCK_CAPTURE ? c->d_blend : nullptr;
-    const bool min_entropy = c->cfg.bound == P2DE_POSITIVITY_AND_MIN_ENTROPY_BOUND ||
-                             c->cfg.bound == P2DE_POSITIVITY_AND_RELAXED_MIN_ENTROPY_BOUND;
+    const bool min_entropy = c->cfg.bound % 5 == 1 || c->cfg.bound % 5 == 2;
     a.lphi = min_entropy ? c->d_lphi : nullptr;
     a.sfac = c->d_sfac;
     a.cell_beta = c->cfg.cell_beta;
+    a.rhoL = c->d_rhoL;
     a.lo_proj = c->cfg.low_order_surface_flux == P2DE_LF_ON_PROJECTED_VAL;
     if (c->debug) {
         a.dbg_rhsH = c->d_dbg_rhsH;
@@ -467,11 +468,11 @@ int project_faces(p2de_ctx* c, const double* U, int nstage) {
 int limiter_prep(p2de_ctx* c, const double* U, int nstage, const double* tp, double t_value) {
     const p2de_config& g = c->cfg;
     const bool shock = g.shockcapture == P2DE_HENNEMANN_SHOCK_CAPTURE;
-    const bool min_entropy =
-        g.bound == P2DE_POSITIVITY_AND_MIN_ENTROPY_BOUND || g.bound == P2DE_POSITIVITY_AND_RELAXED_MIN_ENTROPY_BOUND;
-    // the plain cell-entropy bound reads neither the indicator nor entropy lower bounds
-    if (!shock && (g.bound == P2DE_POSITIVITY_BOUND || g.bound == P2DE_POSITIVITY_AND_CELL_ENTROPY_BOUND))
-        return P2DE_OK;
+    const int ent = g.bound % 5;
+    const bool min_entropy = ent == 1 || ent == 2;
+    // no entropy bound and the plain cell-entropy bound read neither the indicator nor entropy
+    // lower bounds
+    if (!shock && (ent == 0 || ent == 3)) return P2DE_OK;
     if (min_entropy && nstage == 1)
         LAUNCHCHK(c, timed(c, 2, [&] {
                       return launch_s_min(U, (int64_t)c->K * c->Nq, c->Nd, g.gamma, tp, t_value, g.t0, c->d_spart, c->d_sc,
@@ -503,6 +504,14 @@ int stage_rhs(p2de_ctx* c, const double* U, int nstage, const double* tp, double
     if (st) return st;
     if ((st = limiter_prep(c, U, nstage, tp, t_value))) return st;
     c->last_stage = nstage;
+    if (c->cfg.bound >= P2DE_TVD_BOUND) {
+        // TVD bounds: rho + dt*rhsL of every node first, since a node's bounds take the neighbour
+        // element's nodes across the face (initialize_TVD_bounds! subcell.jl:112-141)
+        PhaseAArgs pre = argsA(c, U, nstage);
+        pre.tvd_pre = 1;
+        pre.rhoL_out = c->d_rhoL;
+        LAUNCHCHK(c, timed(c, 0, [&] { return launch_phaseA(c->Nd, c->N1D, c->basis, c->variantA, c->optA, pre, c->stream); }));
+    }
     LAUNCHCHK(c, timed(c, 0, [&] {
                   return launch_phaseA(c->Nd, c->N1D, c->basis, c->variantA, c->optA, argsA(c, U, nstage), c->stream);
               }));

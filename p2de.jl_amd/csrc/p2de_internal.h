@@ -128,9 +128,14 @@ struct PhaseAArgs {
     double* zsL;                  // Zhang-Shu: L[:, nstage] of this stage, [K] (else nullptr)
     const double* sfac;           // smoothness factor per element (relaxed cell-entropy bound), [K]
     double cell_beta;             // PositivityAndRelaxedCellEntropyBound(beta)
+    // TVD bounds: the first launch (tvd_pre) writes rho_L = rho + dt*rhsL_rho of every node to
+    // rhoL_out and stops; the second reads the node stencils' extrema from rhoL
+    const double* rhoL;
+    double* rhoL_out;
+    int tvd_pre;
     double* dtp;                  // stage 1: per-workgroup minimum of the CFL time step
     // limiter options (element-kernel variants OPT 1 and 2)
-    int bound;                    // P2DE_POSITIVITY*_BOUND
+    int bound;                    // P2DE_*_BOUND: entropy part bound % 5, TVD density bounds when >= 5
     double POSTOL;
     const double* blend;          // per element blending factor of the shock capture (nullptr: 1)
     const double* lphi;           // per node lower bound of the modified specific entropy

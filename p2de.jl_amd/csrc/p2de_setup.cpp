@@ -36,16 +36,17 @@ int setup_config(const p2de_config* cfg, int* Nq, int* Nfp, int* Nh, int* K, std
         return fail_s(err, P2DE_E_UNSUPPORTED, "option set not implemented");
     if (cfg->rhs_limiter != P2DE_SUBCELL_LIMITER && cfg->rhs_limiter != P2DE_ZHANG_SHU_LIMITER)
         return fail_s(err, P2DE_E_UNSUPPORTED, "rhs limiter not implemented");
-    if (cfg->bound < P2DE_POSITIVITY_BOUND || cfg->bound > P2DE_POSITIVITY_AND_RELAXED_CELL_ENTROPY_BOUND)
-        return fail_s(err, P2DE_E_UNSUPPORTED, "limiter bound not implemented (TVD bounds)");
+    if (cfg->bound < P2DE_POSITIVITY_BOUND || cfg->bound > P2DE_TVD_AND_RELAXED_CELL_ENTROPY_BOUND)
+        return fail_s(err, P2DE_E_UNSUPPORTED, "limiter bound not implemented");
+    const int ent = cfg->bound % 5;   // entropy part: none, min, relaxed min, cell, relaxed cell
     // the cell-entropy bounds: in 1D the reference sizes their per-element scratch (Nq - N1D) x K,
     // empty, and indexes past it (State.jl:182-183, subcell.jl:503-505); on the Gauss basis their
     // interface step (subcell.jl:745-789) reads neighbour parameters that the neighbour's own step
     // may already have lowered, so its result depends on the thread schedule
-    if (cfg->bound >= P2DE_POSITIVITY_AND_CELL_ENTROPY_BOUND && (cfg->Nd != 2 || cfg->basis != P2DE_BASIS_LOBATTO))
+    if (ent >= 3 && (cfg->Nd != 2 || cfg->basis != P2DE_BASIS_LOBATTO))
         return fail_s(err, P2DE_E_UNSUPPORTED, "the cell-entropy bounds are implemented for 2D Lobatto");
-    if (cfg->bound == P2DE_POSITIVITY_AND_RELAXED_CELL_ENTROPY_BOUND && !(cfg->cell_beta == cfg->cell_beta))
-        return fail_s(err, P2DE_E_INVALID, "PositivityAndRelaxedCellEntropyBound needs a beta");
+    if (ent == 4 && !(cfg->cell_beta == cfg->cell_beta))
+        return fail_s(err, P2DE_E_INVALID, "the relaxed cell-entropy bound needs a beta");
     // bound(::ZhangShuLimiter) is PositivityBound (traits.jl:61-63)
     if (cfg->rhs_limiter == P2DE_ZHANG_SHU_LIMITER && cfg->bound != P2DE_POSITIVITY_BOUND)
         return fail_s(err, P2DE_E_UNSUPPORTED, "the Zhang-Shu limiter takes no subcell bound");

@@ -142,16 +142,16 @@ __device__ __forceinline__ double s_modified(double g, const double* U) {
 }
 
 // limiting_param(::SubcellLimiter, <min-entropy bounds>) (limiter_utils.jl:19-24) with the bound
-// tuple (zeta*rho_L, zeta*rhoe(u_L), Lphi, Inf, Inf): the positivity parameter lpos, then
+// tuple (Lrho, zeta*rhoe(u_L), Lphi, Urho, Inf) -- (Lrho, Urho) from rho_bound (subcell.jl:350-387):
+// (zeta*rho_L, Inf), or the node's TVD bounds -- the positivity parameter lpos, then
 // limiting_param_bound_phi (:42-50), the bisection (nonlinear_solvers.jl:3-20) between 0 and lpos
 // for s_modified(u_L + l P) >= Lphi - POSTOL
 template <int DIM>
-__device__ __noinline__ double limiting_param_phi(const double* uL, const double* P, double zeta, double ZEROTOL,
-                                                  double Lphi, double POSTOL, double g) {
+__device__ __noinline__ double limiting_param_phi(const double* uL, const double* P, double Lrho, double Urho,
+                                                  double zeta, double ZEROTOL, double Lphi, double POSTOL, double g) {
     constexpr int NC = DIM + 2;
-    const double Lrho = zeta * uL[0];
     const double Lrhoe = zeta * Eq<DIM>::rhoe(uL);
-    const double lpos = bound_rho_rhoe<DIM>(uL, P, Lrho, Lrhoe, INFINITY, INFINITY, ZEROTOL);
+    const double lpos = bound_rho_rhoe<DIM>(uL, P, Lrho, Lrhoe, Urho, INFINITY, ZEROTOL);
     auto ok = [&](double l) {
         double Ul[NC];
         for (int c = 0; c < NC; c++) Ul[c] = uL[c] + l * P[c];
@@ -225,7 +225,7 @@ __device__ __noinline__ int es_subcell_volume(int d, double* L, const double* fH
             poslim += L[pos[idx]] * dd[idx];
         }
     const double diff = sB - sL;
-    const double rhs = bound == 3 ? diff : (1.0 - beta * epsk) * diff;
+    const double rhs = bound % 5 == 3 ? diff : (1.0 - beta * epsk) * diff;
     const double est = poslim - rhs;
     const double tol = jmax(0.0, sL - sB);
     if (!(est > tol)) return 0;
@@ -606,6 +606,9 @@ __global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
         }
         // u_L = Uq + dt*rhsL (subcell.jl:269)
         for (int c = 0; c < NC; c++) sm.a.g2.UL[e][i][c] = sm.U[e][i][c] + dt_lim * rhsL[c];
+        // TVD bounds' first pass: rho_L = rho + dt*rhsL_rho of every node (initialize_TVD_bounds!
+        // subcell.jl:112-118), the same bits the limiter's u_L holds in the second pass
+        if (OPT == OPT_SUBCELL_GENERAL && a.tvd_pre) a.rhoL_out[(size_t)k * NQ + i] = sm.a.g2.UL[e][i][0];
         for (int d = 0; d < DIM; d++)
             for (int c = 0; c < NC; c++) {
                 sm.a.g2.RH[e][i][d][c] = rH[d][c];
@@ -620,6 +623,7 @@ __global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
             }
         }
     }
+    if (OPT == OPT_SUBCELL_GENERAL && a.tvd_pre) return;   // uniform over the workgroup
     if (a.stage1) {
         // block min, then one global atomic min on the (positive) bit pattern
         for (int off = 32; off > 0; off >>= 1) {
@@ -727,8 +731,26 @@ __global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
         for (int c = 0; c < NC; c++) dfb[c] = sm.x.fb.fH[e][r][c] - sm.x.fb.fL[e][r][c];
         // the bound's limiting_param (limiter_utils.jl:10-24); Lphi is the node's lower entropy bound
         auto lim = [&](int q) {
-            if (OPT == OPT_SUBCELL_GENERAL && (a.bound == 1 || a.bound == 2))
-                return limiting_param_phi<DIM>(sm.a.g2.UL[e][q], P, zeta, ZEROTOL, a.lphi[(size_t)k * NQ + q], a.POSTOL, g);
+            if (OPT == OPT_SUBCELL_GENERAL && (a.bound != 0 && a.bound != 3 && a.bound != 4)) {
+                const double* uL = sm.a.g2.UL[e][q];
+                const int ent = a.bound % 5;
+                double Lrho = zeta * uL[0], Urho = INFINITY;
+                if (a.bound >= 5) {   // rho_bound(::TVD...) (subcell.jl:350-368): the stencil's extrema of rho_L
+                    Lrho = Urho = a.rhoL[(size_t)k * NQ + q];
+                    for (int x = 0; x < 2 * DIM; x++) {
+                        const int st = T.stn[q][x];
+                        const double r = st >= 0 ? a.rhoL[(size_t)k * NQ + st] : a.rhoL[a.nb_node[(size_t)k * NFP + (-st - 1)]];
+                        Lrho = jmin(Lrho, r);
+                        Urho = jmax(Urho, r);
+                    }
+                }
+                if (ent == 1 || ent == 2)
+                    return limiting_param_phi<DIM>(uL, P, Lrho, Urho, zeta, ZEROTOL, a.lphi[(size_t)k * NQ + q], a.POSTOL, g);
+                const double Lrhoe = zeta * Eq<DIM>::rhoe(uL);
+                // the 4-of-5 destructuring (limiter_utils.jl:11) takes (Lrho, Lrhoe, Lphi = 0, Urho)
+                return a.upper_mode == 0 ? bound_rho_rhoe<DIM>(uL, P, Lrho, Lrhoe, 0.0, Urho, ZEROTOL)
+                                         : bound_rho_rhoe<DIM>(uL, P, Lrho, Lrhoe, Urho, INFINITY, ZEROTOL);
+            }
             return limiting_param<DIM>(sm.a.g2.UL[e][q], P, zeta, ZEROTOL, a.upper_mode);
         };
         double l = 1.0;
@@ -766,7 +788,7 @@ __global__ __launch_bounds__(NT) void phaseA_kernel(PhaseAArgs a) {
     // ---- phase 5b': cell-entropy bounds (enforce_ES_subcell! subcell.jl:462-466; the interface step
     //      :745-789 is the Gauss basis's, for Lobatto the interface fluxes coincide)
     if constexpr (DIM == 2 && OPT == OPT_SUBCELL_GENERAL) {
-        if (a.bound >= 3) {
+        if (a.bound % 5 >= 3) {
             for (int w = tid; w < E * NQ; w += NT) {
                 const int e = w / NQ, i = w - e * NQ;
                 if (k0 + e >= K) continue;
@@ -844,19 +866,20 @@ __global__ __launch_bounds__(NT) void limiter_prep_kernel(PrepArgs a) {
     const int nel = min(E, a.K - k0);
     const ElemTables& T = *a.T;
     const double g = a.gamma;
-    const bool need_ind = a.shock != 0 || a.bound == 2 || a.bound == 4;
+    const int ent = a.bound % 5;   // the entropy part of the bound (TVD bounds add 5)
+    const bool need_ind = a.shock != 0 || ent == 2 || ent == 4;
     for (int w = tid; w < nel * NQ; w += NT) {
         const int e = w / NQ, i = w - e * NQ;
         const double* src = a.U + ((size_t)(k0 + e) * NQ + i) * NC;
         double u[NC];
         for (int c = 0; c < NC; c++) u[c] = src[c];
-        if (a.bound != 0) sS[e][i] = s_modified<DIM>(g, u);
+        if (ent == 1 || ent == 2) sS[e][i] = s_modified<DIM>(g, u);
         if (need_ind) sI[e][i] = u[0] * Eq<DIM>::p(g, u);   // initialize_indicator! :82-94
     }
     __syncthreads();
     for (int e = tid; e < nel; e += NT) {
         const int k = k0 + e;
-        double f = a.bound == 1 ? 1.0 : 0.0;
+        double f = ent == 1 ? 1.0 : 0.0;
         if (need_ind) {
             double mod[NQ];
             for (int i = 0; i < NQ; i++) mod[i] = 0.0;
@@ -881,7 +904,7 @@ __global__ __launch_bounds__(NT) void limiter_prep_kernel(PrepArgs a) {
                 const double alpha = 1.0 / (1.0 + exp(-s_factor / TN * (ind - TN)));
                 a.blend[k] = jmax(jmin(1.0 - alpha, 1.0), alphamax);
             }
-            if (a.bound == 2 || a.bound == 4) {
+            if (ent == 2 || ent == 4) {
                 const double kappa = 1.0;
                 const double s0 = log(pow((double)N, -4.0)) / log(10.0);
                 const double sk = log(ind) / log(10.0);
@@ -894,7 +917,7 @@ __global__ __launch_bounds__(NT) void limiter_prep_kernel(PrepArgs a) {
         if (a.sfac) a.sfac[k] = f;
     }
     __syncthreads();
-    if (a.bound == 0 || a.bound >= 3) return;
+    if (ent == 0 || ent >= 3) return;
     for (int w = tid; w < nel * NQ; w += NT) {
         const int e = w / NQ, q = w - e * NQ;
         const int k = k0 + e;

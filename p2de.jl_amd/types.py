@@ -126,6 +126,26 @@ class TVDBound:
 
 
 @dataclass(frozen=True)
+class TVDAndMinEntropyBound:
+    pass
+
+
+@dataclass(frozen=True)
+class TVDAndRelaxedMinEntropyBound:
+    pass
+
+
+@dataclass(frozen=True)
+class TVDAndCellEntropyBound:
+    pass
+
+
+@dataclass(frozen=True)
+class TVDAndRelaxedCellEntropyBound:
+    beta: float = 0.5
+
+
+@dataclass(frozen=True)
 class NoShockCapture:
     pass
 
@@ -209,8 +229,15 @@ class Param:
         return int(self.K[0]) * int(self.K[1])
 
 
+# SubcellLimiter bounds in the order of their C-ABI codes (P2DE_*_BOUND): positivity or TVD density
+# bounds, each with no entropy bound, the min-entropy bound, its relaxed form, the cell-entropy
+# bound or its relaxed form
 BOUNDS = (PositivityBound, PositivityAndMinEntropyBound, PositivityAndRelaxedMinEntropyBound,
-          PositivityAndCellEntropyBound, PositivityAndRelaxedCellEntropyBound)
+          PositivityAndCellEntropyBound, PositivityAndRelaxedCellEntropyBound,
+          TVDBound, TVDAndMinEntropyBound, TVDAndRelaxedMinEntropyBound,
+          TVDAndCellEntropyBound, TVDAndRelaxedCellEntropyBound)
+CELL_ENTROPY_BOUNDS = (PositivityAndCellEntropyBound, PositivityAndRelaxedCellEntropyBound,
+                       TVDAndCellEntropyBound, TVDAndRelaxedCellEntropyBound)
 
 
 def limiter_codes(lim) -> tuple:
@@ -219,9 +246,8 @@ def limiter_codes(lim) -> tuple:
     bd = 0
     beta = 0.5
     if isinstance(lim, SubcellLimiter):
-        bd = {PositivityBound: 0, PositivityAndMinEntropyBound: 1, PositivityAndRelaxedMinEntropyBound: 2,
-              PositivityAndCellEntropyBound: 3, PositivityAndRelaxedCellEntropyBound: 4}.get(type(lim.bound), -1)
-        if isinstance(lim.bound, PositivityAndRelaxedCellEntropyBound):
+        bd = BOUNDS.index(type(lim.bound)) if type(lim.bound) in BOUNDS else -1
+        if isinstance(lim.bound, (PositivityAndRelaxedCellEntropyBound, TVDAndRelaxedCellEntropyBound)):
             beta = float(lim.bound.beta)
     sc = lim.shockcapture
     if isinstance(sc, HennemannShockCapture):
@@ -265,11 +291,8 @@ def check_supported(param: Param) -> None:
     if not isinstance(lim, (SubcellLimiter, ZhangShuLimiter)):
         raise UnsupportedOption(f"rhs_limiter {lim!r}: SubcellLimiter or ZhangShuLimiter")
     if isinstance(lim, SubcellLimiter) and not isinstance(lim.bound, BOUNDS):
-        raise UnsupportedOption(f"SubcellLimiter bound {lim.bound!r}: PositivityBound, PositivityAndMinEntropyBound, "
-                                "PositivityAndRelaxedMinEntropyBound, PositivityAndCellEntropyBound or "
-                                "PositivityAndRelaxedCellEntropyBound")
-    if isinstance(lim, SubcellLimiter) and isinstance(lim.bound, (PositivityAndCellEntropyBound,
-                                                                  PositivityAndRelaxedCellEntropyBound)):
+        raise UnsupportedOption(f"SubcellLimiter bound {lim.bound!r}: " + ", ".join(b.__name__ for b in BOUNDS))
+    if isinstance(lim, SubcellLimiter) and isinstance(lim.bound, CELL_ENTROPY_BOUNDS):
         # 1D: the reference sizes the bound's scratch (Nq - N1D) x K = 0 x K and indexes past it
         # (State.jl:182-183, subcell.jl:503-505); Gauss: its interface step (subcell.jl:745-789)
         # reads neighbour parameters the neighbour's own step may already have changed

@@ -79,7 +79,7 @@ def fp2():
 @pytest.mark.parametrize("field,value,code", [
     ("basis", 1, S.P2DE_E_UNSUPPORTED), ("rhs_limiter", 5, S.P2DE_E_UNSUPPORTED),
     ("entropyproj_limiter", 1, S.P2DE_E_UNSUPPORTED), ("shockcapture", 3, S.P2DE_E_UNSUPPORTED),
-    ("bound", 5, S.P2DE_E_UNSUPPORTED),
+    ("bound", 10, S.P2DE_E_UNSUPPORTED),
     ("high_order_surface_flux", 0, S.P2DE_E_UNSUPPORTED), ("precision", 7, S.P2DE_E_INVALID),
     ("upper_bound_mode", 5, S.P2DE_E_INVALID), ("Nd", 3, S.P2DE_E_INVALID), ("N", 9, S.P2DE_E_UNSUPPORTED),
     ("Kx", 0, S.P2DE_E_INVALID), ("gamma", 1.0, S.P2DE_E_INVALID)])
@@ -215,8 +215,9 @@ def test_limiter_options_are_checked():
     _, _, _, g1 = problem("sod_1d", K=8)
     _, _, _, gg = problem("smoke_vortex", basis="gauss")
     for fp in (g1, gg):
-        st, msg = _create(fp, bound=3)
-        assert st == S.P2DE_E_UNSUPPORTED and "cell-entropy" in msg
+        for bound in (3, 4, 8, 9):
+            st, msg = _create(fp, bound=bound)
+            assert st == S.P2DE_E_UNSUPPORTED and "cell-entropy" in msg
     ce.VDM_inv = None
     st, msg = _create(ce)
     assert st == S.P2DE_E_INVALID and "VDM_inv" in msg

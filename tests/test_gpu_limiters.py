@@ -46,12 +46,27 @@ def _s_mod(U):
     return rhoe * U[..., 0] ** -1.4
 
 
+def flux_weighted_mismatch(orc, L_gpu, fp, nstage):
+    """Largest |l_gpu - l_cpu| |fbar_H - fbar_L| over subcell faces, relative to the potentials'
+    size: the change of the limited flux.  The TVD bounds are tight (the node's own rho_L is one of
+    the values its bounds range over), so they limit faces whose jump fbar_H - fbar_L is small, and
+    l = (bound - rho_L)/P inherits the jump's relative rounding; the limited flux l*jump does not."""
+    NLs = fp.Nq + fp.N1D
+    L_cpu = orc.field("L_local").reshape(3, fp.K, fp.Nd, NLs)[nstage - 1]
+    L_gpu = L_gpu.reshape(fp.K, fp.Nd, NLs)
+    fH = orc.field("f_bar_H").reshape(fp.Nd, fp.K, NLs, fp.Nc).transpose(1, 0, 2, 3)
+    fL = orc.field("f_bar_L").reshape(fp.Nd, fp.K, NLs, fp.Nc).transpose(1, 0, 2, 3)
+    jump = np.max(np.abs(fH - fL), axis=-1)
+    scale = np.max(np.abs(fH)) + np.max(np.abs(fL))
+    return float(np.max(np.abs(L_gpu - L_cpu) * jump) / scale)
+
+
 def check(dev, orc, fp, U, t, dt, nstage, zs, smooth=False):
     from p2de_jl_amd.solver import FIELD_L_LOCAL, FIELD_RHSH, FIELD_RHSL
     dt_gpu, r_gpu = dev.rhs(t, dt, nstage)
     dt_cpu = orc.rhs(U, t, dt, nstage)
     Nc = fp.Nc
-    entropy = fp.bound != 0 and smooth
+    entropy = fp.bound % 5 in (1, 2) and smooth          # the min-entropy parts
     e_u = normwise_rel(r_gpu, orc.field("rhsU"), Nc)
     e_h = normwise_rel(dev.field(FIELD_RHSH), orc.field("rhsH"), Nc)
     e_l = normwise_rel(dev.field(FIELD_RHSL), orc.field("rhsL"), Nc)
@@ -74,6 +89,9 @@ def check(dev, orc, fp, U, t, dt, nstage, zs, smooth=False):
         L_gpu = dev.field(FIELD_L_LOCAL)[:fp.K]
         e_L = float(np.max(np.abs(L_gpu - L_cpu)))
         nlim = int(np.sum(L_cpu < 1))
+    elif fp.bound >= 5:
+        e_L = flux_weighted_mismatch(orc, dev.field(FIELD_L_LOCAL), fp, nstage)
+        nlim = int(np.sum(orc.field("L_local").reshape(3, -1)[nstage - 1] < 1))
     else:
         e_L = limiter_mismatch(orc, dev.field(FIELD_L_LOCAL), fp, nstage)
         nlim = int(np.sum(orc.field("L_local").reshape(3, -1)[nstage - 1] < 1))
@@ -84,6 +102,13 @@ def check(dev, orc, fp, U, t, dt, nstage, zs, smooth=False):
     if not entropy:
         assert e_L <= 1e-12
     assert abs(dt_gpu - dt_cpu) <= 1e-14 * abs(dt_cpu)
+    if fp.bound >= 5 and nstage > 1:
+        # the local maximum principle of the TVD bounds on the GPU's own update
+        lb = orc.field("lbound_rho").reshape(fp.K, fp.Nq)
+        ub = orc.field("ubound_rho").reshape(fp.K, fp.Nq)
+        rho_new = U[..., 0] + dt * r_gpu.reshape(U.shape)[..., 0]
+        slack = 1e-13 * np.max(np.abs(rho_new))
+        assert np.all(rho_new >= lb - slack) and np.all(rho_new <= ub + slack)
 
 
 @pytest.mark.parametrize("N", [1, 2, 3, 4])
@@ -284,5 +309,65 @@ def test_cell_entropy_steps(limiter):
               f"(oracle sensitivity {sens:.1e})", flush=True)
         assert abs(dt_gpu - dt_cpu) <= 1e-13 * dt_cpu
         assert err <= max(1e-12, 100 * sens)
+        rep = dev.check_positivity()
+        assert rep.violations == 0 and rep.min_rho > 0 and rep.min_p > 0
+
+
+# ---------------------------------------------------------------------------------------------
+# TVD density bounds (subcell.jl:87-141): a first element-kernel launch writes rho + dt*rhsL of
+# every node, the second takes each node's bounds from its low-order stencil across the element
+# faces.  The default bar (1e-6 on smooth states for the min-entropy parts) with L held in the
+# limited flux (flux_weighted_mismatch), both upper-bound destructurings, and the local maximum
+# principle on the GPU's own update.
+TVD = ["tvd", "tvd_minentropy", "tvd_relaxed_minentropy"]
+
+
+@pytest.mark.parametrize("mode", [0, 1], ids=["reference", "intended"])
+@pytest.mark.parametrize("N", [1, 2, 3, 4])
+@pytest.mark.parametrize("limiter", TVD)
+def test_tvd_rhs_parity(limiter, N, mode):
+    import dataclasses
+    from p2de_jl_amd.solver import DeviceSolver
+    solver, state, sp, fp = problem("smoke_vortex", N=N, limiter=limiter)
+    fp = dataclasses.replace(fp, upper_bound_mode=mode)
+    print(f"case smoke_vortex N={N} limiter {limiter} upper_bound_mode {mode}", flush=True)
+    dev = DeviceSolver(fp)
+    dev.set_debug(True)
+    orc = Oracle(fp)
+    U0 = state.preallocation.Uq
+    dev.set_state(U0)
+    check(dev, orc, fp, U0, 0.0, 0.9 * fp.CFL * fp.dt0, 1, False, smooth=True)
+    for trial, U in enumerate([perturbed_state(U0, seed=2, amp=0.3), hot_node(U0, N)]):
+        dev.set_state(U)
+        for nstage, dt in [(1, 0.9 * fp.CFL * fp.dt0), (2, 0.5 * fp.CFL * fp.dt0), (3, 0.7 * fp.CFL * fp.dt0)]:
+            check(dev, orc, fp, U, 0.01, dt, nstage, False, smooth=trial == 1)
+
+
+@pytest.mark.parametrize("case", [("tvd", "lobatto", "smoke_vortex"), ("tvd", "gauss", "smoke_vortex"),
+                                  ("tvd_minentropy", "gauss", "smoke_vortex"), ("tvd", "lobatto", "sod_1d"),
+                                  ("tvd_minentropy", "gauss", "sod_1d"), ("tvd_cellentropy", "lobatto", "smoke_vortex")],
+                         ids=lambda c: "-".join(c))
+def test_tvd_steps_parity(case):
+    from p2de_jl_amd.solver import DeviceSolver
+    limiter, basis, name = case
+    kw = dict(K=16) if name == "sod_1d" else dict(T=0.05)
+    solver, state, sp, fp = problem(name, N=3, limiter=limiter, basis=basis, **kw)
+    print(f"case {name} N=3 limiter {limiter} basis {basis}", flush=True)
+    dev = DeviceSolver(fp)
+    orc, twin = Oracle(fp), Oracle(fp)
+    U = state.preallocation.Uq.copy()
+    U2 = U * (1 + 1e-15 * np.random.default_rng(7).uniform(-1, 1, U.shape))
+    dev.set_state(U)
+    t_cpu = t_gpu = t2 = 0.0
+    for step in range(3):
+        t_cpu, dt_cpu = orc.step(U, t_cpu)
+        t2, _ = twin.step(U2, t2)
+        t_gpu, dt_gpu = dev.step(t_gpu)
+        err = normwise_rel(dev.get_state(), U, fp.Nc)
+        sens = normwise_rel(U2, U, fp.Nc)
+        print(f"  step {step}: dt {dt_gpu:.17g} vs {dt_cpu:.17g} state err {err:.2e} (oracle sensitivity {sens:.1e})",
+              flush=True)
+        assert abs(dt_gpu - dt_cpu) <= 1e-13 * dt_cpu
+        assert err <= max(1e-12 if fp.bound % 5 in (0, 3) else 1e-6, 100 * sens)
         rep = dev.check_positivity()
         assert rep.violations == 0 and rep.min_rho > 0 and rep.min_p > 0

@@ -130,7 +130,7 @@ def test_unsupported_options_are_rejected():
     from dataclasses import replace
     base1 = PKG.configs.sod_1d(K=4)[0]
     for bad in (replace(base1, rhs_limiter=T.SubcellLimiter(bound=T.PositivityAndCellEntropyBound())),
-                replace(base, rhs_limiter=T.SubcellLimiter(bound=T.TVDBound())),
+                replace(base1, rhs_limiter=T.SubcellLimiter(bound=T.TVDAndCellEntropyBound())),
                 replace(base, entropyproj_limiter=T.NodewiseScaledExtrapolation()),
                 replace(base, N=5)):
         with pytest.raises(T.UnsupportedOption):

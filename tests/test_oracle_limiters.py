@@ -22,6 +22,10 @@ so they are pinned by what the definitions fix:
   inequality the bound enforces, with dvdf, 1'B psi and sum dv.fbar_L formed here from the
   entropy variables and the oracle's subcell flux potentials), the changed faces are those of
   largest dvdf and at most one of them keeps l > 0; elsewhere nothing changes;
+* TVD bounds: the density bounds are the minimum and maximum of rho + dt*rhsL over the node and
+  its low-order neighbours (found by coordinates), and the limited update keeps the density of
+  every node within them (the local maximum principle the bound is for), with either
+  upper-bound destructuring;
 * all set-ups conserve and stay positive through the reference's smoke problem.
 """
 import dataclasses
@@ -132,8 +136,8 @@ def test_zhang_shu_bounds_maximality_and_blend(N):
     assert viol.any()
 
 
-def _coordinate_stencil_min(s, fp, svals):
-    """min of s over each node and its four (two in 1D) low-order neighbours, the neighbours
+def _coordinate_stencil_min(s, fp, svals, op=min):
+    """min (or op) of s over each node and its four low-order neighbours, the neighbours
     located by coordinates: the adjacent node of the same line, or at an element boundary the
     neighbouring element's node at the same point (periodic box)."""
     x, y = s.md.xq.T, s.md.yq.T                      # (K, Nq)
@@ -165,7 +169,7 @@ def _coordinate_stencil_min(s, fp, svals):
                                 (d == 1 and abs(ddx) < 1e-9 and side * ddy > 0)]
                         assert len(pick) == 1
                         vals.append(svals[pick[0]])
-            out[k, q] = min(vals)
+            out[k, q] = op(vals)
     return out
 
 
@@ -323,8 +327,50 @@ def _check_cell_entropy(fp, fp0, U, limiter):
     return changed_elems
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("limiter", ["tvd", "tvd_minentropy", "tvd_relaxed_cellentropy"])
+@pytest.mark.parametrize("N", [2, 3])
+def test_tvd_bounds_and_local_maximum_principle(limiter, N, mode):
+    from conftest import perturbed_state
+    s, U0, fp = _setup(N=N, limiter=limiter)
+    fp = dataclasses.replace(fp, upper_bound_mode=mode)
+    orc = Oracle(fp)
+    for U in (perturbed_state(U0, seed=4, amp=0.3), U0):
+        for nstage, dt in ((2, 0.5 * fp.CFL * fp.dt0), (3, 0.9 * fp.CFL * fp.dt0)):
+            orc.rhs(U, 0.01, dt, nstage)
+            rhoL = U[..., 0] + dt * orc.field("rhsL").reshape(U.shape)[..., 0]
+            lb = orc.field("lbound_rho").reshape(fp.K, fp.Nq)
+            ub = orc.field("ubound_rho").reshape(fp.K, fp.Nq)
+            assert np.array_equal(lb, _coordinate_stencil_min(s, fp, rhoL, min))
+            assert np.array_equal(ub, _coordinate_stencil_min(s, fp, rhoL, max))
+            rho_new = U[..., 0] + dt * orc.field("rhsU").reshape(U.shape)[..., 0]
+            slack = 1e-13 * np.max(np.abs(rho_new))
+            assert np.all(rho_new >= lb - slack) and np.all(rho_new <= ub + slack)
+            assert np.any(lb < ub)
+
+
+@pytest.mark.parametrize("limiter", ["tvd", "tvd_minentropy"])
+def test_tvd_bounds_in_1d(limiter):
+    """Sod (1D, transmissive): the TVD bounds reach across element faces, and the density stays
+    within them."""
+    s, U, fp = _setup("sod_1d", K=20, N=3, limiter=limiter)
+    fp = dataclasses.replace(fp, upper_bound_mode=1)
+    orc = Oracle(fp)
+    dt = 0.5 * fp.CFL * fp.dt0
+    orc.rhs(U, 0.01, dt, 2)
+    lb = orc.field("lbound_rho").reshape(fp.K, fp.Nq)
+    ub = orc.field("ubound_rho").reshape(fp.K, fp.Nq)
+    rho_new = U[..., 0] + dt * orc.field("rhsU").reshape(U.shape)[..., 0]
+    assert np.all(rho_new >= lb - 1e-14) and np.all(rho_new <= ub + 1e-14)
+    t = 0.0
+    while t < 0.02:
+        t, _ = orc.step(U, t)
+    assert orc.count_nonpositive(U)[0] == 0
+
+
 @pytest.mark.parametrize("limiter", ["zhangshu", "zhangshu_shock", "subcell_shock", "minentropy", "relaxed_minentropy",
-                                     "cellentropy", "relaxed_cellentropy"])
+                                     "cellentropy", "relaxed_cellentropy", "tvd", "tvd_minentropy",
+                                     "tvd_relaxed_minentropy", "tvd_cellentropy", "tvd_relaxed_cellentropy"])
 @pytest.mark.parametrize("N", [1, 2, 3, 4])
 def test_smoke_setups_conserve_and_stay_positive(limiter, N):
     """The reference's smoke problem (test/test_smoke.jl:44-66) to T = 2e-2 with each set-up."""
