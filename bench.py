@@ -316,9 +316,11 @@ def main():
                            "update_kernel_ms_per_step": ms[1] / steps_t,
                            "positivity_violations": int(rep.violations), "min_p": rep.min_p}
         cmp["gauss_fp64"] = gauss
-        # the other limiter set-ups of test/test_smoke.jl:44-52, FP64 drop-in, four-state Riemann 256x256 N=3
+        # the other limiter set-ups of test/test_smoke.jl:44-52 and the TVD bounds, FP64 drop-in,
+        # four-state Riemann 256x256 N=3
         lims = {}
-        for lim in ("zhangshu", "zhangshu_shock", "subcell_shock", "minentropy", "relaxed_minentropy"):
+        for lim in ("zhangshu", "zhangshu_shock", "subcell_shock", "minentropy", "relaxed_minentropy",
+                    "cellentropy", "relaxed_cellentropy", "tvd", "tvd_minentropy"):
             solver, state, sp = pkg.configs.build("riemann4_2d", limiter=lim)
             fp = pkg.flat_problem(solver, sp)
             U = state.preallocation.Uq
