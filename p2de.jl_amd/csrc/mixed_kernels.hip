@@ -783,20 +783,25 @@ __global__ __launch_bounds__(NT) void mix_dt_kernel(MixArgs a) {
 }
 
 // per instance: step start (SSPRK33.jl:30) or RHS-evaluation set-up
+// SSPRK33.jl:28-30: done once t >= T, else the step's dt cap min(CFL*dt0, T - t)
+__device__ __forceinline__ void step_start(const MixInst& I, StepScalars* sc) {
+    const double t = sc->t;
+    if (!(t < I.T)) {
+        sc->done = 1;
+        return;
+    }
+    sc->done = 0;
+    const double dt = jmin(I.CFL * I.dt0, I.T - t);
+    sc->dt_lim = dt;
+    sc->dt_bits = d_to_bits(dt);
+}
+
 __global__ void mix_init_kernel(const MixInst* inst, int n, int mode, int only, double dt_lim, double dt_cap) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     StepScalars* sc = inst[g].sc;
     if (mode == 0) {
-        const double t = sc->t;
-        if (!(t < inst[g].T)) {
-            sc->done = 1;
-            return;
-        }
-        sc->done = 0;
-        const double dt = jmin(inst[g].CFL * inst[g].dt0, inst[g].T - t);
-        sc->dt_lim = dt;
-        sc->dt_bits = d_to_bits(dt);
+        step_start(inst[g], sc);
     } else if (g == only) {
         sc->done = 0;
         sc->dt_lim = dt_lim;
@@ -813,8 +818,21 @@ __global__ __launch_bounds__(256) void mix_dtred_kernel(const MixInst* inst, int
     if (i >= n || (only >= 0 && i != only)) return;
     const MixInst& I = inst[i];
     if (I.sc->done) return;
-    double m = INFINITY;
-    for (int t = I.tile_lo + (int)threadIdx.x; t < I.tile_hi; t += 256) m = dtp[t] < m ? dtp[t] : m;
+    // four independent chains, so that the loads of a thread are in flight together (a NaN
+    // minimum is skipped in every chain, as the single chain skipped it)
+    double m0 = INFINITY, m1 = INFINITY, m2 = INFINITY, m3 = INFINITY;
+    int t = I.tile_lo + (int)threadIdx.x;
+    for (; t + 768 < I.tile_hi; t += 1024) {
+        const double a = dtp[t], b = dtp[t + 256], c = dtp[t + 512], d = dtp[t + 768];
+        m0 = a < m0 ? a : m0;
+        m1 = b < m1 ? b : m1;
+        m2 = c < m2 ? c : m2;
+        m3 = d < m3 ? d : m3;
+    }
+    for (; t < I.tile_hi; t += 256) m0 = dtp[t] < m0 ? dtp[t] : m0;
+    m1 = m1 < m0 ? m1 : m0;
+    m3 = m3 < m2 ? m3 : m2;
+    double m = m3 < m1 ? m3 : m1;
     for (int off = 32; off > 0; off >>= 1) {
         const double o = __shfl_xor(m, off, 64);
         m = o < m ? o : m;
@@ -829,13 +847,18 @@ __global__ __launch_bounds__(256) void mix_dtred_kernel(const MixInst* inst, int
     }
 }
 
+// the end of a step (t += dt, SSPRK33.jl:42) and the start of the next one, in one launch
 __global__ void mix_post_kernel(const MixInst* inst, int n) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     StepScalars* sc = inst[g].sc;
-    if (sc->done) return;
-    sc->t = sc->t + bits_to_d(sc->dt_bits);
-    sc->steps = sc->steps + 1;
+    if (!sc->done) {
+        const double dt = bits_to_d(sc->dt_bits);
+        sc->t = sc->t + dt;
+        sc->steps = sc->steps + 1;
+        sc->last_dt = dt;
+    }
+    step_start(inst[g], sc);
 }
 
 __device__ __forceinline__ unsigned long long order_key_m(double v) {
@@ -873,11 +896,12 @@ struct MixVariant {
     int NT, E[MIX_MAX_N1D + 1];
     const void* stage;
     const void* dt;
+    int NTdt;            // the CFL pass has one work item per node: at most 128 threads
 };
 
-#define MV(NT, E2, E3, E4, E5)                                                            \
-    {NT, {0, 0, E2, E3, E4, E5}, (const void*)&mix_stage_kernel<NT, E2, E3, E4, E5>,     \
-     (const void*)&mix_dt_kernel<NT, E2, E3, E4, E5>}
+#define MV(NT, E2, E3, E4, E5)                                                                    \
+    {NT, {0, 0, E2, E3, E4, E5}, (const void*)&mix_stage_kernel<NT, E2, E3, E4, E5>,             \
+     (const void*)&mix_dt_kernel<(NT > 128 ? 128 : NT), E2, E3, E4, E5>, (NT > 128 ? 128 : NT)}
 
 // single-degree shapes first (one degree per problem), then all-degree shapes for batches
 const MixVariant kMix[] = {
@@ -905,7 +929,7 @@ int mix_blocks_per_cu(int v, int* stage_blocks, int* dt_blocks) {
     if (v < 0 || v >= kNumMix) return (int)hipErrorInvalidValue;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(stage_blocks, kMix[v].stage, kMix[v].NT, 0);
     if (e != hipSuccess) return (int)e;
-    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(dt_blocks, kMix[v].dt, kMix[v].NT, 0);
+    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(dt_blocks, kMix[v].dt, kMix[v].NTdt, 0);
 }
 
 int mix_launch_stage(int v, const MixArgs& a, int grid, void* stream) {
@@ -919,7 +943,7 @@ int mix_launch_dt(int v, const MixArgs& a, int grid, void* stream) {
     if (v < 0 || v >= kNumMix || grid < 1) return (int)hipErrorInvalidValue;
     MixArgs arg = a;
     void* params[] = {&arg};
-    return (int)hipLaunchKernel(kMix[v].dt, dim3(grid), dim3(kMix[v].NT), params, 0, (hipStream_t)stream);
+    return (int)hipLaunchKernel(kMix[v].dt, dim3(grid), dim3(kMix[v].NTdt), params, 0, (hipStream_t)stream);
 }
 
 int mix_launch_init(const MixInst* inst, int n, int mode, int only, double dt_lim, double dt_cap, void* stream) {

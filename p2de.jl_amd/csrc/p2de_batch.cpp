@@ -384,12 +384,14 @@ MixArgs margs(p2de_batch* b, int mode, int only, const double* Uin, const double
     return a;
 }
 
-// One SSP-RK3 step of every instance (SSPRK33.jl:29-43): A holds U^n, B and C the stages.
-int launch_step(p2de_batch* b) {
+// One SSP-RK3 step of every instance (SSPRK33.jl:29-43): A holds U^n, B and C the stages.  The
+// last launch also starts the next step (done flag, dt cap), so the first step of a sequence
+// needs `with_init`; repeating the start is harmless.
+int launch_step(p2de_batch* b, bool with_init) {
     void* s = b->stream;
     const int v = b->variants[b->vsel];
     double *A = b->d_U[0], *B = b->d_U[1], *C = b->d_U[2];
-    BLAUNCH(b, timed(b, 2, [&] { return mix_launch_init(b->d_inst, b->n, 0, -1, 0.0, 0.0, s); }));
+    if (with_init) BLAUNCH(b, timed(b, 2, [&] { return mix_launch_init(b->d_inst, b->n, 0, -1, 0.0, 0.0, s); }));
     BLAUNCH(b, timed(b, 1, [&] { return mix_launch_dt(v, margs(b, MODE_STAGE1, -1, A, A, nullptr), b->grid_dt, s); }));
     BLAUNCH(b, timed(b, 2, [&] { return mix_launch_dt_reduce(b->d_inst, b->n, b->d_dtp, -1, s); }));
     BLAUNCH(b, timed(b, 0, [&] { return mix_launch_stage(v, margs(b, MODE_STAGE1, -1, A, A, B), b->grid_stage, s); }));
@@ -404,7 +406,7 @@ int ensure_graph(p2de_batch* b) {
     BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     const bool timing = b->timing;
     b->timing = false;
-    int st = launch_step(b);
+    int st = launch_step(b, false);
     b->timing = timing;
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(b->stream, &g);
@@ -422,11 +424,12 @@ int enqueue(p2de_batch* b, int steps) {
     if (b->use_graph && !b->timing) {
         int st = ensure_graph(b);
         if (st) return st;
+        if (steps > 0) BLAUNCH(b, mix_launch_init(b->d_inst, b->n, 0, -1, 0.0, 0.0, b->stream));
         for (int i = 0; i < steps; i++) BHIP(b, hipGraphLaunch(b->gexec, b->stream));
         return P2DE_OK;
     }
     for (int i = 0; i < steps; i++) {
-        int st = launch_step(b);
+        int st = launch_step(b, true);
         if (st) return st;
     }
     return P2DE_OK;
@@ -472,7 +475,7 @@ int batch_instance_stats(p2de_batch* b, int inst, double* t, double* last_dt, in
     if (st) return st;
     const StepScalars& S = b->h_sc[inst];
     if (t) *t = S.t;
-    if (last_dt) *last_dt = h_bits_to_double(S.dt_bits);
+    if (last_dt) *last_dt = S.last_dt;
     if (steps) *steps = S.steps;
     if (viol) *viol = (int64_t)S.viol;
     return P2DE_OK;
@@ -489,7 +492,7 @@ int batch_instance_restore(p2de_batch* b, int inst, double t, double last_dt, in
     S.done = 0;
     S.hold = 0;
     S.steps = steps;
-    std::memcpy(&S.dt_bits, &last_dt, sizeof(double));
+    S.last_dt = last_dt;
     return write_sc(b);
 }
 
@@ -695,7 +698,7 @@ int p2de_batch_get_time(p2de_batch* b, int inst, double* t, double* last_dt, int
     int st = read_sc(b);
     if (st) return st;
     if (t) *t = b->h_sc[inst].t;
-    if (last_dt) *last_dt = h_bits_to_double(b->h_sc[inst].dt_bits);
+    if (last_dt) *last_dt = b->h_sc[inst].last_dt;
     if (steps) *steps = b->h_sc[inst].steps;
     return P2DE_OK;
 }

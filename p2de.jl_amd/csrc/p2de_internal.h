@@ -102,6 +102,7 @@ struct StepScalars {
     unsigned long long viol;      // positivity violations
     unsigned long long first_bad; // min over (stage<<40 | node) of offending nodes
     double s_min;                 // minimum modified specific entropy of the state at t0 (min-entropy bounds)
+    double last_dt;               // the last step's dt (batched path: dt_bits already holds the next step's cap)
 };
 
 struct PhaseAArgs {
