@@ -133,6 +133,11 @@ __device__ __forceinline__ double lim_param(const double* uL, double Lrho, doubl
     const double a = P[0] * P[3] - 0.5 * (P[1] * P[1] + P[2] * P[2]);
     const double b = uL[3] * P[0] + uL[0] * P[3] - uL[1] * P[1] - uL[2] * P[2] - P[0] * Lrhoe;
     const double c = uL[3] * uL[0] - 0.5 * (uL[1] * uL[1] + uL[2] * uL[2]) - uL[0] * Lrhoe;
+    // g(x) = a x^2 + b x + c is rho * (rhoe - Lrhoe) along uL + x P.  With g(0) > 0 and g(l) > 0,
+    // no root lies in (0, l] when g is concave (its one positive root is beyond l) or its vertex
+    // -b/(2a) is outside (0, l): the root selection below then leaves l unchanged, so the
+    // square root and the division are skipped.
+    if (c > 0.0 && fma(fma(a, l, b), l, c) > 0.0 && (a <= 0.0 || b >= 0.0 || -b >= 2.0 * a * l)) return l;
     const double disc = b * b - 4.0 * a * c;
     // a == 0 gives non-finite roots that fail every test of rhoe_quadratic_solve: l stays 1
     if (disc >= 0.0 && a != 0.0) {
@@ -159,24 +164,22 @@ struct SmemM {
     double geo[E][4];            // J, rxJ, syJ, 1/J
     float BF[4][MF];             // B*f* - LF (high order == low order at interior faces)
     float BFL[4][MF];            // low-order surface term (differs at inflow faces)
-    float RH[2][4][M];           // rhsxyH
-    float RL[2][4][M];           // rhsxyL
+    // subcell flux potentials, accumulated in FP64 and stored once rounded: the limiter (5b)
+    // and the limited update (5c) read the same values, so the update is the convex
+    // combination the limiter bounded
+    float fL[4][MB];             // low-order potentials
+    float df[4][MB];             // f_bar_H - f_bar_L
     union {
-        struct {
+        struct {                 // phases 1-4
             float np[8][M];      // NodeP fields
             float PH[4][MV];
             float PL[4][ML];
         } a;
-        struct {
-            // subcell flux potentials, accumulated in FP64 and stored once rounded: the
-            // limiter (5b) and the limited update (5c) read the same values, so the update is
-            // the convex combination the limiter bounded
-            float fL[4][MB];     // low-order potentials
-            float df[4][MB];     // f_bar_H - f_bar_L
-            float ls[2][MB];     // limiting parameter seen from the node above/right (0) and below/left (1)
+        struct {                 // phases 5u-5c
             double UL[4][M];
             double Lrhoe[M];
             double iwJ[M];       // 1 / (wq J)
+            float ls[2][MB];     // limiting parameter seen from the node above/right (0) and below/left (1)
         } b;
     } x;
 };
@@ -230,8 +233,21 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
     const double* Uin = a.Uin + I.node_off * 4;
     StepScalars* sc = I.sc;
     const int mN = nel * NQ;      // valid nodes of the tile
+    // the step's dt and the limiter's dt (stage 1 limits with the input dt min(CFL*dt0, T - t),
+    // not the CFL one) are fixed during the launch: read them before the first barrier
+    const double dt = bits_to_d(sc->dt_bits);
+    const double dt_lim = (a.mode == MODE_STAGE2 || a.mode == MODE_STAGE3) ? dt : sc->dt_lim;
+    const double zeta = I.zeta;
 
-    // ---- 0: element states (one node per lane, 32 contiguous bytes) and metric -> LDS
+    // ---- 0: element states (one node per lane, 32 contiguous bytes) and metric -> LDS.  The
+    //      face table entries of the lane's first face node are loaded here too: phase 2 then
+    //      waits for one load (the neighbour's state) instead of two dependent ones.
+    int f_bc = 0, f_nb = 0;
+    if (tid < nel * NFP) {
+        const size_t fi = (size_t)k0 * NFP + tid;
+        f_bc = I.bc[fi];
+        f_nb = I.nb_node[fi];
+    }
     {
         const double4* src = (const double4*)(Uin + (size_t)k0 * NQ * 4);
 #pragma unroll 1
@@ -270,8 +286,8 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
         const int nq = e * NQ + T.fq[f];
         const NodeP P = ld_np(sm, nq);
         const size_t fi = (size_t)k * NFP + f;
-        const int bcc = I.bc[fi];
-        const int nb = I.nb_node[fi];
+        const int bcc = w == tid ? f_bc : I.bc[fi];
+        const int nb = w == tid ? f_nb : I.nb_node[fi];
         const double4 up = ((const double4*)Uin)[nb];
         const double UP[4] = {up.x, up.y, up.z, up.w};
         const NodeP PP = node_prims(gd, UP);
@@ -343,10 +359,61 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
     }
     __syncthreads();
 
-    // ---- 4: node assembly (assemble_rhs!, flux_differencing.jl:331-361; low-order surface and
-    //      mass scaling, low_order_graph_viscosity.jl:175-220)
+    // ---- 4/5a: element assembly along lines (assemble_rhs!, flux_differencing.jl:331-361; the
+    //      low-order line and surface terms, low_order_graph_viscosity.jl:175-220) fused with the
+    //      subcell flux potentials (accumulate_f_bar!, subcell.jl:163-206).  One work item per
+    //      (line, component): node q of a line in direction d contributes
+    //      wq J rhsxyH[d] = -wq hx and wq J rhsxyL[d] = lx, where hx, lx are its assembled sums
+    //      in that direction (rhsxyH = -hx / J, rhsxyL = lx / (wq J)); the increments are
+    //      summed in FP64 and the potentials stored rounded once.  Items are numbered over the
+    //      tile's capacity E, so that the component index is a compile-time division; the lines
+    //      of absent elements are skipped.
+    constexpr int NLN = E * 2 * N1D;
 #pragma unroll 1
-    for (int n = tid; n < mN; n += NT) {
+    for (int w = tid; w < 4 * NLN; w += NT) {
+        const int c = w / NLN, wl = w - c * NLN;
+        const int e = wl / (2 * N1D), L = wl - e * (2 * N1D);
+        if (e >= nel) continue;
+        const int d = L / N1D, line = L - d * N1D;
+        const int fe = e * NFP;
+        const int f0 = fe + (d == 0 ? line : 2 * N1D + line);
+        const int f1 = fe + (d == 0 ? N1D + line : 3 * N1D + line);
+        const int base = e * NFB + L * (N1D + 1);
+        const float* PHc = sm.x.a.PH[c] + e * NPV;
+        const float* PLc = sm.x.a.PL[c] + e * NPL + (d * N1D + line) * LW;
+        const uint16_t(*G)[N1D - 1] = d == 0 ? T.gx : T.gy;
+        double fh = (double)sm.BF[c][f0], fl = (double)sm.BFL[c][f0];
+        sm.fL[c][base] = (float)fl;
+        sm.df[c][base] = (float)(fh - fl);
+#pragma unroll
+        for (int s = 0; s < N1D - 1; s++) {   // the far node's increment is -BF of the far face
+            const int q = T.lq[L][s];
+            float Q = 0.f;
+#pragma unroll
+            for (int m = 0; m < N1D - 1; m++) {
+                const uint32_t g = G[q][m];
+                Q = fmaf((g & 0x8000u) ? -1.f : 1.f, PHc[g & 0x7fffu], Q);
+            }
+            float h = T.a_v[q] * Q;
+            float l = 0.f;
+            if (s > 0) l -= PLc[s - 1];
+            l += PLc[s];
+            if (s == 0) {   // the near face
+                h = fmaf(d == 0 ? T.a_fx[q] : T.a_fy[q], sm.BF[c][f0], h);
+                l -= sm.BFL[c][f0];
+            }
+            fh = fma(-T.wq[q], (double)h, fh);
+            fl += (double)l;
+            sm.fL[c][base + s + 1] = (float)fl;
+            sm.df[c][base + s + 1] = (float)(fh - fl);
+        }
+        // far end: -BF exactly (FP32 values), so interfaces conserve
+        sm.fL[c][base + N1D] = -sm.BFL[c][f1];
+        sm.df[c][base + N1D] = (float)((double)sm.BFL[c][f1] - (double)sm.BF[c][f1]);
+    }
+    // debug outputs rhsH, rhsL per node (the sums of both directions, as rhs! stores them)
+#pragma unroll 1
+    for (int n = a.dbg_rhsH ? tid : mN; n < mN; n += NT) {
         const int e = n / NQ, i = n - e * NQ;
         const int ia = i % N1D, ib = i / N1D;
         const int pv = e * NPV, pl = e * NPL, fe = e * NFP;
@@ -395,61 +462,17 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
             }
         }
         const float nJ = -(float)sm.geo[e][3];
-        const float iwJ = (float)(sm.geo[e][3] / T.wq[i]);
+        const float iwJ = (float)(sm.geo[e][3] * rcp64(T.wq[i]));
+        const size_t o = ((size_t)I.node_off + (size_t)k0 * NQ + n) * 4;
         for (int c = 0; c < 4; c++) {
-            sm.RH[0][c][n] = hx[c] * nJ;
-            sm.RH[1][c][n] = hy[c] * nJ;
-            sm.RL[0][c][n] = lx[c] * iwJ;
-            sm.RL[1][c][n] = ly[c] * iwJ;
-        }
-        if (a.dbg_rhsH) {
-            const size_t o = ((size_t)I.node_off + (size_t)k0 * NQ + n) * 4;
-            for (int c = 0; c < 4; c++) {
-                a.dbg_rhsH[o + c] = (double)(hx[c] * nJ) + (double)(hy[c] * nJ);
-                a.dbg_rhsL[o + c] = (double)(lx[c] * iwJ) + (double)(ly[c] * iwJ);
-            }
+            a.dbg_rhsH[o + c] = (double)(hx[c] * nJ) + (double)(hy[c] * nJ);
+            a.dbg_rhsL[o + c] = (double)(lx[c] * iwJ) + (double)(ly[c] * iwJ);
         }
     }
-    __syncthreads();
 
-    // ---- 5a: subcell flux potentials in FP64 (accumulate_f_bar!, subcell.jl:163-206)
-#pragma unroll 1
-    for (int w = tid; w < nel * 2 * N1D; w += NT) {
-        const int e = w / (2 * N1D), L = w - e * (2 * N1D);
-        const int d = L / N1D, line = L - d * N1D;
-        const double J = sm.geo[e][0];
-        const int f0 = e * NFP + (d == 0 ? line : 2 * N1D + line);
-        const int f1 = e * NFP + (d == 0 ? N1D + line : 3 * N1D + line);
-        const int base = e * NFB + L * (N1D + 1);
-        double fh[4], fl[4];
-        for (int c = 0; c < 4; c++) {
-            fh[c] = (double)sm.BF[c][f0];
-            fl[c] = (double)sm.BFL[c][f0];
-            sm.x.b.fL[c][base] = (float)fl[c];
-            sm.x.b.df[c][base] = (float)(fh[c] - fl[c]);
-        }
-        for (int s = 1; s < N1D; s++) {
-            const int q = T.lq[L][s - 1];
-            const int n = e * NQ + q;
-            const double wJ = T.wq[q] * J;
-            for (int c = 0; c < 4; c++) {
-                fh[c] = fma(wJ, (double)sm.RH[d][c][n], fh[c]);
-                fl[c] = fma(wJ, (double)sm.RL[d][c][n], fl[c]);
-                sm.x.b.fL[c][base + s] = (float)fl[c];
-                sm.x.b.df[c][base + s] = (float)(fh[c] - fl[c]);
-            }
-        }
-        for (int c = 0; c < 4; c++) {   // far end: -BF exactly (FP32 values), so interfaces conserve
-            sm.x.b.fL[c][base + N1D] = -sm.BFL[c][f1];
-            sm.x.b.df[c][base + N1D] = (float)((double)sm.BFL[c][f1] - (double)sm.BF[c][f1]);
-        }
-    }
     __syncthreads();
 
     // ---- 5u: u_L from the low-order potentials and the bounds (subcell.jl:269, 370-387)
-    const double dt = bits_to_d(sc->dt_bits);
-    const double dt_lim = (a.mode == MODE_STAGE2 || a.mode == MODE_STAGE3) ? dt : sc->dt_lim;
-    const double zeta = I.zeta;
 #pragma unroll 1
     for (int n = tid; n < mN; n += NT) {
         const int e = n / NQ, i = n - e * NQ;
@@ -458,8 +481,8 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
         sm.x.b.iwJ[n] = iwJ;
         double uL[4];
         for (int c = 0; c < 4; c++) {
-            const double r = ((double)sm.x.b.fL[c][rx + 1] - (double)sm.x.b.fL[c][rx]) +
-                             ((double)sm.x.b.fL[c][ry + 1] - (double)sm.x.b.fL[c][ry]);
+            const double r = ((double)sm.fL[c][rx + 1] - (double)sm.fL[c][rx]) +
+                             ((double)sm.fL[c][ry + 1] - (double)sm.fL[c][ry]);
             uL[c] = sm.U[c][n] + dt_lim * (r * iwJ);
             sm.x.b.UL[c][n] = uL[c];
         }
@@ -490,7 +513,7 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
         const int side = (int)(pk >> 16);
         const int r = e * NFB + (int)(pk & 255u), n = e * NQ + (int)((pk >> 8) & 255u);
         double dfr[4];
-        for (int c = 0; c < 4; c++) dfr[c] = (double)sm.x.b.df[c][r];
+        for (int c = 0; c < 4; c++) dfr[c] = (double)sm.df[c][r];
         double l = 1.0;
         if (dfr[0] != 0.0 || dfr[1] != 0.0 || dfr[2] != 0.0 || dfr[3] != 0.0) {
             const double sc_ = (side ? fac_dt : -fac_dt) * sm.x.b.iwJ[n];
@@ -521,10 +544,10 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
         const double ly1 = fminf(sm.x.b.ls[0][ry + 1], sm.x.b.ls[1][ry + 1]);
         double r[4];
         for (int c = 0; c < 4; c++) {
-            const double gx0 = fma(lx0, (double)sm.x.b.df[c][rx], (double)sm.x.b.fL[c][rx]);
-            const double gx1 = fma(lx1, (double)sm.x.b.df[c][rx + 1], (double)sm.x.b.fL[c][rx + 1]);
-            const double gy0 = fma(ly0, (double)sm.x.b.df[c][ry], (double)sm.x.b.fL[c][ry]);
-            const double gy1 = fma(ly1, (double)sm.x.b.df[c][ry + 1], (double)sm.x.b.fL[c][ry + 1]);
+            const double gx0 = fma(lx0, (double)sm.df[c][rx], (double)sm.fL[c][rx]);
+            const double gx1 = fma(lx1, (double)sm.df[c][rx + 1], (double)sm.fL[c][rx + 1]);
+            const double gy0 = fma(ly0, (double)sm.df[c][ry], (double)sm.fL[c][ry]);
+            const double gy1 = fma(ly1, (double)sm.df[c][ry + 1], (double)sm.fL[c][ry + 1]);
             r[c] = ((gx1 - gx0) + (gy1 - gy0)) * iwJ;
         }
         const size_t o = ob + (size_t)n * 4;
@@ -543,7 +566,8 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
             for (int c = 0; c < 4; c++) un[c] = fma(wa, Wc[c], wb * fma(dt, r[c], sm.U[c][n]));
         }
         *(double4*)(a.Uout + o) = make_double4(un[0], un[1], un[2], un[3]);
-        const double pr = (gd - 1.0) * (un[3] - 0.5 * (un[1] * un[1] + un[2] * un[2]) / un[0]);
+        // the sign of the pressure: a reciprocal within an ulp is enough
+        const double pr = (gd - 1.0) * (un[3] - 0.5 * (un[1] * un[1] + un[2] * un[2]) * rcp64(un[0]));
         if (!(un[0] > 0.0) || !(pr > 0.0)) {
             atomicAdd(&sc->viol, 1ull);
             atomicMin(&sc->first_bad, ((unsigned long long)a.mode << 40) | (unsigned long long)((size_t)k0 * NQ + n));
@@ -603,7 +627,7 @@ __device__ __forceinline__ void dt_tile(const MixArgs& a, const MixInst& I, cons
             const float Bf = (m == 0 ? rx : sy) * T.Bf[f];
             lam += 0.5f * fabsf(Bf) * fmaxf(m == 0 ? wx : wy, m == 0 ? PP.wsx : PP.wsy);
         }
-        const double dti = I.CFL * 0.5 * T.wq[i] * sm.geo[e][0] / (double)lam;
+        const double dti = I.CFL * 0.5 * T.wq[i] * sm.geo[e][0] * rcp64((double)lam);
         dtmin = dti < dtmin ? dti : dtmin;
     }
     for (int off = 32; off > 0; off >>= 1) {
