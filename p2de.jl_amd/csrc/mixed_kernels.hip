@@ -164,11 +164,14 @@ struct SmemM {
     double geo[E][4];            // J, rxJ, syJ, 1/J
     float BF[4][MF];             // B*f* - LF (high order == low order at interior faces)
     float BFL[4][MF];            // low-order surface term (differs at inflow faces)
-    // subcell flux potentials, accumulated in FP64 and stored once rounded: the limiter (5b)
-    // and the limited update (5c) read the same values, so the update is the convex
-    // combination the limiter bounded
-    float fL[4][MB];             // low-order potentials
-    float df[4][MB];             // f_bar_H - f_bar_L
+    // subcell flux potentials, accumulated in FP64 and stored once rounded (fd: f_bar_L and
+    // f_bar_H - f_bar_L per face and component); the limiter (5b) replaces each face's pair by
+    // the limited flux f_bar_L + l (f_bar_H - f_bar_L) in FP64, in place (g), from the same
+    // values it bounded, so the update (5c) is the convex combination the limiter bounded
+    union {
+        float2 fd[4][MB];
+        double g[4][MB];
+    } p;
     union {
         struct {                 // phases 1-4
             float np[8][M];      // NodeP fields
@@ -179,7 +182,6 @@ struct SmemM {
             double UL[4][M];
             double Lrhoe[M];
             double iwJ[M];       // 1 / (wq J)
-            float ls[2][MB];     // limiting parameter seen from the node above/right (0) and below/left (1)
         } b;
     } x;
 };
@@ -383,8 +385,7 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
         const float* PLc = sm.x.a.PL[c] + e * NPL + (d * N1D + line) * LW;
         const uint16_t(*G)[N1D - 1] = d == 0 ? T.gx : T.gy;
         double fh = (double)sm.BF[c][f0], fl = (double)sm.BFL[c][f0];
-        sm.fL[c][base] = (float)fl;
-        sm.df[c][base] = (float)(fh - fl);
+        sm.p.fd[c][base] = make_float2((float)fl, (float)(fh - fl));
 #pragma unroll
         for (int s = 0; s < N1D - 1; s++) {   // the far node's increment is -BF of the far face
             const int q = T.lq[L][s];
@@ -404,12 +405,10 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
             }
             fh = fma(-T.wq[q], (double)h, fh);
             fl += (double)l;
-            sm.fL[c][base + s + 1] = (float)fl;
-            sm.df[c][base + s + 1] = (float)(fh - fl);
+            sm.p.fd[c][base + s + 1] = make_float2((float)fl, (float)(fh - fl));
         }
         // far end: -BF exactly (FP32 values), so interfaces conserve
-        sm.fL[c][base + N1D] = -sm.BFL[c][f1];
-        sm.df[c][base + N1D] = (float)((double)sm.BFL[c][f1] - (double)sm.BF[c][f1]);
+        sm.p.fd[c][base + N1D] = make_float2(-sm.BFL[c][f1], (float)((double)sm.BFL[c][f1] - (double)sm.BF[c][f1]));
     }
     // debug outputs rhsH, rhsL per node (the sums of both directions, as rhs! stores them)
 #pragma unroll 1
@@ -481,8 +480,8 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
         sm.x.b.iwJ[n] = iwJ;
         double uL[4];
         for (int c = 0; c < 4; c++) {
-            const double r = ((double)sm.fL[c][rx + 1] - (double)sm.fL[c][rx]) +
-                             ((double)sm.fL[c][ry + 1] - (double)sm.fL[c][ry]);
+            const double r = ((double)sm.p.fd[c][rx + 1].x - (double)sm.p.fd[c][rx].x) +
+                             ((double)sm.p.fd[c][ry + 1].x - (double)sm.p.fd[c][ry].x);
             uL[c] = sm.U[c][n] + dt_lim * (r * iwJ);
             sm.x.b.UL[c][n] = uL[c];
         }
@@ -490,43 +489,63 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
     }
     __syncthreads();
 
-    // ---- 5b: limiter parameter of every subcell face (subcell_bound_limiter!, subcell.jl:248-349),
-    //      one work item per (face, adjacent node): the faces inside elements first, then the
-    //      element-boundary faces, whose f_bar_H - f_bar_L is zero except at inflow boundaries (a
-    //      zero difference makes the parameter irrelevant, so those items finish at once)
+    // ---- 5b: limiter parameter of every subcell face (subcell_bound_limiter!, subcell.jl:248-349):
+    //      one work item per face, the minimum of the parameters of its two adjacent nodes (side 0:
+    //      the node at position s of the line, above/right of face s, P = -4 dt df / (wq J); side
+    //      1: the node at position s - 1, P = +4 dt df / (wq J)).  The faces inside elements
+    //      come first, then the element-boundary faces, which have one node in the element and
+    //      whose f_bar_H - f_bar_L is zero except at inflow boundaries (a zero difference makes
+    //      the parameter irrelevant: 1).  The side whose density increment is not negative is
+    //      evaluated first, and the other is skipped once the minimum is 0.
     const double fac_dt = 4.0 * dt_lim;
-    constexpr int NI = MixTab<N1D>::NI, NBS = MixTab<N1D>::NBS;
+    constexpr int NFI = 2 * N1D * (N1D - 1);   // inner subcell faces per element
+    constexpr int NFO = 4 * N1D;               // element-boundary subcell faces per element
 #pragma unroll 1
-    for (int w = tid; w < nel * (NI + NBS); w += NT) {
-        int e;
-        uint32_t pk;
-        if (w < nel * NI) {
-            e = w / NI;
-            pk = T.sI[w - e * NI];
+    for (int w = tid; w < nel * (NFI + NFO); w += NT) {
+        int e, L, f;
+        if (w < nel * NFI) {
+            e = w / NFI;
+            const int q = w - e * NFI;
+            L = q / (N1D - 1);
+            f = q - L * (N1D - 1) + 1;
         } else {
-            const int jb = w - nel * NI;
-            e = jb / NBS;
-            pk = T.sB[jb - e * NBS];
+            const int jb = w - nel * NFI;
+            e = jb / NFO;
+            const int q = jb - e * NFO;
+            L = q >> 1;
+            f = (q & 1) ? N1D : 0;
         }
-        // side 0: the node at position s (above/right of face s), P = -4 dt df / (wq J);
-        // side 1: the node at position s - 1 (below/left), P = +4 dt df / (wq J)
-        const int side = (int)(pk >> 16);
-        const int r = e * NFB + (int)(pk & 255u), n = e * NQ + (int)((pk >> 8) & 255u);
-        double dfr[4];
-        for (int c = 0; c < 4; c++) dfr[c] = (double)sm.df[c][r];
+        const int r = e * NFB + L * (N1D + 1) + f;
+        double fLr[4], dfr[4];
+        for (int c = 0; c < 4; c++) {
+            const float2 v = sm.p.fd[c][r];
+            fLr[c] = (double)v.x;
+            dfr[c] = (double)v.y;
+        }
         double l = 1.0;
         if (dfr[0] != 0.0 || dfr[1] != 0.0 || dfr[2] != 0.0 || dfr[3] != 0.0) {
-            const double sc_ = (side ? fac_dt : -fac_dt) * sm.x.b.iwJ[n];
-            double P[4], uL[4];
-            for (int c = 0; c < 4; c++) {
-                P[c] = sc_ * dfr[c];
-                uL[c] = sm.x.b.UL[c][n];
+            // P0 of side 0 is -df0 times a positive factor
+            const int first = dfr[0] > 0.0 ? 1 : 0;
+#pragma unroll 1
+            for (int k = 0; k < 2; k++) {
+                const int side = k == 0 ? first : 1 - first;
+                const int pos = f - side;            // the node's position on the line
+                if (pos < 0 || pos >= N1D) continue;  // boundary face: no node on that side
+                const int n = e * NQ + T.lq[L][pos];
+                const double sc_ = (side ? fac_dt : -fac_dt) * sm.x.b.iwJ[n];
+                double P[4], uL[4];
+                for (int c = 0; c < 4; c++) {
+                    P[c] = sc_ * dfr[c];
+                    uL[c] = sm.x.b.UL[c][n];
+                }
+                l = fmin(l, lim_param(uL, zeta * uL[0], sm.x.b.Lrhoe[n], P, I.ZEROTOL, I.upper_mode));
+                if (!(l > 0.0)) break;
             }
-            l = lim_param(uL, zeta * uL[0], sm.x.b.Lrhoe[n], P, I.ZEROTOL, I.upper_mode);
         }
-        // rounded toward zero: a smaller parameter stays admissible; a boundary face has one side
-        sm.x.b.ls[side][r] = __double2float_rz(l);
-        if (w >= nel * NI) sm.x.b.ls[1 - side][r] = 1.0f;
+        // rounded toward zero (a smaller parameter stays admissible); the limited flux of the
+        // face replaces its potentials
+        const double lr = (double)__double2float_rz(l);
+        for (int c = 0; c < 4; c++) sm.p.g[c][r] = fma(lr, dfr[c], fLr[c]);
     }
     __syncthreads();
 
@@ -538,16 +557,10 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
         const int e = n / NQ, i = n - e * NQ;
         const int rx = e * NFB + (T.nfc[i] & 255), ry = e * NFB + (T.nfc[i] >> 8);
         const double iwJ = sm.x.b.iwJ[n];
-        const double lx0 = fminf(sm.x.b.ls[0][rx], sm.x.b.ls[1][rx]);
-        const double lx1 = fminf(sm.x.b.ls[0][rx + 1], sm.x.b.ls[1][rx + 1]);
-        const double ly0 = fminf(sm.x.b.ls[0][ry], sm.x.b.ls[1][ry]);
-        const double ly1 = fminf(sm.x.b.ls[0][ry + 1], sm.x.b.ls[1][ry + 1]);
         double r[4];
         for (int c = 0; c < 4; c++) {
-            const double gx0 = fma(lx0, (double)sm.df[c][rx], (double)sm.fL[c][rx]);
-            const double gx1 = fma(lx1, (double)sm.df[c][rx + 1], (double)sm.fL[c][rx + 1]);
-            const double gy0 = fma(ly0, (double)sm.df[c][ry], (double)sm.fL[c][ry]);
-            const double gy1 = fma(ly1, (double)sm.df[c][ry + 1], (double)sm.fL[c][ry + 1]);
+            const double gx0 = sm.p.g[c][rx], gx1 = sm.p.g[c][rx + 1];
+            const double gy0 = sm.p.g[c][ry], gy1 = sm.p.g[c][ry + 1];
             r[c] = ((gx1 - gx0) + (gy1 - gy0)) * iwJ;
         }
         const size_t o = ob + (size_t)n * 4;

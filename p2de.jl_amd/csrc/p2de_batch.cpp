@@ -242,16 +242,6 @@ int build_mixtab(const ElemTables& T, const p2de_operators* ops, MixTab<N1D>& M,
         }
         M.nfc[i] = (uint16_t)((ib * (N1D + 1) + ia) | (((N1D + ia) * (N1D + 1) + ib) << 8));
     }
-    // limiter items: side 0 = the node at position s (above/right of face s), side 1 = position s - 1
-    int ni = 0, nb = 0;
-    for (int L = 0; L < 2 * N1D; L++) {
-        const int d = L / N1D, line = L % N1D;
-        for (int s = 1; s < N1D; s++)
-            for (int side = 0; side < 2; side++)
-                M.sI[ni++] = pack(L * (N1D + 1) + s, node(d, line, side ? s - 1 : s), side);
-        M.sB[nb++] = pack(L * (N1D + 1), node(d, line, 0), 0);
-        M.sB[nb++] = pack(L * (N1D + 1) + N1D, node(d, line, N1D - 1), 1);
-    }
     return P2DE_OK;
 }
 

@@ -40,8 +40,6 @@ struct MixTab {
     static constexpr int NQ = N1D * N1D, NFP = 4 * N1D, NP1 = N1D * (N1D - 1) / 2;
     static constexpr int NPV = 2 * N1D * NP1, NPL = 2 * N1D * (N1D - 1);
     static constexpr int NFB = 2 * N1D * (N1D + 1);          // subcell faces per element
-    static constexpr int NI = 4 * N1D * (N1D - 1);            // (inner subcell face, adjacent node) pairs
-    static constexpr int NBS = 4 * N1D;                       // element-boundary subcell faces
     double wq[NQ];
     float cH[NPV];        // Srh_db[i, j] (d = 0) or Ssh_db[i, j] (d = 1)
     float cL[NPL];        // Sr0[i, j] or Ss0[i, j]
@@ -52,8 +50,6 @@ struct MixTab {
     // index tables of the element (the kernels read them instead of recomputing the layout):
     uint32_t pH[NPV];     // volume pair p: i | j << 8 | d << 16
     uint32_t pL[NPL];     // low-order pair p: i | j << 8 | d << 16
-    uint32_t sI[NI];      // inner (face, node) item: subcell face r | node << 8 | side << 16
-    uint32_t sB[NBS];     // boundary face item: r | node << 8 | side << 16
     uint16_t gx[NQ][N1D - 1];   // node i's volume pairs along its r-line: p | 0x8000 if i is the pair's j
     uint16_t gy[NQ][N1D - 1];   // ... along its s-line
     uint16_t nfc[NQ];     // node i's left r-line subcell face | bottom s-line subcell face << 8
