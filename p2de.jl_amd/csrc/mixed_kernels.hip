@@ -947,6 +947,7 @@ const MixVariant kMix[] = {
     MV(128, 0, 8, 0, 0), MV(64, 0, 4, 0, 0), MV(256, 0, 16, 0, 0), MV(256, 0, 12, 0, 0),
     MV(128, 16, 0, 0, 0), MV(64, 8, 0, 0, 0),
     MV(256, 0, 12, 8, 5), MV(256, 0, 12, 8, 4), MV(256, 12, 12, 8, 5), MV(256, 12, 12, 8, 4),
+    MV(256, 0, 16, 8, 5), MV(256, 0, 10, 8, 5), MV(256, 0, 16, 8, 4), MV(256, 0, 12, 9, 5),
 };
 #undef MV
 constexpr int kNumMix = (int)(sizeof(kMix) / sizeof(kMix[0]));
