@@ -381,19 +381,21 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
         const int f0 = fe + (d == 0 ? line : 2 * N1D + line);
         const int f1 = fe + (d == 0 ? N1D + line : 3 * N1D + line);
         const int base = e * NFB + L * (N1D + 1);
-        const float* PHc = sm.x.a.PH[c] + e * NPV;
+        // the line's volume pairs: pair (a > b) of positions a, b has index a(a-1)/2 + b
+        const float* PHl = sm.x.a.PH[c] + e * NPV + (d * N1D + line) * MixTab<N1D>::NP1;
         const float* PLc = sm.x.a.PL[c] + e * NPL + (d * N1D + line) * LW;
-        const uint16_t(*G)[N1D - 1] = d == 0 ? T.gx : T.gy;
         double fh = (double)sm.BF[c][f0], fl = (double)sm.BFL[c][f0];
         sm.p.fd[c][base] = make_float2((float)fl, (float)(fh - fl));
 #pragma unroll
         for (int s = 0; s < N1D - 1; s++) {   // the far node's increment is -BF of the far face
             const int q = T.lq[L][s];
+            // node s adds the pairs it is the upper node of and subtracts the others (S is
+            // skew-symmetric, fS symmetric), over the other positions in increasing order
             float Q = 0.f;
 #pragma unroll
-            for (int m = 0; m < N1D - 1; m++) {
-                const uint32_t g = G[q][m];
-                Q = fmaf((g & 0x8000u) ? -1.f : 1.f, PHc[g & 0x7fffu], Q);
+            for (int o = 0; o < N1D; o++) {
+                if (o < s) Q = Q + PHl[s * (s - 1) / 2 + o];
+                if (o > s) Q = Q - PHl[o * (o - 1) / 2 + s];
             }
             float h = T.a_v[q] * Q;
             float l = 0.f;

@@ -161,6 +161,49 @@ def test_batch_positivity_conservation_and_independence():
             assert np.array_equal(bs.get_state(i), res[i])
 
 
+def test_batch_steps_at_once_and_one_at_a_time_agree():
+    """Steps enqueued all at once (one start launch, then the step graph replayed, each step's
+    end starting the next) and one at a time give the same states and times bit for bit, and a
+    state written between enqueues is the one whose time step is taken."""
+    from p2de_jl_amd.solver import BatchSolver
+    inst = _instances()
+    fps = [fp for fp, _ in inst]
+    steps = 6
+    once, single = BatchSolver(fps), BatchSolver(fps)
+    for bs in (once, single):
+        for i, (fp, U) in enumerate(inst):
+            bs.set_state(i, U)
+        bs.set_time(0.0)
+    once.enqueue_steps(steps)
+    for k in range(steps):
+        single.enqueue_steps(1)
+    once.synchronize()
+    single.synchronize()
+    for i in range(len(inst)):
+        ta, dta, na = once.get_time(i)
+        tb, dtb, nb = single.get_time(i)
+        print(f"  instance {i}: t {ta!r} vs {tb!r}, last dt {dta!r} vs {dtb!r}", flush=True)
+        assert (ta, dta, na) == (tb, dtb, nb) and na == steps
+        assert np.array_equal(once.get_state(i), single.get_state(i))
+    # a new state between enqueues: its own wave speeds set the next dt
+    fresh = BatchSolver(fps)
+    for i, (fp, U) in enumerate(inst):
+        U2 = U.copy()
+        U2[..., 3] *= 1.5            # hotter gas: faster waves, a smaller dt
+        once.set_state(i, U2)
+        fresh.set_state(i, U2)
+    t0 = [once.get_time(i)[0] for i in range(len(inst))]
+    for i in range(len(inst)):
+        fresh.set_time(t0[i], i)
+    once.enqueue_steps(1)
+    fresh.enqueue_steps(1)
+    for i in range(len(inst)):
+        t1, dt1, _ = once.get_time(i)
+        t2, dt2, _ = fresh.get_time(i)
+        assert dt1 == dt2 and t1 == t2
+        assert np.array_equal(once.get_state(i), fresh.get_state(i))
+
+
 def test_mixed_trajectory_against_fp64():
     """20 SSP-RK3 steps of the vortex on a 16x16 mesh: mixed against the FP64 drop-in."""
     from p2de_jl_amd.solver import BatchSolver, DeviceSolver
