@@ -950,6 +950,9 @@ const MixVariant kMix[] = {
     MV(128, 16, 0, 0, 0), MV(64, 8, 0, 0, 0),
     MV(256, 0, 12, 8, 5), MV(256, 0, 12, 8, 4), MV(256, 12, 12, 8, 5), MV(256, 12, 12, 8, 4),
     MV(256, 0, 16, 8, 5), MV(256, 0, 10, 8, 5), MV(256, 0, 16, 8, 4), MV(256, 0, 12, 9, 5),
+    // 512 threads, two workgroups per CU (up to 76 KB of LDS each): twice the elements per tile
+    MV(512, 0, 32, 16, 10), MV(512, 0, 32, 16, 11),
+    MV(512, 0, 0, 16, 0), MV(512, 0, 32, 0, 0), MV(512, 0, 0, 0, 10),
 };
 #undef MV
 constexpr int kNumMix = (int)(sizeof(kMix) / sizeof(kMix[0]));
