@@ -288,13 +288,14 @@ int build_tiles(p2de_batch* b) {
         b->hinst[i].tile_hi = (int)tiles.size();
     }
     b->ntiles = (int)tiles.size();
-    // XCD chunks of about equal estimated cost: DOFs, with the degree-4 tiles weighted by the
-    // per-DOF time the single-degree kernels measure (about 1.3x that of degrees 1-3)
+    // XCD chunks of about equal cost: DOFs (the fused stage kernel takes about the same time per
+    // DOF at degrees 2, 3 and 4; on the 48-instance batch, weighting the degree-4 tiles by 1.3
+    // made a step 4% slower, and any weight other than 1 was slower)
     {
         std::vector<double> cost(tiles.size() + 1, 0.0);
         for (size_t t = 0; t < tiles.size(); t++) {
             const int n1d = tiles[t].n1d;
-            cost[t + 1] = cost[t] + (double)tiles[t].nel * n1d * n1d * (n1d == 5 ? 1.3 : 1.0);
+            cost[t + 1] = cost[t] + (double)tiles[t].nel * n1d * n1d;
         }
         b->chunk[0] = 0;
         size_t t = 0;
