@@ -100,13 +100,23 @@ __device__ __forceinline__ float logmean_f(float a, float b) {
     return d * rcpf(__logf(b * rcpf(a)));
 }
 
-// Chandrashekar flux fS (compressible_Navier_Stokes.jl:196-249), components of direction d
-__device__ __forceinline__ void fS_f(float g2m1, const NodeP& L, const NodeP& R, int d, float F[4]) {
-    const float rl = logmean_f(L.rho, R.rho), bl = logmean_f(L.beta, R.beta);
+// 1 / logmean(a, b), the same two forms: 2 (1 + f^2/3 + f^4/5 + f^6/7) / (a+b), ln(b/a) / (b-a)
+__device__ __forceinline__ float inv_logmean_f(float a, float b) {
+    const float rs = rcpf(a + b), d = b - a;
+    const float f = d * rs;
+    const float u = f * f;
+    if (u < 1e-2f) return 2.f * rs * fmaf(u, fmaf(u, fmaf(u, 1.f / 7.f, 0.2f), 1.f / 3.f), 1.f);
+    return __logf(b * rcpf(a)) * rcpf(d);
+}
+
+// Chandrashekar flux fS (compressible_Navier_Stokes.jl:196-249), components of direction d;
+// ig2m1 = 1 / (2 (gamma - 1)).  The beta log mean enters only as its reciprocal.
+__device__ __forceinline__ void fS_f(float ig2m1, const NodeP& L, const NodeP& R, int d, float F[4]) {
+    const float rl = logmean_f(L.rho, R.rho), ibl = inv_logmean_f(L.beta, R.beta);
     const float ra = 0.5f * (L.rho + R.rho), ua = 0.5f * (L.u + R.u), va = 0.5f * (L.v + R.v);
     const float un = fmaf(L.u, R.u, L.v * R.v);
     const float pa = ra * rcpf(L.beta + R.beta);
-    const float f4 = fmaf(0.5f * rl, un, fmaf(rl, rcpf(g2m1 * bl), pa));
+    const float f4 = fmaf(0.5f * rl, un, fmaf(rl, ibl * ig2m1, pa));
     const float vn = d == 0 ? ua : va;
     const float m = rl * vn;
     F[0] = m;
@@ -231,7 +241,7 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
     constexpr int LW = N1D - 1;
     const int tid = threadIdx.x;
     const double gd = I.gamma;
-    const float g2m1 = (float)(2.0 * (gd - 1.0));
+    const float ig2m1 = (float)(0.5 / (gd - 1.0));
     const double* Uin = a.Uin + I.node_off * 4;
     StepScalars* sc = I.sc;
     const int mN = nel * NQ;      // valid nodes of the tile
@@ -340,7 +350,7 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
         const int i = e * NQ + (int)(pk & 255u), j = e * NQ + (int)((pk >> 8) & 255u);
         const float Sd = (float)sm.geo[e][1 + d] * T.cH[p];
         float F[4];
-        fS_f(g2m1, ld_np(sm, i), ld_np(sm, j), d, F);
+        fS_f(ig2m1, ld_np(sm, i), ld_np(sm, j), d, F);
         for (int c = 0; c < 4; c++) sm.x.a.PH[c][w] = Sd * F[c];
     }
     // ---- 3b: low-order line pairs (accumulate_low_order_rhs_volume!, :139-173)
