@@ -292,7 +292,10 @@ int p2de_batch_get_kernel_timing(p2de_batch *b, double ms[3], int64_t count[3]);
 int p2de_batch_set_graph(p2de_batch *b, int on);                     /* default on */
 /* Launch shapes (threads NT, elements per tile E[N+1] for each degree) that cover the
  * batch's degrees; p2de_batch_autotune times each on `steps` steps (state and times are
- * restored) and keeps the fastest, caching the choice in cache_path when not NULL. */
+ * restored) and keeps the fastest, caching the choice in cache_path when not NULL.  The
+ * cache is a text file of "key value" lines shared with p2de_autotune; a batch line holds the
+ * shape itself (NT:E2,E3,E4,E5, keyed by the elements per degree), so a build whose list of
+ * shapes differs times again rather than taking another shape. */
 int p2de_batch_num_variants(const p2de_batch *b);
 int p2de_batch_set_variant(p2de_batch *b, int v);
 int p2de_batch_get_variant(const p2de_batch *b, int *v, int *NT, int *E /* [6] */);

@@ -1,10 +1,11 @@
 // mixed_kernels.hip -- mixed-precision (FP32 bulk, FP64 limiter and RK) fused stage kernels of
 // the limited ESDG scheme on 2D LGL quads, for gfx950.
 //
-// Per SSP-RK3 step: init (per instance: done flag, dt cap), the CFL pass and its reduction
-// (global min of the low-order time step per instance, low_order_graph_viscosity.jl:222-243),
-// three fused stage launches (low- and high-order RHS, subcell limiter, RK stage, positivity
-// check) and a post kernel (t += dt).  Each launch is a persistent workgroup pool: the tile
+// Per SSP-RK3 step: the CFL pass and its reduction (global min of the low-order time step per
+// instance, low_order_graph_viscosity.jl:222-243), three fused stage launches (low- and
+// high-order RHS, subcell limiter, RK stage, positivity check) and a post kernel (t += dt, then
+// the next step's done flag and dt cap; one init launch starts the first step of a sequence).
+// Each launch is a persistent workgroup pool: the tile
 // list (E_N elements of one instance per tile) is cut into 8 contiguous chunks, one per XCD,
 // and the workgroups of an XCD (blockIdx % 8) stride through its chunk, so neighbouring
 // elements share an L2.  No global atomics sit on the tile path: device-scope atomics to one
@@ -22,7 +23,8 @@
 //    MinvVhT[i,Nq+f] == MinvVhT[i,i] (checked on the host), so those pairs are not formed;
 //  * f_bar on the far end of a line is -BF of the far face (telescoping), which is stored
 //    exactly so that the element-interface fluxes are conservative to FP64 roundoff.
-// The limited update is assembled from the same FP64 flux potentials the limiter saw, so
+// The limited update is assembled from the same FP64 flux potentials the limiter saw (the
+// limited flux of each subcell face is formed once, in FP64, from them and the parameter), so
 // rho >= zeta*rho_L and rhoe >= zeta*rhoe_L hold to FP64 roundoff exactly as in the FP64 path.
 #include <hip/hip_runtime.h>
 

@@ -445,6 +445,29 @@ double h_bits_to_double(unsigned long long u) {
     return d;
 }
 
+// Autotuner cache: one "key value" line per problem; the batch stores the chosen launch shape
+// (NT:E2,E3,E4,E5), not its index, so a build with another list of shapes re-tunes instead of
+// taking a different shape.  Lines written by the FP64 contexts are kept as they are.
+std::string shape_str(int v) {
+    int NT = 0, E[MIX_MAX_N1D + 1] = {0};
+    mix_variant_shape(v, &NT, E);
+    std::ostringstream o;
+    o << NT << ':' << E[2] << ',' << E[3] << ',' << E[4] << ',' << E[5];
+    return o.str();
+}
+
+std::map<std::string, std::string> read_tune_cache(const char* path) {
+    std::map<std::string, std::string> m;
+    std::ifstream f(path);
+    std::string line;
+    while (std::getline(f, line)) {
+        std::istringstream ls(line);
+        std::string k, v;
+        if (ls >> k >> v) m[k] = v;
+    }
+    return m;
+}
+
 std::string tune_key(const p2de_batch* b) {
     std::ostringstream k;
     k << "mixed";
@@ -872,14 +895,14 @@ int p2de_batch_autotune(p2de_batch* b, int steps, const char* cache_path, int* c
     const std::string key = tune_key(b);
     const int nv = (int)b->variants.size();
     if (cache_path) {
-        std::ifstream f(cache_path);
-        std::string k;
-        int v;
-        while (f >> k >> v)
-            if (k == key && v >= 0 && v < nv) {
-                if (chosen) *chosen = v;
-                return p2de_batch_set_variant(b, v);
-            }
+        const auto cache = read_tune_cache(cache_path);
+        const auto it = cache.find(key);
+        if (it != cache.end())
+            for (int v = 0; v < nv; v++)
+                if (shape_str(b->variants[v]) == it->second) {
+                    if (chosen) *chosen = v;
+                    return p2de_batch_set_variant(b, v);
+                }
     }
     const size_t nU = (size_t)b->total_nodes * 4;
     double* saved = nullptr;
@@ -926,15 +949,9 @@ int p2de_batch_autotune(p2de_batch* b, int steps, const char* cache_path, int* c
     if ((st = write_sc(b))) return st;
     (void)hipFree(saved);
     if (cache_path) {
-        std::map<std::string, int> cache;
-        {
-            std::ifstream f(cache_path);
-            std::string k;
-            int v;
-            while (f >> k >> v) cache[k] = v;
-        }
-        cache[key] = best;
-        std::ofstream f(cache_path);
+        auto cache = read_tune_cache(cache_path);
+        cache[key] = shape_str(b->variants[best]);
+        std::ofstream f(cache_path, std::ios::trunc);
         for (auto& kv : cache) f << kv.first << " " << kv.second << "\n";
     }
     if (chosen) *chosen = best;

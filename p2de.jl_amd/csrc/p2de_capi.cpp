@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstddef>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <map>
@@ -1021,20 +1022,34 @@ std::string tune_key(const p2de_ctx* c) {
 }
 
 // Tuning cache: one "key variant" pair per line.
-std::map<std::string, int> read_cache(const char* path) {
-    std::map<std::string, int> m;
+// Autotuner cache: one "key value" line per problem (the element-kernel shape index here; the
+// batched path stores its launch shape), read line by line so that either kind of line is kept.
+std::map<std::string, std::string> read_cache(const char* path) {
+    std::map<std::string, std::string> m;
     if (!path) return m;
     std::ifstream f(path);
-    std::string k;
-    int v;
-    while (f >> k >> v) m[k] = v;
+    std::string line;
+    while (std::getline(f, line)) {
+        std::istringstream ls(line);
+        std::string k, v;
+        if (ls >> k >> v) m[k] = v;
+    }
     return m;
 }
 
-void write_cache(const char* path, const std::map<std::string, int>& m) {
+void write_cache(const char* path, const std::map<std::string, std::string>& m) {
     if (!path) return;
     std::ofstream f(path, std::ios::trunc);
     for (auto& kv : m) f << kv.first << " " << kv.second << "\n";
+}
+
+// the cached shape index, or -1 when the entry is absent or not an index
+int cached_index(const std::map<std::string, std::string>& m, const std::string& key) {
+    const auto it = m.find(key);
+    if (it == m.end() || it->second.empty()) return -1;
+    char* end = nullptr;
+    const long v = std::strtol(it->second.c_str(), &end, 10);
+    return (*end == '\0' && v >= 0 && v < 1000000) ? (int)v : -1;
 }
 
 }  // namespace
@@ -1077,9 +1092,9 @@ int p2de_autotune(p2de_ctx* ctx, int steps, const char* cache_path, int* chosen)
     const std::string key = tune_key(ctx);
     auto cache = read_cache(cache_path);
     const int nv = num_variants_A(ctx->Nd, ctx->N1D, ctx->basis);
-    auto it = cache.find(key);
-    if (it != cache.end() && it->second >= 0 && it->second < nv) {
-        ctx->variantA = it->second;
+    const int cv = cached_index(cache, key);
+    if (cv >= 0 && cv < nv) {
+        ctx->variantA = cv;
         if (chosen) *chosen = ctx->variantA;
         return P2DE_OK;
     }
@@ -1125,7 +1140,7 @@ int p2de_autotune(p2de_ctx* ctx, int steps, const char* cache_path, int* chosen)
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_sc, ctx->h_sc, sizeof(StepScalars), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     (void)hipFree(saved);
-    cache[key] = best;
+    cache[key] = std::to_string(best);
     write_cache(cache_path, cache);
     if (chosen) *chosen = best;
     return P2DE_OK;

@@ -204,6 +204,38 @@ def test_batch_steps_at_once_and_one_at_a_time_agree():
         assert np.array_equal(once.get_state(i), fresh.get_state(i))
 
 
+def test_batch_autotune_cache_keeps_the_shape(tmp_path):
+    """The autotuner's cache stores the launch shape (threads, elements per tile per degree):
+    a second call reads it back; a shape this build does not have is timed again; lines of
+    another kind (an FP64 context's shape index) are kept."""
+    from p2de_jl_amd.solver import BatchSolver
+    inst = _instances()
+    bs = BatchSolver([fp for fp, _ in inst])
+    for i, (fp, U) in enumerate(inst):
+        bs.set_state(i, U)
+    bs.set_time(0.0)
+    cache = tmp_path / "tune.txt"
+    cache.write_text("fp64:other-problem 3\n")
+    v = bs.autotune(1, str(cache))
+    lines = cache.read_text().splitlines()
+    assert "fp64:other-problem 3" in lines
+    (line,) = [l for l in lines if l.startswith("mixed")]
+    key, shape = line.split()
+    _, NT, E = bs.variant()
+    assert shape == f"{NT}:{E[2]},{E[3]},{E[4]},{E[5]}"
+    assert bs.autotune(1, str(cache)) == v
+    assert bs.variant()[0] == v
+    cache.write_text(f"{key} 4096:0,1,1,1\n")
+    v2 = bs.autotune(1, str(cache))
+    assert 0 <= v2 < bs.num_variants()
+    (line2,) = [l for l in cache.read_text().splitlines() if l.startswith("mixed")]
+    assert line2.split()[1] != "4096:0,1,1,1"
+    # the state and time the tuner borrowed are restored
+    for i, (fp, U) in enumerate(inst):
+        assert np.array_equal(bs.get_state(i), U)
+        assert bs.get_time(i)[0] == 0.0
+
+
 def test_mixed_trajectory_against_fp64():
     """20 SSP-RK3 steps of the vortex on a 16x16 mesh: mixed against the FP64 drop-in."""
     from p2de_jl_amd.solver import BatchSolver, DeviceSolver
