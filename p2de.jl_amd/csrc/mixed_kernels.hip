@@ -271,6 +271,10 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
             sm.U[1][n] = v.y;
             sm.U[2][n] = v.z;
             sm.U[3][n] = v.w;
+            // ---- 1: node primitives (FP64 energy differences -> FP32), from the lane's own
+            //      node: no barrier between the two phases
+            const double U[4] = {v.x, v.y, v.z, v.w};
+            st_np(sm, n, node_prims(gd, U));
         }
         for (int w = tid; w < nel; w += NT) {
             const Geo G = I.geo[k0 + w];
@@ -279,14 +283,6 @@ __device__ __forceinline__ void stage_tile(const MixArgs& a, const MixInst& I, c
             sm.geo[w][2] = G.syJ;
             sm.geo[w][3] = 1.0 / G.J;
         }
-    }
-    __syncthreads();
-
-    // ---- 1: node primitives (FP64 energy differences -> FP32)
-#pragma unroll 1
-    for (int n = tid; n < mN; n += NT) {
-        const double U[4] = {sm.U[0][n], sm.U[1][n], sm.U[2][n], sm.U[3][n]};
-        st_np(sm, n, node_prims(gd, U));
     }
     __syncthreads();
 
